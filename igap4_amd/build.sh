@@ -1,0 +1,11 @@
+#!/bin/bash
+# Builds igap4_amd/libigap_gpu.so for gfx950 from igap4_amd/csrc.
+set -e
+HERE="$(cd "$(dirname "$0")" && pwd)"
+HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
+"$HIPCC" -O3 -std=c++17 --offload-arch=gfx950 -fPIC -shared \
+    -Xclang -target-feature -Xclang -packed-fp32-ops \
+    ${IGAP_EXTRA_FLAGS} \
+    "$HERE/csrc/igap_kernels.hip" "$HERE/csrc/igap_capi.cpp" \
+    -o "$HERE/libigap_gpu.so.tmp"
+mv "$HERE/libigap_gpu.so.tmp" "$HERE/libigap_gpu.so"

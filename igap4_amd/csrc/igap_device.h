@@ -1,0 +1,334 @@
+// igap_device.h -- device-side building blocks of the IGA assembly kernels.
+//
+// Per Gauss point: 1-D quadratic B-spline value/1st/2nd derivative with the
+// MTOL-guarded denominators of eval_Bspline (src/meshcore_iga.c:243-327), the
+// NURBS quotient rule and geometric map of Residual_iga
+// (src/assmcore_iga.c:152-208), and the pointwise physics of
+// phys/gradelast.c / phys/gradelasttime.c derived from the stored energy
+//   Psi = c3 e26(e26^2-3e25^2) + c2 rho^2 + c4 rho^4 + e0 e24^2
+//       + e345 (E12^2+E13^2+E23^2) + le^2 sum_K |dev(E_II,K)|^2
+// (phys/gradelast.c:10-48,1068) by the chain rule through the strain
+// measures E_IJ and EK_IK = dE_II/dX_K.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace igap {
+
+constexpr double MTOL = 1.e-15;   // src/meshcore_iga.c:10
+constexpr int NQ = 4;             // Gauss points per direction (src/assmcore_iga.c:113)
+
+// ---------------------------------------------------------------- params
+struct MeshK {
+    int nelem[3];
+    int na[3];        // active control points per direction (nelem + 2)
+    int nb[3];        // owned control points (periodic: nelem)
+    int periodic[3];
+    const double *knot[3];
+    const double *w;  // active numbering
+    const double *X;  // 3 * active
+    double par_periodic[9];
+    double gq[4], gw[4];   // Gauss-Legendre nodes/weights on [0,1]
+};
+
+struct PhysK {
+    double c2, c3, c4, e0, e345, le;
+    double rho, cii, dt;
+    int torder;   // 0 static, 2 transient
+    int nnode;    // 1 static, 8 transient
+    double s[8], cr[8], ct[8];   // transient rule in s (nodes, residual, tangent weights)
+};
+
+// jet slot of the symmetric second derivative (I,K): 11,12,13,22,23,33 -> 4..9
+__device__ __forceinline__ int sidx(int I, int K)
+{
+    int a = I < K ? I : K, b = I < K ? K : I;
+    return (a == 0) ? 4 + b : (a == 1 ? 6 + b : 9);
+}
+
+__host__ __device__ __forceinline__ int own1(int a, int n, int periodic) { return periodic ? (a >= n ? a - n : a) : a; }
+
+// ---------------------------------------------------------------- basis
+// value/first/second derivative of the three quadratic B-splines of the span
+// [kV[2], kV[3]] of the five-knot window kV at x
+__device__ __forceinline__ void bspline1d(const double *kV, double x, double *v, double *d1, double *d2)
+{
+    double h23 = kV[3] - kV[2] + MTOL;
+    double n11 = (kV[3] - x) / h23, n12 = (x - kV[2]) / h23;
+    double i31 = 1.0 / (kV[3] - kV[1] + MTOL), i42 = 1.0 / (kV[4] - kV[2] + MTOL);
+    // i = 0 uses N1[0] = 0, N1[1];  i = 1 uses N1[1], N1[2];  i = 2 uses N1[2], N1[3] = 0
+    v[0] = (kV[3] - x) / (kV[3] - kV[1] + MTOL) * n11;
+    v[1] = (x - kV[1]) / (kV[3] - kV[1] + MTOL) * n11 + (kV[4] - x) / (kV[4] - kV[2] + MTOL) * n12;
+    v[2] = (x - kV[2]) / (kV[4] - kV[2] + MTOL) * n12;
+    d1[0] = -2.0 * i31 * n11;
+    d1[1] = 2.0 * i31 * n11 - 2.0 * i42 * n12;
+    d1[2] = 2.0 * i42 * n12;
+    double dn11 = -1.0 / h23, dn12 = 1.0 / h23;
+    d2[0] = -2.0 * i31 * dn11;
+    d2[1] = 2.0 * i31 * dn11 - 2.0 * i42 * dn12;
+    d2[2] = 2.0 * i42 * dn12;
+}
+
+// tensor-product jet of basis b = 9i+3j+k from 1-D tables t[d][order][i]
+__device__ __forceinline__ void tensor_jet(const double (*t)[3][3], int b, double *N)
+{
+    int i = b / 9, j = (b / 3) % 3, k = b % 3;
+    double a0 = t[0][0][i], b0 = t[0][1][i], c0 = t[0][2][i];
+    double a1 = t[1][0][j], b1 = t[1][1][j], c1 = t[1][2][j];
+    double a2 = t[2][0][k], b2 = t[2][1][k], c2 = t[2][2][k];
+    N[0] = a0 * a1 * a2;
+    N[1] = b0 * a1 * a2;
+    N[2] = a0 * b1 * a2;
+    N[3] = a0 * a1 * b2;
+    N[4] = c0 * a1 * a2;
+    N[5] = b0 * b1 * a2;
+    N[6] = b0 * a1 * b2;
+    N[7] = a0 * c1 * a2;
+    N[8] = a0 * b1 * b2;
+    N[9] = a0 * a1 * c2;
+}
+
+// NURBS quotient rule (src/assmcore_iga.c:153-165): R = N w / W to 2nd order
+__device__ __forceinline__ void nurbs_param(const double *N, double wb, const double *W, double *R)
+{
+#pragma unroll
+    for (int a = 0; a < 10; a++) R[a] = N[a] * wb / W[0];
+#pragma unroll
+    for (int a = 1; a < 10; a++) R[a] -= R[0] * W[a] / W[0];
+    R[4] -= (R[1] * W[1] + R[1] * W[1]) / W[0];
+    R[5] -= (R[2] * W[1] + R[1] * W[2]) / W[0];
+    R[6] -= (R[3] * W[1] + R[1] * W[3]) / W[0];
+    R[7] -= (R[2] * W[2] + R[2] * W[2]) / W[0];
+    R[8] -= (R[3] * W[2] + R[2] * W[3]) / W[0];
+    R[9] -= (R[3] * W[3] + R[3] * W[3]) / W[0];
+}
+
+// parametric -> physical derivatives (src/assmcore_iga.c:179-208).
+// Gi = inverse of G_ij = dx_i/dxi_j (row-major), X2[3*6] = d2x_d/dxi dxi.
+__device__ __forceinline__ void phys_jet(double *R, const double *Gi, const double *X2)
+{
+    double p1 = R[1], p2 = R[2], p3 = R[3];
+    double r1 = p1 * Gi[0] + p2 * Gi[3] + p3 * Gi[6];
+    double r2 = p1 * Gi[1] + p2 * Gi[4] + p3 * Gi[7];
+    double r3 = p1 * Gi[2] + p2 * Gi[5] + p3 * Gi[8];
+    R[1] = r1; R[2] = r2; R[3] = r3;
+    double n[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) n[s] = R[4 + s] - r1 * X2[s] - r2 * X2[6 + s] - r3 * X2[12 + s];
+    // Hm = [[n0 n1 n2],[n1 n3 n4],[n2 n4 n5]];  R_ij = sum_pq Gi[p][i] Hm[p][q] Gi[q][j]
+    const int pi_[6] = {0, 0, 0, 1, 1, 2}, pj_[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        int i = pi_[s], j = pj_[s];
+        double c0 = n[0] * Gi[j] + n[1] * Gi[3 + j] + n[2] * Gi[6 + j];
+        double c1 = n[1] * Gi[j] + n[3] * Gi[3 + j] + n[4] * Gi[6 + j];
+        double c2 = n[2] * Gi[j] + n[4] * Gi[3 + j] + n[5] * Gi[6 + j];
+        R[4 + s] = Gi[i] * c0 + Gi[3 + i] * c1 + Gi[6 + i] * c2;
+    }
+}
+
+__device__ __forceinline__ double inv3(const double *G, double *Gi)
+{
+    double det = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) + G[2] * (G[3] * G[7] - G[4] * G[6]);
+    double id = 1.0 / det;
+    Gi[0] = id * (G[4] * G[8] - G[5] * G[7]);
+    Gi[1] = id * (G[2] * G[7] - G[1] * G[8]);
+    Gi[2] = id * (G[1] * G[5] - G[2] * G[4]);
+    Gi[3] = id * (G[5] * G[6] - G[3] * G[8]);
+    Gi[4] = id * (G[0] * G[8] - G[2] * G[6]);
+    Gi[5] = id * (G[2] * G[3] - G[0] * G[5]);
+    Gi[6] = id * (G[3] * G[7] - G[4] * G[6]);
+    Gi[7] = id * (G[1] * G[6] - G[0] * G[7]);
+    Gi[8] = id * (G[0] * G[4] - G[1] * G[3]);
+    return det;
+}
+
+// ---------------------------------------------------------------- physics
+// Strain-measure state at one jet: F[3k+I], H[9k+3I+K], S[3I+J] (S_II =
+// dPsi/dE_II, S_IJ = dPsi/dE_IJ / 2), G[3I+K] = dPsi/dEK_IK, N[3I+J] =
+// d2Psi/dE_II dE_JJ.
+struct State {
+    double F[9], H[27], S[9], G[9], N[9];
+};
+
+constexpr double IS2 = 0.70710678118654752440;
+constexpr double IS3 = 0.57735026918962576451;
+constexpr double IS6 = 0.40824829046386301637;
+
+__device__ __forceinline__ void phys_state(const double *u, const PhysK &P, State &st)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int I = 0; I < 3; I++) {
+            st.F[3 * k + I] = (k == I ? 1.0 : 0.0) + u[10 * k + 1 + I];
+#pragma unroll
+            for (int K = 0; K < 3; K++) st.H[9 * k + 3 * I + K] = u[10 * k + sidx(I, K)];
+        }
+    double E[9], EK[9];
+#pragma unroll
+    for (int I = 0; I < 3; I++)
+#pragma unroll
+        for (int J = 0; J < 3; J++) {
+            double s = st.F[I] * st.F[J] + st.F[3 + I] * st.F[3 + J] + st.F[6 + I] * st.F[6 + J];
+            E[3 * I + J] = 0.5 * (s - (I == J ? 1.0 : 0.0));
+        }
+#pragma unroll
+    for (int I = 0; I < 3; I++)
+#pragma unroll
+        for (int K = 0; K < 3; K++)
+            EK[3 * I + K] = st.F[I] * st.H[3 * I + K] + st.F[3 + I] * st.H[9 + 3 * I + K] + st.F[6 + I] * st.H[18 + 3 * I + K];
+    const double v24[3] = {IS3, IS3, IS3};
+    const double v25[3] = {IS2, -IS2, 0.0};
+    const double v26[3] = {IS6, IS6, -2 * IS6};
+    double e24 = (E[0] + E[4] + E[8]) * IS3;
+    double e25 = (E[0] - E[4]) * IS2;
+    double e26 = (E[0] + E[4] - 2 * E[8]) * IS6;
+    double rho2 = e25 * e25 + e26 * e26;
+    double P25 = -6 * P.c3 * e25 * e26 + 2 * P.c2 * e25 + 4 * P.c4 * rho2 * e25;
+    double P26 = 3 * P.c3 * (e26 * e26 - e25 * e25) + 2 * P.c2 * e26 + 4 * P.c4 * rho2 * e26;
+    double P2525 = -6 * P.c3 * e26 + 2 * P.c2 + 4 * P.c4 * rho2 + 8 * P.c4 * e25 * e25;
+    double P2526 = -6 * P.c3 * e25 + 8 * P.c4 * e25 * e26;
+    double P2626 = 6 * P.c3 * e26 + 2 * P.c2 + 4 * P.c4 * rho2 + 8 * P.c4 * e26 * e26;
+#pragma unroll
+    for (int I = 0; I < 3; I++) {
+        st.S[4 * I] = 2 * P.e0 * e24 * v24[I] + P25 * v25[I] + P26 * v26[I];
+#pragma unroll
+        for (int J = 0; J < 3; J++)
+            st.N[3 * I + J] = 2 * P.e0 * v24[I] * v24[J] + P2525 * v25[I] * v25[J]
+                            + P2526 * (v25[I] * v26[J] + v26[I] * v25[J]) + P2626 * v26[I] * v26[J];
+    }
+    st.S[1] = st.S[3] = P.e345 * E[1];
+    st.S[2] = st.S[6] = P.e345 * E[2];
+    st.S[5] = st.S[7] = P.e345 * E[5];
+    double le2 = P.le * P.le;
+#pragma unroll
+    for (int K = 0; K < 3; K++) {
+        double tr = (EK[K] + EK[3 + K] + EK[6 + K]) / 3.0;
+#pragma unroll
+        for (int I = 0; I < 3; I++) st.G[3 * I + K] = 2 * le2 * (EK[3 * I + K] - tr);
+    }
+}
+
+// r += s * dPsi/du (jet slots 10k + a; value slots untouched)
+__device__ __forceinline__ void phys_residual_acc(const State &st, double s, double *r)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int L = 0; L < 3; L++) {
+            double v = st.S[3 * L] * st.F[3 * k] + st.S[3 * L + 1] * st.F[3 * k + 1] + st.S[3 * L + 2] * st.F[3 * k + 2];
+            v += st.G[3 * L] * st.H[9 * k + 3 * L] + st.G[3 * L + 1] * st.H[9 * k + 3 * L + 1] + st.G[3 * L + 2] * st.H[9 * k + 3 * L + 2];
+            r[10 * k + 1 + L] += s * v;
+        }
+#pragma unroll
+        for (int I = 0; I < 3; I++)
+#pragma unroll
+            for (int K = I; K < 3; K++) {
+                double v = st.G[3 * I + K] * st.F[3 * k + I];
+                if (I != K) v += st.G[3 * K + I] * st.F[3 * k + K];
+                r[10 * k + sidx(I, K)] += s * v;
+            }
+    }
+}
+
+// affine macro-field of par_periodic (src/assmcore_iga.c:225-229): the value
+// gets a_kd * xq_d (knot-space coordinate), the first derivatives a_kd
+__device__ __forceinline__ void add_periodic(const MeshK &M, const double *xq, double *u)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        u[10 * k] += M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
+#pragma unroll
+        for (int d = 0; d < 3; d++) u[10 * k + 1 + d] += M.par_periodic[3 * k + d];
+    }
+}
+
+// gradelast_residual: r = dPsi/du at the jet u[30]
+__device__ __forceinline__ void phys_residual_static(const PhysK &P, const double *u, double *r)
+{
+#pragma unroll
+    for (int i = 0; i < 30; i++) r[i] = 0.0;
+    State st;
+    phys_state(u, P, st);
+    phys_residual_acc(st, 1.0, r);
+}
+
+// gradelasttime_residual from the midpoint jet m0 = (u0+u1)/2, the increment
+// jet d = (u-u0)/2 and the value slots uv = {u, u1, u0}[k]:
+//   value rows  cii (u-u0)/(2dt) + rho (u+u0-2u1)/dt^2     (:4128,4138,4148)
+//   other rows  sum_j cr_j dPsi(m0 + s_j d)  (the truncated series of :4129-4158)
+__device__ __forceinline__ void phys_residual_transient(const PhysK &P, const double *m0, const double *dl, const double *uv,
+                                                        double *r)
+{
+#pragma unroll
+    for (int i = 0; i < 30; i++) r[i] = 0.0;
+    for (int g = 0; g < 8; g++) {
+        double us[30];
+#pragma unroll
+        for (int i = 0; i < 30; i++) us[i] = m0[i] + P.s[g] * dl[i];
+        State st;
+        phys_state(us, P, st);
+        phys_residual_acc(st, P.cr[g], r);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        r[10 * k] = (P.cii * (uv[k] - uv[6 + k])) / (2. * P.dt) + (P.rho * (uv[k] + uv[6 + k] - 2 * uv[3 + k])) / (P.dt * P.dt);
+}
+
+// ---------------------------------------------------------------- tangent entries
+// Entries of d2Psi/du2 at one state, by block type.  M = I - 11^T/3.
+__device__ __forceinline__ double Mdev(int a, int b) { return (a == b ? 2.0 : -1.0) / 3.0; }
+
+// first-gradient (k,L) x first-gradient (l,M)
+__device__ __forceinline__ double c_ff(const State &st, const PhysK &P, int k, int L, int l, int M)
+{
+    double v = st.N[3 * L + M] * st.F[3 * k + L] * st.F[3 * l + M];
+    if (L == M) {
+        double t = 0;
+#pragma unroll
+        for (int J = 0; J < 3; J++)
+            if (J != L) t += st.F[3 * k + J] * st.F[3 * l + J];
+        v += 0.5 * P.e345 * t;
+    } else {
+        v += 0.5 * P.e345 * st.F[3 * k + M] * st.F[3 * l + L];
+    }
+    double hh = st.H[9 * k + 3 * L] * st.H[9 * l + 3 * M] + st.H[9 * k + 3 * L + 1] * st.H[9 * l + 3 * M + 1]
+              + st.H[9 * k + 3 * L + 2] * st.H[9 * l + 3 * M + 2];
+    v += 2 * P.le * P.le * Mdev(L, M) * hh;
+    if (k == l) v += st.S[3 * L + M];
+    return v;
+}
+
+// first-gradient (k,L) x second-gradient (l,(I,K))
+__device__ __forceinline__ double c_fh(const State &st, const PhysK &P, int k, int L, int l, int I, int K)
+{
+    double g2 = 2 * P.le * P.le;
+    double v = g2 * Mdev(L, I) * st.H[9 * k + 3 * L + K] * st.F[3 * l + I];
+    if (I != K) v += g2 * Mdev(L, K) * st.H[9 * k + 3 * L + I] * st.F[3 * l + K];
+    if (k == l) {
+        if (L == I) v += st.G[3 * I + K];
+        if (I != K && L == K) v += st.G[3 * K + I];
+    }
+    return v;
+}
+
+// second-gradient (k,(I,K)) x second-gradient (l,(J,K2))
+__device__ __forceinline__ double c_hh(const State &st, const PhysK &P, int k, int I, int K, int l, int J, int K2)
+{
+    double v = 0;
+    // pairs (component, gradient direction) of each slot
+    int ai0 = I, ag0 = K, ai1 = K, ag1 = I, na_ = (I != K) ? 2 : 1;
+    int bi0 = J, bg0 = K2, bi1 = K2, bg1 = J, nb_ = (J != K2) ? 2 : 1;
+    if (ag0 == bg0) v += Mdev(ai0, bi0) * st.F[3 * k + ai0] * st.F[3 * l + bi0];
+    if (nb_ == 2 && ag0 == bg1) v += Mdev(ai0, bi1) * st.F[3 * k + ai0] * st.F[3 * l + bi1];
+    if (na_ == 2 && ag1 == bg0) v += Mdev(ai1, bi0) * st.F[3 * k + ai1] * st.F[3 * l + bi0];
+    if (na_ == 2 && nb_ == 2 && ag1 == bg1) v += Mdev(ai1, bi1) * st.F[3 * k + ai1] * st.F[3 * l + bi1];
+    return 2 * P.le * P.le * v;
+}
+
+__constant__ const int kPairI[6] = {0, 0, 0, 1, 1, 2};
+__constant__ const int kPairK[6] = {0, 1, 2, 1, 2, 2};
+
+}  // namespace igap

@@ -1,0 +1,821 @@
+// igap_kernels.hip -- CDNA4 (gfx950) kernels of the IGA residual/tangent assembly.
+//
+//   k_residual_elem   per-element residual vectors R_e[81]
+//                     (volume loop of Residual_iga, src/assmcore_iga.c:118-259)
+//   k_residual_gather deterministic per-dof sum of R_e + Neumann load +
+//                     Dirichlet rows R_d = U_d - g_d (:258, :273-497)
+//   k_tangent_elem    per-element tangents K_e[81x81] on the matrix cores
+//                     (volume loop of Tangent_iga, :607-764)
+//   k_tangent_gather  deterministic per-nonzero sum of K_e into the CSR values
+//                     with MatZeroRowsColumns(diag=1) fused (:763, :844)
+//   k_csr_cols        column indices of the CSR pattern
+//
+// Element numbering e = (ex*ny + ey)*nz + ez (the reference's x->y->z loop
+// order); local basis b = 9*bx + 3*by + bz; local dof 3*b + i.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "igap_device.h"
+
+namespace igap {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// ============================================================================
+// butterfly reduce-scatter across the 64 lanes of a wave: each step halves the
+// per-lane vector (lanes with the step's bit set keep the upper half), so
+// summing N values over 64 lanes costs ~N shuffles instead of 6N.
+// ============================================================================
+template <int N>
+__device__ __forceinline__ void rs_step(const double *in, double *out, int mask, bool up)
+{
+    constexpr int N1 = (N + 1) / 2, N2 = N - N1;
+#pragma unroll
+    for (int i = 0; i < N1; i++) {
+        double hi = (i < N2) ? in[N1 + i] : 0.0;
+        double send = up ? in[i] : hi;
+        double recv = __shfl_xor(send, mask, 64);
+        out[i] = (up ? hi : in[i]) + recv;
+    }
+}
+
+__device__ __forceinline__ void rs_track(int N, bool up, int &base, int &cnt)
+{
+    int N1 = (N + 1) / 2;
+    if (up) {
+        base += N1;
+        cnt = cnt - N1 > 0 ? cnt - N1 : 0;
+    } else {
+        cnt = cnt < N1 ? cnt : N1;
+    }
+}
+
+// ============================================================================
+// residual: one wave per element, lane = Gauss point (4x4x4 = 64)
+// ============================================================================
+template <int NH>
+__global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                       double *__restrict__ Re, int nel)
+{
+    constexpr int STR = 4 + 3 * NH;   // X(3) w(1) U(3*NH) per control point
+    __shared__ double sCP[4][27 * STR];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int e = blockIdx.x * 4 + wv;
+    const bool valid = e < nel;
+    const int ee = valid ? e : nel - 1;
+    const int nz = M.nelem[2], ny = M.nelem[1];
+    const int ex = ee / (ny * nz), ey = (ee / nz) % ny, ez = ee % nz;
+    double *cp = sCP[wv];
+    if (lane < 27) {
+        int bx = lane / 9, by = (lane / 3) % 3, bz = lane % 3;
+        int ax = ex + bx, ay = ey + by, az = ez + bz;
+        int64_t id = ((int64_t)M.na[1] * M.na[2]) * ax + (int64_t)M.na[2] * ay + az;
+        cp[STR * lane + 0] = M.X[3 * id + 0];
+        cp[STR * lane + 1] = M.X[3 * id + 1];
+        cp[STR * lane + 2] = M.X[3 * id + 2];
+        cp[STR * lane + 3] = M.w[id];
+        int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ax, M.nb[0], M.periodic[0])
+                   + (int64_t)M.nb[2] * own1(ay, M.nb[1], M.periodic[1]) + own1(az, M.nb[2], M.periodic[2]);
+#pragma unroll
+        for (int h = 0; h < NH; h++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cp[STR * lane + 4 + 3 * h + k] = Uh[h * ndof + 3 * oc + k];
+    }
+    __syncthreads();
+
+    const int qd[3] = {lane >> 4, (lane >> 2) & 3, lane & 3};
+    const int ed[3] = {ex, ey, ez};
+    double t[3][3][3], xq[3], dX[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double *kV = M.knot[d] + ed[d];
+        double X0 = kV[2], X1 = kV[3], c = M.gq[qd[d]];
+        xq[d] = (-c + 1.0) * X0 + c * X1;
+        dX[d] = X1 - X0;
+        bspline1d(kV, xq[d], t[d][0], t[d][1], t[d][2]);
+    }
+    double W[10];
+#pragma unroll
+    for (int a = 0; a < 10; a++) W[a] = 0.0;
+    for (int b = 0; b < 27; b++) {
+        double N[10];
+        tensor_jet(t, b, N);
+        double wb = cp[STR * b + 3];
+#pragma unroll
+        for (int a = 0; a < 10; a++) W[a] += wb * N[a];
+    }
+    double Gi[9], X2[18], detG;
+    {
+        double Xj[30];
+#pragma unroll
+        for (int a = 0; a < 30; a++) Xj[a] = 0.0;
+        for (int b = 0; b < 27; b++) {
+            double N[10], R[10];
+            tensor_jet(t, b, N);
+            nurbs_param(N, cp[STR * b + 3], W, R);
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int a = 0; a < 10; a++) Xj[10 * d + a] += cp[STR * b + d] * R[a];
+        }
+        double G[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) G[3 * i + j] = Xj[10 * i + 1 + j];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int s = 0; s < 6; s++) X2[6 * d + s] = Xj[10 * d + 4 + s];
+        detG = inv3(G, Gi);
+    }
+
+    double rr[30];
+    if (NH == 1) {
+        double u[30];
+#pragma unroll
+        for (int a = 0; a < 30; a++) u[a] = 0.0;
+        for (int b = 0; b < 27; b++) {
+            double N[10], R[10];
+            tensor_jet(t, b, N);
+            nurbs_param(N, cp[STR * b + 3], W, R);
+            phys_jet(R, Gi, X2);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double Uv = cp[STR * b + 4 + k];
+#pragma unroll
+                for (int a = 0; a < 10; a++) u[10 * k + a] += R[a] * Uv;
+            }
+        }
+        add_periodic(M, xq, u);
+        phys_residual_static(P, u, rr);
+    } else {
+        // midpoint m0 = (u0+u1)/2 and increment d = (u-u0)/2 jets, plus the
+        // value slots of the three levels for the inertia term
+        double m0[30], dl[30], uv[9];
+#pragma unroll
+        for (int a = 0; a < 30; a++) { m0[a] = 0.0; dl[a] = 0.0; }
+#pragma unroll
+        for (int a = 0; a < 9; a++) uv[a] = 0.0;
+        for (int b = 0; b < 27; b++) {
+            double N[10], R[10];
+            tensor_jet(t, b, N);
+            nurbs_param(N, cp[STR * b + 3], W, R);
+            phys_jet(R, Gi, X2);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double U0 = cp[STR * b + 4 + k], U1 = cp[STR * b + 7 + k], U2 = cp[STR * b + 10 + k];
+                double mv = 0.5 * (U2 + U1), dv = 0.5 * (U0 - U2);
+#pragma unroll
+                for (int a = 0; a < 10; a++) { m0[10 * k + a] += R[a] * mv; dl[10 * k + a] += R[a] * dv; }
+                uv[k] += R[0] * U0; uv[3 + k] += R[0] * U1; uv[6 + k] += R[0] * U2;
+            }
+        }
+        // par_periodic adds the same affine field to every level: it enters m0 only
+        add_periodic(M, xq, m0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double pv = M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
+            uv[k] += pv; uv[3 + k] += pv; uv[6 + k] += pv;
+        }
+        phys_residual_transient(P, m0, dl, uv, rr);
+    }
+    const double wgt = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
+
+    // projection onto the 27 basis functions in three groups of nine, each
+    // group's 27 values summed over the 64 lanes by a reduce-scatter
+#pragma unroll 1
+    for (int grp = 0; grp < 3; grp++) {
+        double val[27];
+#pragma unroll
+        for (int bb = 0; bb < 9; bb++) {
+            const int b = 9 * grp + bb;
+            double N[10], R[10];
+            tensor_jet(t, b, N);
+            nurbs_param(N, cp[STR * b + 3], W, R);
+            phys_jet(R, Gi, X2);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+#pragma unroll
+                for (int a = 0; a < 10; a++) s += R[a] * rr[10 * k + a];
+                val[3 * bb + k] = wgt * s;
+            }
+        }
+        double v14[14], v7[7], v4[4], v2[2], v1a[1], v1b[1];
+        int base = 0, cnt = 27;
+        rs_step<27>(val, v14, 32, lane & 32); rs_track(27, lane & 32, base, cnt);
+        rs_step<14>(v14, v7, 16, lane & 16);  rs_track(14, lane & 16, base, cnt);
+        rs_step<7>(v7, v4, 8, lane & 8);      rs_track(7, lane & 8, base, cnt);
+        rs_step<4>(v4, v2, 4, lane & 4);      rs_track(4, lane & 4, base, cnt);
+        rs_step<2>(v2, v1a, 2, lane & 2);     rs_track(2, lane & 2, base, cnt);
+        rs_step<1>(v1a, v1b, 1, lane & 1);    rs_track(1, lane & 1, base, cnt);
+        if (valid && cnt > 0) Re[(int64_t)e * 81 + 27 * grp + base] = v1b[0];
+    }
+}
+
+// contributing (element, local index) pairs of owned control point a along one
+// direction, sorted by element (the reference accumulates in element order)
+__device__ __forceinline__ int elem_pairs(int a, int nelem, int nb, int periodic, int *es, int *ls)
+{
+    int cnt = 0;
+    if (!periodic) {
+        for (int l = 2; l >= 0; l--) {
+            int e = a - l;
+            if (e >= 0 && e < nelem) { es[cnt] = e; ls[cnt] = l; cnt++; }
+        }
+        return cnt;
+    }
+    for (int l = 2; l >= 0; l--) {
+        int e = a - l;
+        if (e < 0) e += nb;
+        es[cnt] = e; ls[cnt] = l; cnt++;
+    }
+    // insertion sort by element
+    for (int i = 1; i < cnt; i++)
+        for (int j = i; j > 0 && es[j] < es[j - 1]; j--) {
+            int te = es[j]; es[j] = es[j - 1]; es[j - 1] = te;
+            int tl = ls[j]; ls[j] = ls[j - 1]; ls[j - 1] = tl;
+        }
+    return cnt;
+}
+
+__global__ __launch_bounds__(256) void k_residual_gather(MeshK M, const double *__restrict__ Re, const double *__restrict__ Uh,
+                                                         const double *__restrict__ Fneu, const int32_t *__restrict__ mask,
+                                                         const double *__restrict__ g, double *__restrict__ R, int64_t ncp)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= ncp) return;
+    int nb1 = M.nb[1], nb2 = M.nb[2];
+    int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+    int exs[3], lxs[3], eys[3], lys[3], ezs[3], lzs[3];
+    int nx = elem_pairs(ax, M.nelem[0], M.nb[0], M.periodic[0], exs, lxs);
+    int ny = elem_pairs(ay, M.nelem[1], M.nb[1], M.periodic[1], eys, lys);
+    int nz = elem_pairs(az, M.nelem[2], M.nb[2], M.periodic[2], ezs, lzs);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                int64_t e = ((int64_t)exs[x] * M.nelem[1] + eys[y]) * M.nelem[2] + ezs[z];
+                int lb = 9 * lxs[x] + 3 * lys[y] + lzs[z];
+                const double *r = Re + e * 81 + 3 * lb;
+                s0 += r[0]; s1 += r[1]; s2 += r[2];
+            }
+    int64_t d = 3 * a;
+    R[d + 0] = mask[d + 0] ? Uh[d + 0] - g[d + 0] : s0 + Fneu[d + 0];
+    R[d + 1] = mask[d + 1] ? Uh[d + 1] - g[d + 1] : s1 + Fneu[d + 1];
+    R[d + 2] = mask[d + 2] ? Uh[d + 2] - g[d + 2] : s2 + Fneu[d + 2];
+}
+
+// ============================================================================
+// tangent: one workgroup (4 waves) per element.
+//   stage A  (once)      geometry and u-jets of the 64 Gauss points, 4 lanes
+//                        per point splitting the 27 basis functions
+//   stage B  (16 chunks of QC = 4 Gauss points)
+//     B1  physical basis jets R_q[10][27] -> LDS
+//     B2  pointwise tangent D_q = w_q C_q (30x30) -> LDS
+//     B3  K^{ij} += sum_{q,a} R_q[a][b] * (sum_b' D_q[(i,a),(j,b')] R_q[b'][c])
+//         on V_MFMA_{F64,F32}_16X16X4: M = b, N = c (27 -> 32), K = (q,a);
+//         wave w owns the (b,c) tile pair (w>>1, w&1) of all nine (i,j)
+//         blocks and forms its B operand on the VALU from D and R.
+// ============================================================================
+constexpr int QC = 4;
+constexpr int GEO = 38;   // per Gauss point: wgt, Gi[9], X2[18], W[10]
+constexpr int STW = 63;   // State: F[9] H[27] S[9] G[9] N[9]
+
+// Operands of the 16x16x4 products: lane l supplies A[l&15][l>>4] and
+// B[l>>4][l&15].  Result register r of lane l holds D[row(l>>4, r)][l&15];
+// the row map of each instruction is pinned by igap_gpu_selftest_mfma.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double4_t acc_t;
+    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kr, int r) { return 4 * r + kr; }
+};
+template <> struct Mfma<float> {
+    typedef float4_t acc_t;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kr, int r) { return 4 * kr + r; }
+};
+
+__device__ __forceinline__ void load_state(const double *s, State &st)
+{
+#pragma unroll
+    for (int i = 0; i < 9; i++) st.F[i] = s[i];
+#pragma unroll
+    for (int i = 0; i < 27; i++) st.H[i] = s[9 + i];
+#pragma unroll
+    for (int i = 0; i < 9; i++) { st.S[i] = s[36 + i]; st.G[i] = s[45 + i]; st.N[i] = s[54 + i]; }
+}
+
+template <typename T, int NH>
+__global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                        T *__restrict__ Ke, int nel)
+{
+    constexpr int STR = 4 + 3 * NH;
+    constexpr int NNODE = (NH == 1) ? 1 : 8;
+    __shared__ double sT[3][4][3][3];          // 1-D tables [d][q_d][order][i]
+    __shared__ double sCP[27 * STR];
+    __shared__ double sGeo[64 * GEO];
+    __shared__ double sJet[64 * 30 * NH];
+    __shared__ double sSt[QC * NNODE * STW];
+    __shared__ T sR[QC][10][32];
+    __shared__ T sD[QC][900];
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int e = blockIdx.x;
+    const int nz = M.nelem[2], ny = M.nelem[1];
+    const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
+    const int ed[3] = {ex, ey, ez};
+
+    // ---- element data, 1-D tables, zero D
+    if (tid < 27) {
+        int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
+        int ax = ex + bx, ay = ey + by, az = ez + bz;
+        int64_t id = ((int64_t)M.na[1] * M.na[2]) * ax + (int64_t)M.na[2] * ay + az;
+        sCP[STR * tid + 0] = M.X[3 * id + 0];
+        sCP[STR * tid + 1] = M.X[3 * id + 1];
+        sCP[STR * tid + 2] = M.X[3 * id + 2];
+        sCP[STR * tid + 3] = M.w[id];
+        int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ax, M.nb[0], M.periodic[0])
+                   + (int64_t)M.nb[2] * own1(ay, M.nb[1], M.periodic[1]) + own1(az, M.nb[2], M.periodic[2]);
+#pragma unroll
+        for (int h = 0; h < NH; h++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) sCP[STR * tid + 4 + 3 * h + k] = Uh[h * ndof + 3 * oc + k];
+    } else if (tid >= 64 && tid < 64 + 12) {
+        int d = (tid - 64) / 4, q = (tid - 64) % 4;
+        const double *kV = M.knot[d] + ed[d];
+        double c = M.gq[q];
+        double x = (-c + 1.0) * kV[2] + c * kV[3];
+        bspline1d(kV, x, sT[d][q][0], sT[d][q][1], sT[d][q][2]);
+    }
+    for (int i = tid; i < QC * 900; i += 256) (&sD[0][0])[i] = (T)0;
+    __syncthreads();
+
+    // ---- stage A: 64 Gauss points x 4 parts
+    {
+        const int q = tid >> 2, part = tid & 3;
+        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+        double t[3][3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int o = 0; o < 3; o++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) t[d][o][i] = sT[d][qd[d]][o][i];
+        double W[10];
+#pragma unroll
+        for (int a = 0; a < 10; a++) W[a] = 0.0;
+        for (int b = part; b < 27; b += 4) {
+            double N[10];
+            tensor_jet(t, b, N);
+            double wb = sCP[STR * b + 3];
+#pragma unroll
+            for (int a = 0; a < 10; a++) W[a] += wb * N[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 10; a++) { W[a] += __shfl_xor(W[a], 1, 64); W[a] += __shfl_xor(W[a], 2, 64); }
+        double Xj[30];
+#pragma unroll
+        for (int a = 0; a < 30; a++) Xj[a] = 0.0;
+        for (int b = part; b < 27; b += 4) {
+            double N[10], R[10];
+            tensor_jet(t, b, N);
+            nurbs_param(N, sCP[STR * b + 3], W, R);
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int a = 0; a < 10; a++) Xj[10 * d + a] += sCP[STR * b + d] * R[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 30; a++) { Xj[a] += __shfl_xor(Xj[a], 1, 64); Xj[a] += __shfl_xor(Xj[a], 2, 64); }
+        double G[9], Gi[9], X2[18];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) G[3 * i + j] = Xj[10 * i + 1 + j];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int s = 0; s < 6; s++) X2[6 * d + s] = Xj[10 * d + 4 + s];
+        double detG = inv3(G, Gi);
+        double xq[3], dX[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double *kV = M.knot[d] + ed[d];
+            double c = M.gq[qd[d]];
+            xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
+            dX[d] = kV[3] - kV[2];
+        }
+        for (int h = 0; h < NH; h++) {
+            double u[30];
+#pragma unroll
+            for (int a = 0; a < 30; a++) u[a] = 0.0;
+            for (int b = part; b < 27; b += 4) {
+                double N[10], R[10];
+                tensor_jet(t, b, N);
+                nurbs_param(N, sCP[STR * b + 3], W, R);
+                phys_jet(R, Gi, X2);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    double Uv = sCP[STR * b + 4 + 3 * h + k];
+#pragma unroll
+                    for (int a = 0; a < 10; a++) u[10 * k + a] += R[a] * Uv;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 30; a++) { u[a] += __shfl_xor(u[a], 1, 64); u[a] += __shfl_xor(u[a], 2, 64); }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                u[10 * k] += M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
+#pragma unroll
+                for (int d = 0; d < 3; d++) u[10 * k + 1 + d] += M.par_periodic[3 * k + d];
+            }
+            if (part == 0) {
+#pragma unroll
+                for (int a = 0; a < 30; a++) sJet[q * 30 * NH + 30 * h + a] = u[a];
+            }
+        }
+        if (part == 0) {
+            double *g = sGeo + q * GEO;
+            g[0] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
+#pragma unroll
+            for (int i = 0; i < 9; i++) g[1 + i] = Gi[i];
+#pragma unroll
+            for (int i = 0; i < 18; i++) g[10 + i] = X2[i];
+#pragma unroll
+            for (int i = 0; i < 10; i++) g[28 + i] = W[i];
+        }
+    }
+    __syncthreads();
+
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) acc[i] = (acc_t){0, 0, 0, 0};
+    const int bt = wv >> 1, ctile = wv & 1, m = lane & 15, kr = lane >> 4;
+
+    for (int chunk = 0; chunk < 64 / QC; chunk++) {
+        const int q0 = chunk * QC;
+        // ---- B1: physical basis jets
+        if (tid < QC * 32) {
+            int ql = tid >> 5, b = tid & 31, q = q0 + ql;
+            if (b < 27) {
+                const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+                double t[3][3][3];
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+#pragma unroll
+                    for (int o = 0; o < 3; o++)
+#pragma unroll
+                        for (int i = 0; i < 3; i++) t[d][o][i] = sT[d][qd[d]][o][i];
+                const double *g = sGeo + q * GEO;
+                double N[10], R[10];
+                tensor_jet(t, b, N);
+                nurbs_param(N, sCP[STR * b + 3], g + 28, R);
+                phys_jet(R, g + 1, g + 10);
+#pragma unroll
+                for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)R[a];
+            } else {
+#pragma unroll
+                for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)0;
+            }
+        } else if (tid < QC * 32 + QC * NNODE) {
+            // ---- B2a: strain-measure states at the rule nodes
+            int r = tid - QC * 32, ql = r / NNODE, j = r % NNODE, q = q0 + ql;
+            const double *u = sJet + q * 30 * NH;
+            double us[30];
+            if (NH == 1) {
+#pragma unroll
+                for (int i = 0; i < 30; i++) us[i] = u[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 30; i++) us[i] = 0.5 * (u[60 + i] + u[30 + i]) + P.s[j] * (0.5 * (u[i] - u[60 + i]));
+            }
+            State st;
+            phys_state(us, P, st);
+            double *s = sSt + (ql * NNODE + j) * STW;
+#pragma unroll
+            for (int i = 0; i < 9; i++) s[i] = st.F[i];
+#pragma unroll
+            for (int i = 0; i < 27; i++) s[9 + i] = st.H[i];
+#pragma unroll
+            for (int i = 0; i < 9; i++) { s[36 + i] = st.S[i]; s[45 + i] = st.G[i]; s[54 + i] = st.N[i]; }
+        }
+        __syncthreads();
+        // ---- B2b: D_q = w_q C_q, wave wv handles Gauss point q0 + wv
+        {
+            const int ql = wv, q = q0 + ql;
+            const double wgt = sGeo[q * GEO];
+            T *D = sD[ql];
+            for (int f = lane; f < 81; f += 64) {
+                int k = f / 27, L = (f / 9) % 3, l = (f / 3) % 3, Mi = f % 3;
+                double v = 0.0;
+                for (int j = 0; j < NNODE; j++) {
+                    State st;
+                    load_state(sSt + (ql * NNODE + j) * STW, st);
+                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_ff(st, P, k, L, l, Mi);
+                }
+                D[(10 * k + 1 + L) * 30 + 10 * l + 1 + Mi] = (T)(wgt * v);
+            }
+            for (int f = lane; f < 162; f += 64) {
+                int k = f / 54, L = (f / 18) % 3, l = (f / 6) % 3, p = f % 6;
+                int I = kPairI[p], K = kPairK[p];
+                double v = 0.0;
+                for (int j = 0; j < NNODE; j++) {
+                    State st;
+                    load_state(sSt + (ql * NNODE + j) * STW, st);
+                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_fh(st, P, k, L, l, I, K);
+                }
+                T tv = (T)(wgt * v);
+                D[(10 * k + 1 + L) * 30 + 10 * l + 4 + p] = tv;
+                D[(10 * l + 4 + p) * 30 + 10 * k + 1 + L] = tv;
+            }
+            for (int f = lane; f < 324; f += 64) {
+                int k = f / 108, p = (f / 18) % 6, l = (f / 6) % 3, p2 = f % 6;
+                double v = 0.0;
+                for (int j = 0; j < NNODE; j++) {
+                    State st;
+                    load_state(sSt + (ql * NNODE + j) * STW, st);
+                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_hh(st, P, k, kPairI[p], kPairK[p], l, kPairI[p2], kPairK[p2]);
+                }
+                D[(10 * k + 4 + p) * 30 + 10 * l + 4 + p2] = (T)(wgt * v);
+            }
+            if (NH != 1 && lane < 3)
+                D[(10 * lane) * 30 + 10 * lane] = (T)(wgt * (P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt)));
+        }
+        __syncthreads();
+        // ---- B3: matrix-core contraction, K-dim = (q, a) of this chunk
+#pragma unroll 2
+        for (int s = 0; s < QC * 10 / 4; s++) {
+            const int kk = 4 * s + kr, ql = kk / 10, a = kk % 10;
+            const T A = sR[ql][a][16 * bt + m];
+            T Rc[10];
+#pragma unroll
+            for (int bb = 0; bb < 10; bb++) Rc[bb] = sR[ql][bb][16 * ctile + m];
+            const T *Drow = sD[ql] + a * 30;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const T *d = Drow + 300 * i + 10 * j;
+                    T y = d[0] * Rc[0];
+#pragma unroll
+                    for (int bb = 1; bb < 10; bb++) y += d[bb] * Rc[bb];
+                    acc[3 * i + j] = Mfma<T>::run(A, y, acc[3 * i + j]);
+                }
+        }
+        __syncthreads();
+    }
+
+    // ---- write K_e (row 3b+i, column 3c+j): acc[r] holds tile row Mfma<T>::row(kr, r), column m
+    T *out = Ke + (int64_t)e * 6561;
+    const int c = 16 * ctile + m;
+    if (c < 27) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int b = 16 * bt + Mfma<T>::row(kr, r);
+            if (b < 27) {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) out[(3 * b + i) * 81 + 3 * c + j] = acc[3 * i + j][r];
+            }
+        }
+    }
+}
+
+// sorted owned control points within +-2 of a (clipped, or wrapped when periodic)
+__device__ __forceinline__ int stencil1(int a, int n, int periodic, int *S)
+{
+    int cnt = 0;
+    if (!periodic) {
+        int lo = a - 2 < 0 ? 0 : a - 2, hi = a + 2 > n - 1 ? n - 1 : a + 2;
+        for (int b = lo; b <= hi; b++) S[cnt++] = b;
+        return cnt;
+    }
+    int tmp[5];
+    for (int t = 0; t < 5; t++) { int v = a + t - 2; v = (v % n + n) % n; tmp[t] = v; }
+    for (int i = 1; i < 5; i++)
+        for (int j = i; j > 0 && tmp[j] < tmp[j - 1]; j--) { int x = tmp[j]; tmp[j] = tmp[j - 1]; tmp[j - 1] = x; }
+    for (int t = 0; t < 5; t++)
+        if (cnt == 0 || tmp[t] != S[cnt - 1]) S[cnt++] = tmp[t];
+    return cnt;
+}
+
+// elements along one direction containing owned points a and b, sorted, with
+// their local indices
+__device__ __forceinline__ int elem_pairs2(int a, int b, int nelem, int nb, int periodic, int *es, int *la, int *lb)
+{
+    int cnt = 0;
+    if (!periodic) {
+        int lo = (a > b ? a : b) - 2, hi = a < b ? a : b;
+        if (lo < 0) lo = 0;
+        if (hi > nelem - 1) hi = nelem - 1;
+        for (int e = lo; e <= hi; e++) { es[cnt] = e; la[cnt] = a - e; lb[cnt] = b - e; cnt++; }
+        return cnt;
+    }
+    for (int l = 2; l >= 0; l--) {
+        int e = a - l;
+        if (e < 0) e += nb;
+        int m = b - e;
+        if (m < 0) m += nb;
+        if (m <= 2) { es[cnt] = e; la[cnt] = l; lb[cnt] = m; cnt++; }
+    }
+    for (int i = 1; i < cnt; i++)
+        for (int j = i; j > 0 && es[j] < es[j - 1]; j--) {
+            int t = es[j]; es[j] = es[j - 1]; es[j - 1] = t;
+            t = la[j]; la[j] = la[j - 1]; la[j - 1] = t;
+            t = lb[j]; lb[j] = lb[j - 1]; lb[j - 1] = t;
+        }
+    return cnt;
+}
+
+// one workgroup per owned control point a (its three rows); thread t <-> the
+// t-th control point of the row's sorted column stencil
+template <typename TK, typename TV>
+__global__ __launch_bounds__(128) void k_tangent_gather(MeshK M, const TK *__restrict__ Ke, const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ mask, TV *__restrict__ vals)
+{
+    const int64_t a = blockIdx.x;
+    int nb1 = M.nb[1], nb2 = M.nb[2];
+    int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+    int Sx[5], Sy[5], Sz[5];
+    int nsx = stencil1(ax, M.nb[0], M.periodic[0], Sx);
+    int nsy = stencil1(ay, M.nb[1], M.periodic[1], Sy);
+    int nsz = stencil1(az, M.nb[2], M.periodic[2], Sz);
+    int t = threadIdx.x;
+    if (t >= nsx * nsy * nsz) return;
+    int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
+    int bx = Sx[sx], by = Sy[sy], bz = Sz[sz];
+    int ex[3], lax[3], lbx[3], ey[3], lay[3], lby[3], ez[3], laz[3], lbz[3];
+    int nx = elem_pairs2(ax, bx, M.nelem[0], M.nb[0], M.periodic[0], ex, lax, lbx);
+    int ny = elem_pairs2(ay, by, M.nelem[1], M.nb[1], M.periodic[1], ey, lay, lby);
+    int nz = elem_pairs2(az, bz, M.nelem[2], M.nb[2], M.periodic[2], ez, laz, lbz);
+    double s[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) s[i] = 0.0;
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                int64_t e = ((int64_t)ex[x] * M.nelem[1] + ey[y]) * M.nelem[2] + ez[z];
+                int la = 9 * lax[x] + 3 * lay[y] + laz[z], lb = 9 * lbx[x] + 3 * lby[y] + lbz[z];
+                const TK *k = Ke + e * 6561 + (3 * la) * 81 + 3 * lb;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) s[3 * i + j] += (double)k[81 * i + j];
+            }
+    int64_t bcp = ((int64_t)M.nb[1] * M.nb[2]) * bx + (int64_t)M.nb[2] * by + bz;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        int64_t row = 3 * a + i;
+        int64_t pos = rowptr[row] + 3 * t;
+        bool mr = mask[row] != 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            int64_t col = 3 * bcp + j;
+            double v = (mr || mask[col]) ? (row == col ? 1.0 : 0.0) : s[3 * i + j];
+            vals[pos + j] = (TV)v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(128) void k_csr_cols(MeshK M, const int64_t *__restrict__ rowptr, int32_t *__restrict__ col)
+{
+    const int64_t a = blockIdx.x;
+    int nb1 = M.nb[1], nb2 = M.nb[2];
+    int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+    int Sx[5], Sy[5], Sz[5];
+    int nsx = stencil1(ax, M.nb[0], M.periodic[0], Sx);
+    int nsy = stencil1(ay, M.nb[1], M.periodic[1], Sy);
+    int nsz = stencil1(az, M.nb[2], M.periodic[2], Sz);
+    int t = threadIdx.x;
+    if (t >= nsx * nsy * nsz) return;
+    int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
+    int64_t bcp = ((int64_t)M.nb[1] * M.nb[2]) * Sx[sx] + (int64_t)M.nb[2] * Sy[sy] + Sz[sz];
+    for (int i = 0; i < 3; i++) {
+        int64_t pos = rowptr[3 * a + i] + 3 * t;
+        for (int j = 0; j < 3; j++) col[pos + j] = (int32_t)(3 * bcp + j);
+    }
+}
+
+template <typename T>
+__global__ void k_fill(T *p, T v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = v;
+}
+
+__global__ void k_copy(double *dst, const double *src, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) dst[i] = src[i];
+}
+
+__global__ void k_selftest_mfma(double *o64, float *o32)
+{
+    int l = threadIdx.x;
+    int i = l & 15, k = l >> 4;
+    // A[i][k] = 1 + i + 0.25 k ; B[k][j] = 2 - 0.5 j + k
+    double a = 1.0 + i + 0.25 * k;
+    double b = 2.0 - 0.5 * i + k;   // lane holds B[k = l>>4][j = l&15]
+    double4_t d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, (double4_t){0, 0, 0, 0}, 0, 0, 0);
+    float4_t f = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, (float4_t){0, 0, 0, 0}, 0, 0, 0);
+    // raw result registers: out[(r*64 + lane)]
+    for (int r = 0; r < 4; r++) {
+        o64[r * 64 + l] = d[r];
+        o32[r * 64 + l] = f[r];
+    }
+}
+
+// ============================================================================
+// launchers (host, C++ linkage; called by the C ABI in igap_capi.cpp)
+// ============================================================================
+}  // namespace igap
+
+#include "igap_launch.h"
+
+namespace igap {
+
+hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Re, int nel, hipStream_t s)
+{
+    dim3 grid((nel + 3) / 4), block(256);
+    if (P.torder == 0) hipLaunchKernelGGL(k_residual_elem<1>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
+    else hipLaunchKernelGGL(k_residual_elem<3>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_gather(const MeshK &M, const double *Re, const double *Uh, const double *Fneu, const int32_t *mask,
+                                  const double *g, double *R, int64_t ncp, hipStream_t s)
+{
+    dim3 grid((unsigned)((ncp + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_residual_gather, grid, block, 0, s, M, Re, Uh, Fneu, mask, g, R, ncp);
+    return hipGetLastError();
+}
+
+hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, void *Ke, int prec, int nel,
+                               hipStream_t s)
+{
+    dim3 grid(nel), block(256);
+    if (prec == 0) {
+        if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel);
+        else hipLaunchKernelGGL((k_tangent_elem<double, 3>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel);
+    } else {
+        if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<float, 1>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel);
+        else hipLaunchKernelGGL((k_tangent_elem<float, 3>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tangent_gather(const MeshK &M, const void *Ke, int prec, const int64_t *rowptr, const int32_t *mask, void *vals,
+                                 int64_t ncp, hipStream_t s)
+{
+    dim3 grid((unsigned)ncp), block(128);
+    if (prec == 0) hipLaunchKernelGGL((k_tangent_gather<double, double>), grid, block, 0, s, M, (const double *)Ke, rowptr, mask, (double *)vals);
+    else hipLaunchKernelGGL((k_tangent_gather<float, float>), grid, block, 0, s, M, (const float *)Ke, rowptr, mask, (float *)vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_cols(const MeshK &M, const int64_t *rowptr, int32_t *col, int64_t ncp, hipStream_t s)
+{
+    dim3 grid((unsigned)ncp), block(128);
+    hipLaunchKernelGGL(k_csr_cols, grid, block, 0, s, M, rowptr, col);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_f64(double *dst, const double *src, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest_mfma(double *o64, float *o32, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, s, o64, o32);
+    return hipGetLastError();
+}
+
+}  // namespace igap

@@ -1,0 +1,23 @@
+// igap_launch.h -- host launchers of the assembly kernels (igap_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "igap_device.h"
+
+namespace igap {
+
+hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Re, int nel, hipStream_t s);
+hipError_t launch_residual_gather(const MeshK &M, const double *Re, const double *Uh, const double *Fneu, const int32_t *mask,
+                                  const double *g, double *R, int64_t ncp, hipStream_t s);
+hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, void *Ke, int prec, int nel,
+                               hipStream_t s);
+hipError_t launch_tangent_gather(const MeshK &M, const void *Ke, int prec, const int64_t *rowptr, const int32_t *mask, void *vals,
+                                 int64_t ncp, hipStream_t s);
+hipError_t launch_csr_cols(const MeshK &M, const int64_t *rowptr, int32_t *col, int64_t ncp, hipStream_t s);
+hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t s);
+hipError_t launch_copy_f64(double *dst, const double *src, int64_t n, hipStream_t s);
+hipError_t launch_selftest_mfma(double *o64, float *o32, hipStream_t s);
+
+}  // namespace igap

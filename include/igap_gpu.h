@@ -1,0 +1,164 @@
+/* igap_gpu.h -- C ABI of the MI355X residual/tangent assembly for IGAP4.
+ *
+ * One patch lives in one GPU's HBM.  A context is built from descriptors that
+ * carry the same data as the reference's Mesh / Phys / BC structs, and its two
+ * assembly entry points have the semantics of the Assm callbacks that IGAP4
+ * hands to PETSc SNES:
+ *
+ *   reference                                         this library
+ *   ------------------------------------------------  -------------------------------
+ *   Residual_ptr  Residual_iga(SNES,Vec U,Vec R,ctx)   igap_gpu_residual(ctx,U,R)
+ *     include/assmimpl.h:7, src/assmcore_iga.c:16-502
+ *   Tangent_ptr   Tangent_iga(SNES,Vec U,Mat J,Mat P,   igap_gpu_tangent(ctx,U,vals,prec)
+ *     ctx)  include/assmimpl.h:8, assmcore_iga.c:507-858
+ *   IGAPAssmInit  src/interface.c:131-203               igap_gpu_create
+ *   setnz_iga + MatCreateAIJ  assmcore_iga.c:1054-1087  igap_gpu_csr_* (device CSR)
+ *   IGAPFinalize  src/interface.c:372-435               igap_gpu_destroy
+ *
+ * All device pointers are HIP device addresses on the context's GPU; every
+ * call is stream-ordered on the context's stream and returns before the
+ * kernels finish unless stated otherwise.  Every function returns an int
+ * status (IGAP_OK = 0) instead of aborting; igap_gpu_last_error() gives text.
+ * A context is used from one host thread at a time.
+ */
+#ifndef IGAP_GPU_H
+#define IGAP_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    IGAP_OK = 0,
+    IGAP_ERR_ARG = 1,      /* invalid argument / unsupported configuration */
+    IGAP_ERR_HIP = 2,      /* HIP runtime error (message has the HIP text) */
+    IGAP_ERR_NOMEM = 3,    /* device allocation failed */
+    IGAP_ERR_STATE = 4     /* call out of order (e.g. no pattern built) */
+};
+
+enum { IGAP_PHYS_GRADELAST = 0, IGAP_PHYS_GRADELASTTIME = 2 };  /* = torder */
+enum { IGAP_F64 = 0, IGAP_F32 = 1 };
+
+/* Mesh of one patch: struct __Mesh (include/meshimpl.h:6-29) for one rank
+ * after mpipart_iga (src/meshcore_iga.c:211-228).  Control points use the
+ * "active" numbering id = (na1*na2)*i + na2*j + k with na_d = nelem_d + 2
+ * (src/assmcore_iga.c:110-111,138); in a periodic direction the last two
+ * layers alias the first two.  porder must be 2. */
+typedef struct {
+    int           porder;
+    int           nelem[3];
+    int           periodic[3];
+    const double *knot[3];   /* knotVector[d], nelem[d]+1+2*porder entries (host) */
+    const double *w;         /* wVector, na0*na1*na2 (host)                        */
+    const double *X;         /* XVector, 3*na0*na1*na2 (host)                      */
+} igap_mesh_desc;
+
+/* Physics: struct __Phys (include/physimpl.h:13-28) restricted to the two
+ * reference physics.  par = par_mat {r, curv, e0, e345, le, rho, cii, dt, t}
+ * (phys/physgradelast.c:21).  use_coef = 0 derives the energy coefficients
+ * (c2,c3,c4,e0,e345,le) from par as phys/gradelast.c:46-48 does; use_coef = 1
+ * takes coef[] as given (e.g. St Venant-Kirchhoff: mu,0,0,3lambda/2+mu,2mu,l). */
+typedef struct {
+    int    kind;             /* IGAP_PHYS_GRADELAST or IGAP_PHYS_GRADELASTTIME */
+    double par[9];
+    int    use_coef;
+    double coef[6];
+} igap_phys_desc;
+
+/* Boundary conditions: struct __BC (include/bcimpl.h:8-15).  bc_type index
+ * (2*3)*face + 3*order + dof, 0 = Dirichlet, 1 = Neumann.  The Neumann
+ * callback is the constant table of neumann_default_iga (assmcore_iga.c:
+ * 1036-1052) with user values, neumann[(2*3)*face + 3*order + dof]. */
+typedef struct {
+    int    bc_type[36];
+    double par_periodic[9];
+    double par_dirichlet[9];
+    double neumann[36];
+} igap_bc_desc;
+
+typedef struct igap_gpu_ctx igap_gpu_ctx;
+
+/* ---- set-up ------------------------------------------------------------ */
+/* Defaults of the reference constructors: PhysGradElast / PhysGradElastTime
+ * (phys/physgradelast.c:9-35, phys/physgradelasttime.c:9-34) and
+ * IGAPBCAddDefault (src/interface.c:60-74). */
+void igap_phys_gradelast(igap_phys_desc *ph);
+void igap_phys_gradelasttime(igap_phys_desc *ph);
+void igap_bc_default(igap_bc_desc *bc);
+
+/* igabeam / igabeamtest geometry (mesh/igabeam.c:9-73, mesh/igabeamtest.c:68-70)
+ * with per-direction element counts.  knot[d] holds nelem[d]+5 doubles, w
+ * na-product, X 3*na-product; all caller-owned host memory. */
+int igap_mesh_igabeam(const double L[3], const int nelem[3], const int periodic[3], int perturbed,
+                      double *knot0, double *knot1, double *knot2, double *w, double *X);
+
+/* IGAPAssmInit (src/interface.c:131-203) for one GPU: uploads the patch,
+ * builds the CSR pattern, the Dirichlet row set and the Neumann load, and
+ * allocates U_hist, R and the matrix values.  device < 0 keeps the current
+ * device.  flags: reserved, pass 0. */
+int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc *phys, const igap_bc_desc *bc,
+                    int device, int flags, igap_gpu_ctx **ctx);
+int igap_gpu_destroy(igap_gpu_ctx *ctx);
+const char *igap_gpu_last_error(void);
+
+/* Replace the boundary conditions (e.g. the Dirichlet load factor of a load
+ * step) without rebuilding the context. */
+int igap_gpu_set_bc(igap_gpu_ctx *ctx, const igap_bc_desc *bc);
+
+/* ---- sizes and device buffers owned by the context ---------------------- */
+int igap_gpu_sizes(const igap_gpu_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t *nelem, int *torder);
+/* core->U_hist (include/coreimpl.h:17): (torder+1)*ndof doubles, level h at
+ * offset h*ndof, 0 = current iterate (src/interface.c:127). */
+double *igap_gpu_uhist(igap_gpu_ctx *ctx);
+/* device CSR pattern (rows = owned dofs dof = 3*cp+i, columns sorted) */
+const int64_t *igap_gpu_csr_rowptr(const igap_gpu_ctx *ctx);
+const int32_t *igap_gpu_csr_col(const igap_gpu_ctx *ctx);
+/* matrix values owned by the context (FP64 and FP32 arrays of nnz entries) */
+double *igap_gpu_csr_val64(igap_gpu_ctx *ctx);
+float *igap_gpu_csr_val32(igap_gpu_ctx *ctx);
+/* Dirichlet row mask (int32, 1 = constrained) and inserted values g */
+const int32_t *igap_gpu_dirichlet_mask(const igap_gpu_ctx *ctx);
+const double *igap_gpu_dirichlet_values(const igap_gpu_ctx *ctx);
+
+/* ---- the assembly callbacks -------------------------------------------- */
+/* Residual_iga: copies U (ndof doubles, device) into U_hist level 0, then
+ * R = volume integral - Neumann load, Dirichlet rows R_d = U_d - g_d.
+ * R may be NULL to write the context's own R buffer (igap_gpu_rvec). */
+int igap_gpu_residual(igap_gpu_ctx *ctx, const double *U, double *R);
+double *igap_gpu_rvec(igap_gpu_ctx *ctx);
+/* Tangent_iga: copies U into U_hist level 0 (U may be NULL to use U_hist as
+ * it is), zeroes the values, assembles the volume tangent deterministically
+ * and applies MatZeroRowsColumns(diag = 1) to the Dirichlet rows.
+ * prec = IGAP_F64 writes vals as double, IGAP_F32 as float (FP32 MFMA);
+ * vals may be NULL for the context's own arrays. */
+int igap_gpu_tangent(igap_gpu_ctx *ctx, const double *U, void *vals, int prec);
+
+/* ---- helpers ------------------------------------------------------------ */
+int igap_gpu_synchronize(igap_gpu_ctx *ctx);
+int igap_gpu_malloc(igap_gpu_ctx *ctx, void **ptr, int64_t bytes);
+int igap_gpu_free(igap_gpu_ctx *ctx, void *ptr);
+int igap_gpu_memcpy_htod(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int igap_gpu_memcpy_dtoh(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int igap_gpu_memcpy_dtod(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
+
+/* Timing of the assembly kernels with HIP events recorded on the context's
+ * stream around each launch.  which: 0 residual volume kernel, 1 tangent
+ * volume kernel, 2 CSR gather kernel, 3 whole residual call, 4 whole tangent
+ * call.  Accumulated milliseconds and launch counts since the last reset. */
+int igap_gpu_timing_enable(igap_gpu_ctx *ctx, int on);
+int igap_gpu_timing_get(igap_gpu_ctx *ctx, int which, double *ms_total, int64_t *count);
+int igap_gpu_timing_reset(igap_gpu_ctx *ctx);
+
+/* Self test of the matrix-core operand layout the tangent kernels assume:
+ * one 16x16x4 product with V_MFMA_F64_16X16X4F64 and one with
+ * V_MFMA_F32_16X16X4F32 of A[i][k] = 1 + i + k/4, B[k][j] = 2 - j/2 + k,
+ * lane l supplying A[l&15][l>>4] and B[l>>4][l&15]; out64/out32 receive the
+ * 4 x 64 raw result registers (out[r*64 + l]).  Host pointers. */
+int igap_gpu_selftest_mfma(double *out64, float *out32);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

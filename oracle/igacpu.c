@@ -17,9 +17,9 @@
  * The volume loops keep the reference's work profile: the per-Gauss-point
  * naive 81x81x10x10 tangent contraction (assmcore_iga.c:723-749) and a
  * per-element insertion into CSR with a per-row search (the MatSetValues ADD
- * of assmcore_iga.c:763).  OpenMP threads split each of the 27 element
- * colours (elements whose indices agree mod 3 share no control point), so the
- * inserts of one colour never touch the same row.
+ * of assmcore_iga.c:763).  OpenMP threads split each element colour (see
+ * colour1: elements of one colour share no control point), so the inserts of
+ * one colour never touch the same row.
  */
 #include "igacpu.h"
 
@@ -882,6 +882,17 @@ void oc_element_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc,
 /* assembly                                                                    */
 /* ========================================================================== */
 
+/* Element colouring for the threaded loops: elements of one colour share no
+ * control point.  Along a direction the colour is e mod 3; in a periodic
+ * direction whose element count is not a multiple of 3 the trailing
+ * elements (which wrap onto the first control points) get colours 3 and 4. */
+static int ncolour1(const oc_mesh *m, int d) { return (m->periodic[d] && m->nelem[d] % 3) ? 3 + m->nelem[d] % 3 : 3; }
+static int colour1(const oc_mesh *m, int d, int e)
+{
+    int reg = m->periodic[d] ? 3 * (m->nelem[d] / 3) : m->nelem[d];
+    return e < reg ? e % 3 : 3 + (e - reg);
+}
+
 static int in_box(const int *box, int ex, int ey, int ez)
 {
     if (!box) return 1;
@@ -903,13 +914,14 @@ void oc_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const dou
     int64_t nd = oc_ndof(m);
     set_threads(nthreads);
     for (int64_t i = 0; i < nd; i++) R[i] = 0.0;   /* VecSet(RR,0) :105 */
-    for (int col = 0; col < 27; col++) {
-        int cx = col / 9, cy = (col / 3) % 3, cz = col % 3;
-        int nx = (m->nelem[0] - cx + 2) / 3, ny = (m->nelem[1] - cy + 2) / 3, nz = (m->nelem[2] - cz + 2) / 3;
-        int64_t tot = (int64_t)nx * ny * nz;
+    int ncx = ncolour1(m, 0), ncy = ncolour1(m, 1), ncz = ncolour1(m, 2);
+    int64_t nel = (int64_t)m->nelem[0] * m->nelem[1] * m->nelem[2];
+    for (int col = 0; col < ncx * ncy * ncz; col++) {
+        int cx = col / (ncy * ncz), cy = (col / ncz) % ncy, cz = col % ncz;
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t t = 0; t < tot; t++) {
-            int ex = cx + 3 * (int)(t / ((int64_t)ny * nz)), ey = cy + 3 * (int)((t / nz) % ny), ez = cz + 3 * (int)(t % nz);
+        for (int64_t t = 0; t < nel; t++) {
+            int ex = (int)(t / ((int64_t)m->nelem[1] * m->nelem[2])), ey = (int)((t / m->nelem[2]) % m->nelem[1]), ez = (int)(t % m->nelem[2]);
+            if (colour1(m, 0, ex) != cx || colour1(m, 1, ey) != cy || colour1(m, 2, ez) != cz) continue;
             if (!in_box(box, ex, ey, ez)) continue;
             double val[81];
             oc_element_residual(m, ph, bc, U_hist, ex, ey, ez, val);
@@ -952,13 +964,14 @@ void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const doub
     int64_t nd = oc_ndof(m);
     set_threads(nthreads);
     for (int64_t i = 0; i < rowptr[nd]; i++) val[i] = 0.0;   /* MatZeroEntries :594 */
-    for (int cc = 0; cc < 27; cc++) {
-        int cx = cc / 9, cy = (cc / 3) % 3, cz = cc % 3;
-        int nx = (m->nelem[0] - cx + 2) / 3, ny = (m->nelem[1] - cy + 2) / 3, nz = (m->nelem[2] - cz + 2) / 3;
-        int64_t tot = (int64_t)nx * ny * nz;
+    int ncx = ncolour1(m, 0), ncy = ncolour1(m, 1), ncz = ncolour1(m, 2);
+    int64_t nel = (int64_t)m->nelem[0] * m->nelem[1] * m->nelem[2];
+    for (int cc = 0; cc < ncx * ncy * ncz; cc++) {
+        int cx = cc / (ncy * ncz), cy = (cc / ncz) % ncy, cz = cc % ncz;
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t t = 0; t < tot; t++) {
-            int ex = cx + 3 * (int)(t / ((int64_t)ny * nz)), ey = cy + 3 * (int)((t / nz) % ny), ez = cz + 3 * (int)(t % nz);
+        for (int64_t t = 0; t < nel; t++) {
+            int ex = (int)(t / ((int64_t)m->nelem[1] * m->nelem[2])), ey = (int)((t / m->nelem[2]) % m->nelem[1]), ez = (int)(t % m->nelem[2]);
+            if (colour1(m, 0, ex) != cx || colour1(m, 1, ey) != cy || colour1(m, 2, ez) != cz) continue;
             if (!in_box(box, ex, ey, ez)) continue;
             double *ke = (double *)malloc(6561 * sizeof(double));
             oc_element_tangent(m, ph, bc, U_hist, ex, ey, ez, ke);

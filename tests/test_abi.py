@@ -1,0 +1,62 @@
+"""The C ABI library loads here (no GPU needed), exports every symbol that
+include/igap_gpu.h declares, and its host-only entry points agree with the
+oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "igap_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(igap_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(igap):
+    from igap4_amd import _lib
+    L = C.CDLL(_lib.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # and the Python binding types every one of them
+    assert sorted(_lib.declared_symbols()) == syms
+
+
+def test_mesh_generator_matches_oracle(igap, oracle):
+    for nel, per, pert in (((3, 2, 4), (0, 0, 0), False), ((5, 6, 5), (1, 0, 1), False), ((7, 6, 6), (0, 0, 0), True)):
+        a = igap.MeshIGABeam(L=(4, 1, 1), nelem=nel, periodic=per, perturbed=pert)
+        b = oracle.Mesh(nel, L=(4, 1, 1), periodic=per, perturbed=pert)
+        assert all(np.array_equal(x, y) for x, y in zip(a.knots, b.knots))
+        assert np.array_equal(a.w, b.w) and np.array_equal(a.X, b.X)
+        assert a.ndof == b.ndof
+
+
+def test_phys_defaults_are_the_reference_constructors(igap, oracle):
+    s = igap.PhysGradElast()
+    t = igap.PhysGradElastTime()
+    assert s.kind == 0 and t.kind == 2 and s.use_coef == 0
+    assert np.array_equal(np.array(s.par[:]), oracle.REF_PAR_STATIC)
+    assert np.array_equal(np.array(t.par[:]), oracle.REF_PAR_TRANSIENT)
+    bc = igap.BCDefault()
+    assert list(bc.bc_type) == [1] * 36
+    assert not any(bc.par_dirichlet) and not any(bc.par_periodic) and not any(bc.neumann)
+
+
+def test_bad_descriptor_is_an_error_not_a_crash(igap):
+    """Validation happens on the host before any device work."""
+    from igap4_amd import _lib
+    m = igap.MeshIGABeam(nelem=(4, 3, 3), periodic=(1, 0, 0))
+    d = m.desc()
+    ph = igap.PhysGradElast()
+    bc = igap.BCDefault()
+    ctx = C.c_void_p()
+    rc = _lib.lib().igap_gpu_create(C.byref(d), C.byref(ph), C.byref(bc), -1, 0, C.byref(ctx))
+    assert rc == 1 and b"periodic" in _lib.lib().igap_gpu_last_error()
+    d.porder = 3
+    rc = _lib.lib().igap_gpu_create(C.byref(d), C.byref(ph), C.byref(bc), -1, 0, C.byref(ctx))
+    assert rc == 1 and b"porder" in _lib.lib().igap_gpu_last_error()
