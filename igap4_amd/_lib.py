@@ -41,6 +41,29 @@ class BCDesc(C.Structure):
                 ("par_dirichlet", C.c_double * 9), ("neumann", C.c_double * 36)]
 
 
+class SolverOpts(C.Structure):
+    """igap_solver_opts (IGAPSolvInit settings, src/interface.c:207-250)."""
+    _fields_ = [("snes_atol", C.c_double), ("snes_rtol", C.c_double), ("snes_maxit", C.c_int),
+                ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double), ("ksp_maxit", C.c_int),
+                ("ksp_restart", C.c_int), ("pc", C.c_int), ("tangent_prec", C.c_int),
+                ("linesearch", C.c_int)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("newton_its", C.c_int), ("ksp_its", C.c_int), ("ls_backtracks", C.c_int),
+                ("n_residual", C.c_int), ("converged", C.c_int), ("fnorm0", C.c_double),
+                ("fnorm", C.c_double), ("last_ksp_relres", C.c_double),
+                ("t_residual_ms", C.c_double), ("t_tangent_ms", C.c_double), ("t_pc_ms", C.c_double),
+                ("t_ksp_ms", C.c_double), ("t_total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+IGAP_PC_ILU0 = 0
+IGAP_PC_JACOBI = 1
+IGAP_PC_NONE = 2
+
 # every symbol include/igap_gpu.h declares, with its ctypes signature
 _SIGS = {
     "igap_phys_gradelast": (None, [C.POINTER(PhysDesc)]),
@@ -74,6 +97,8 @@ _SIGS = {
     "igap_gpu_timing_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_i64_p]),
     "igap_gpu_timing_reset": (C.c_int, [C.c_void_p]),
     "igap_gpu_selftest_mfma": (C.c_int, [c_double_p, c_float_p]),
+    "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),
+    "igap_gpu_snes_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
 }
 
 _LIB = None

@@ -226,6 +226,16 @@ class Assm:
             vals = self.dtoh(lib().igap_gpu_csr_val32(self.ctx), self.nnz, np.float32)
         return rowptr, col, vals
 
+    # ------------------------------------------------------------ Newton-Krylov
+    def snes_solve(self, opts=None, **kw):
+        """Newton solve from U_hist level 0 (SNESSolve driven by IGAPSolvSolve,
+        src/interface.c:302-328); returns the solver statistics as a dict."""
+        if opts is None:
+            opts = default_solver_opts(**kw)
+        st = _lib.SolverStats()
+        check(lib().igap_gpu_snes_solve(self.ctx, C.byref(opts), C.byref(st)))
+        return st.as_dict()
+
     # ------------------------------------------------------------ timing
     def timing(self, on=True):
         check(lib().igap_gpu_timing_enable(self.ctx, 1 if on else 0))
@@ -237,6 +247,20 @@ class Assm:
         ms, n = C.c_double(), C.c_int64()
         check(lib().igap_gpu_timing_get(self.ctx, which, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def default_solver_opts(**kw):
+    """igap_gpu_solver_default_opts with keyword overrides, e.g.
+    ``default_solver_opts(snes_rtol=1e-10, tangent_prec=1)``."""
+    o = _lib.SolverOpts()
+    check(lib().igap_gpu_solver_default_opts(C.byref(o)))
+    for k, v in kw.items():
+        if k in ("tangent_prec",) and isinstance(v, str):
+            v = _lib.IGAP_F64 if v == "f64" else _lib.IGAP_F32
+        if k == "pc" and isinstance(v, str):
+            v = {"ilu0": _lib.IGAP_PC_ILU0, "jacobi": _lib.IGAP_PC_JACOBI, "none": _lib.IGAP_PC_NONE}[v]
+        setattr(o, k, v)
+    return o
 
 
 def selftest_mfma():

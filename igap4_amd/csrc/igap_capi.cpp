@@ -14,14 +14,14 @@
 #include <vector>
 
 #include "../../include/igap_gpu.h"
+#include "igap_ctx.h"
 #include "igap_launch.h"
 
 using namespace igap;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
-static int fail(int code, const char *fmt, ...)
+int igap::fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -31,39 +31,6 @@ static int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(x)                                                                                  \
-    do {                                                                                           \
-        hipError_t e_ = (x);                                                                       \
-        if (e_ != hipSuccess) return fail(IGAP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));      \
-    } while (0)
-
-struct igap_gpu_ctx {
-    int device;
-    hipStream_t stream;
-    int nelem[3], na[3], nb[3], periodic[3];
-    int64_t ncp_active, ncp, ndof, nnz, nel;
-    int torder;
-    std::vector<double> knot_h[3];
-    PhysK P;
-    MeshK M;
-    // device
-    double *d_knot[3] = {nullptr, nullptr, nullptr};
-    double *d_w = nullptr, *d_X = nullptr;
-    double *d_Uh = nullptr, *d_R = nullptr, *d_Re = nullptr;
-    void *d_Ke[2] = {nullptr, nullptr};
-    int64_t *d_rowptr = nullptr;
-    int32_t *d_col = nullptr;
-    double *d_val64 = nullptr;
-    float *d_val32 = nullptr;
-    int32_t *d_mask = nullptr;
-    double *d_g = nullptr, *d_F = nullptr;
-    // timing
-    int timing = 0;
-    hipEvent_t ev[4];
-    double t_ms[5] = {0, 0, 0, 0, 0};
-    int64_t t_n[5] = {0, 0, 0, 0, 0};
-};
 
 // ---------------------------------------------------------------- small math
 static void gauss4(double *x, double *w)
@@ -521,6 +488,7 @@ extern "C" int igap_gpu_destroy(igap_gpu_ctx *c)
                     c->d_val32, c->d_mask, c->d_g, c->d_F};
     for (void *b : bufs)
         if (b) hipFree(b);
+    if (c->solver) solver_destroy(c->solver);
     for (int i = 0; i < 4; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -1,0 +1,51 @@
+// igap_ctx.h -- the context behind the opaque igap_gpu_ctx handle (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/igap_gpu.h"
+#include "igap_device.h"
+
+namespace igap {
+struct Solver;
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void solver_destroy(Solver *s);
+}  // namespace igap
+
+#define HIPCHK(x)                                                                                        \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) return igap::fail(IGAP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));      \
+    } while (0)
+
+struct igap_gpu_ctx {
+    int device;
+    hipStream_t stream;
+    int nelem[3], na[3], nb[3], periodic[3];
+    int64_t ncp_active, ncp, ndof, nnz, nel;
+    int torder;
+    std::vector<double> knot_h[3];
+    igap::PhysK P;
+    igap::MeshK M;
+    // device
+    double *d_knot[3] = {nullptr, nullptr, nullptr};
+    double *d_w = nullptr, *d_X = nullptr;
+    double *d_Uh = nullptr, *d_R = nullptr, *d_Re = nullptr;
+    void *d_Ke[2] = {nullptr, nullptr};
+    int64_t *d_rowptr = nullptr;
+    int32_t *d_col = nullptr;
+    double *d_val64 = nullptr;
+    float *d_val32 = nullptr;
+    int32_t *d_mask = nullptr;
+    double *d_g = nullptr, *d_F = nullptr;
+    // timing
+    int timing = 0;
+    hipEvent_t ev[4];
+    double t_ms[5] = {0, 0, 0, 0, 0};
+    int64_t t_n[5] = {0, 0, 0, 0, 0};
+    // Newton-Krylov solver state (created on first use)
+    igap::Solver *solver = nullptr;
+};

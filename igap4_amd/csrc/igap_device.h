@@ -70,13 +70,14 @@ __device__ __forceinline__ void bspline1d(const double *kV, double x, double *v,
     d2[2] = 2.0 * i42 * dn12;
 }
 
-// tensor-product jet of basis b = 9i+3j+k from 1-D tables t[d][order][i]
-__device__ __forceinline__ void tensor_jet(const double (*t)[3][3], int b, double *N)
+// tensor-product jet of basis b = 9i+3j+k from the 1-D tables of the three
+// directions, t_d[3*order + i] (value, first, second derivative)
+__device__ __forceinline__ void tensor_jet(const double *t0, const double *t1, const double *t2, int b, double *N)
 {
     int i = b / 9, j = (b / 3) % 3, k = b % 3;
-    double a0 = t[0][0][i], b0 = t[0][1][i], c0 = t[0][2][i];
-    double a1 = t[1][0][j], b1 = t[1][1][j], c1 = t[1][2][j];
-    double a2 = t[2][0][k], b2 = t[2][1][k], c2 = t[2][2][k];
+    double a0 = t0[i], b0 = t0[3 + i], c0 = t0[6 + i];
+    double a1 = t1[j], b1 = t1[3 + j], c1 = t1[6 + j];
+    double a2 = t2[k], b2 = t2[3 + k], c2 = t2[6 + k];
     N[0] = a0 * a1 * a2;
     N[1] = b0 * a1 * a2;
     N[2] = a0 * b1 * a2;
@@ -281,50 +282,50 @@ __device__ __forceinline__ void phys_residual_transient(const PhysK &P, const do
 // Entries of d2Psi/du2 at one state, by block type.  M = I - 11^T/3.
 __device__ __forceinline__ double Mdev(int a, int b) { return (a == b ? 2.0 : -1.0) / 3.0; }
 
+// The state as stored in LDS: F[0..8], H[9..35], S[36..44], G[45..53], N[54..62]
+constexpr int ST_F = 0, ST_H = 9, ST_S = 36, ST_G = 45, ST_N = 54, STW = 63;
+
 // first-gradient (k,L) x first-gradient (l,M)
-__device__ __forceinline__ double c_ff(const State &st, const PhysK &P, int k, int L, int l, int M)
+__device__ __forceinline__ double c_ff(const double *st, const PhysK &P, int k, int L, int l, int M)
 {
-    double v = st.N[3 * L + M] * st.F[3 * k + L] * st.F[3 * l + M];
+    const double *F = st + ST_F, *H = st + ST_H;
+    double v = st[ST_N + 3 * L + M] * F[3 * k + L] * F[3 * l + M];
     if (L == M) {
-        double t = 0;
-#pragma unroll
-        for (int J = 0; J < 3; J++)
-            if (J != L) t += st.F[3 * k + J] * st.F[3 * l + J];
+        double t = F[3 * k] * F[3 * l] + F[3 * k + 1] * F[3 * l + 1] + F[3 * k + 2] * F[3 * l + 2] - F[3 * k + L] * F[3 * l + L];
         v += 0.5 * P.e345 * t;
     } else {
-        v += 0.5 * P.e345 * st.F[3 * k + M] * st.F[3 * l + L];
+        v += 0.5 * P.e345 * F[3 * k + M] * F[3 * l + L];
     }
-    double hh = st.H[9 * k + 3 * L] * st.H[9 * l + 3 * M] + st.H[9 * k + 3 * L + 1] * st.H[9 * l + 3 * M + 1]
-              + st.H[9 * k + 3 * L + 2] * st.H[9 * l + 3 * M + 2];
+    double hh = H[9 * k + 3 * L] * H[9 * l + 3 * M] + H[9 * k + 3 * L + 1] * H[9 * l + 3 * M + 1] + H[9 * k + 3 * L + 2] * H[9 * l + 3 * M + 2];
     v += 2 * P.le * P.le * Mdev(L, M) * hh;
-    if (k == l) v += st.S[3 * L + M];
+    if (k == l) v += st[ST_S + 3 * L + M];
     return v;
 }
 
 // first-gradient (k,L) x second-gradient (l,(I,K))
-__device__ __forceinline__ double c_fh(const State &st, const PhysK &P, int k, int L, int l, int I, int K)
+__device__ __forceinline__ double c_fh(const double *st, const PhysK &P, int k, int L, int l, int I, int K)
 {
+    const double *F = st + ST_F, *H = st + ST_H, *G = st + ST_G;
     double g2 = 2 * P.le * P.le;
-    double v = g2 * Mdev(L, I) * st.H[9 * k + 3 * L + K] * st.F[3 * l + I];
-    if (I != K) v += g2 * Mdev(L, K) * st.H[9 * k + 3 * L + I] * st.F[3 * l + K];
+    double v = g2 * Mdev(L, I) * H[9 * k + 3 * L + K] * F[3 * l + I];
+    if (I != K) v += g2 * Mdev(L, K) * H[9 * k + 3 * L + I] * F[3 * l + K];
     if (k == l) {
-        if (L == I) v += st.G[3 * I + K];
-        if (I != K && L == K) v += st.G[3 * K + I];
+        if (L == I) v += G[3 * I + K];
+        if (I != K && L == K) v += G[3 * K + I];
     }
     return v;
 }
 
-// second-gradient (k,(I,K)) x second-gradient (l,(J,K2))
-__device__ __forceinline__ double c_hh(const State &st, const PhysK &P, int k, int I, int K, int l, int J, int K2)
+// second-gradient (k,(I,K)) x second-gradient (l,(J,K2)): each slot carries
+// the (component, gradient direction) pairs (I,K) and, when I != K, (K,I)
+__device__ __forceinline__ double c_hh(const double *st, const PhysK &P, int k, int I, int K, int l, int J, int K2)
 {
+    const double *F = st + ST_F;
     double v = 0;
-    // pairs (component, gradient direction) of each slot
-    int ai0 = I, ag0 = K, ai1 = K, ag1 = I, na_ = (I != K) ? 2 : 1;
-    int bi0 = J, bg0 = K2, bi1 = K2, bg1 = J, nb_ = (J != K2) ? 2 : 1;
-    if (ag0 == bg0) v += Mdev(ai0, bi0) * st.F[3 * k + ai0] * st.F[3 * l + bi0];
-    if (nb_ == 2 && ag0 == bg1) v += Mdev(ai0, bi1) * st.F[3 * k + ai0] * st.F[3 * l + bi1];
-    if (na_ == 2 && ag1 == bg0) v += Mdev(ai1, bi0) * st.F[3 * k + ai1] * st.F[3 * l + bi0];
-    if (na_ == 2 && nb_ == 2 && ag1 == bg1) v += Mdev(ai1, bi1) * st.F[3 * k + ai1] * st.F[3 * l + bi1];
+    if (K == K2) v += Mdev(I, J) * F[3 * k + I] * F[3 * l + J];
+    if (J != K2 && K == J) v += Mdev(I, K2) * F[3 * k + I] * F[3 * l + K2];
+    if (I != K && I == K2) v += Mdev(K, J) * F[3 * k + K] * F[3 * l + J];
+    if (I != K && J != K2 && I == J) v += Mdev(K, K2) * F[3 * k + K] * F[3 * l + K2];
     return 2 * P.le * P.le * v;
 }
 

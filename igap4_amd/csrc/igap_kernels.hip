@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
 {
     constexpr int STR = 4 + 3 * NH;   // X(3) w(1) U(3*NH) per control point
     __shared__ double sCP[4][27 * STR];
+    __shared__ double sTab[4][3][4][9];   // per wave: 1-D tables [d][q_d][3*order + i]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int e = blockIdx.x * 4 + wv;
     const bool valid = e < nel;
@@ -82,25 +83,32 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
 #pragma unroll
             for (int k = 0; k < 3; k++) cp[STR * lane + 4 + 3 * h + k] = Uh[h * ndof + 3 * oc + k];
     }
+    const int ed[3] = {ex, ey, ez};
+    if (lane >= 32 && lane < 44) {
+        int d = (lane - 32) >> 2, q = (lane - 32) & 3;
+        const double *kV = M.knot[d] + ed[d];
+        double c = M.gq[q];
+        double *tb = sTab[wv][d][q];
+        bspline1d(kV, (-c + 1.0) * kV[2] + c * kV[3], tb, tb + 3, tb + 6);
+    }
     __syncthreads();
 
     const int qd[3] = {lane >> 4, (lane >> 2) & 3, lane & 3};
-    const int ed[3] = {ex, ey, ez};
-    double t[3][3][3], xq[3], dX[3];
+    const double *t0 = sTab[wv][0][qd[0]], *t1 = sTab[wv][1][qd[1]], *t2 = sTab[wv][2][qd[2]];
+    double xq[3], dX[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double *kV = M.knot[d] + ed[d];
         double X0 = kV[2], X1 = kV[3], c = M.gq[qd[d]];
         xq[d] = (-c + 1.0) * X0 + c * X1;
         dX[d] = X1 - X0;
-        bspline1d(kV, xq[d], t[d][0], t[d][1], t[d][2]);
     }
     double W[10];
 #pragma unroll
     for (int a = 0; a < 10; a++) W[a] = 0.0;
     for (int b = 0; b < 27; b++) {
         double N[10];
-        tensor_jet(t, b, N);
+        tensor_jet(t0, t1, t2, b, N);
         double wb = cp[STR * b + 3];
 #pragma unroll
         for (int a = 0; a < 10; a++) W[a] += wb * N[a];
@@ -112,7 +120,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
         for (int a = 0; a < 30; a++) Xj[a] = 0.0;
         for (int b = 0; b < 27; b++) {
             double N[10], R[10];
-            tensor_jet(t, b, N);
+            tensor_jet(t0, t1, t2, b, N);
             nurbs_param(N, cp[STR * b + 3], W, R);
 #pragma unroll
             for (int d = 0; d < 3; d++)
@@ -138,7 +146,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
         for (int a = 0; a < 30; a++) u[a] = 0.0;
         for (int b = 0; b < 27; b++) {
             double N[10], R[10];
-            tensor_jet(t, b, N);
+            tensor_jet(t0, t1, t2, b, N);
             nurbs_param(N, cp[STR * b + 3], W, R);
             phys_jet(R, Gi, X2);
 #pragma unroll
@@ -160,7 +168,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
         for (int a = 0; a < 9; a++) uv[a] = 0.0;
         for (int b = 0; b < 27; b++) {
             double N[10], R[10];
-            tensor_jet(t, b, N);
+            tensor_jet(t0, t1, t2, b, N);
             nurbs_param(N, cp[STR * b + 3], W, R);
             phys_jet(R, Gi, X2);
 #pragma unroll
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
         for (int bb = 0; bb < 9; bb++) {
             const int b = 9 * grp + bb;
             double N[10], R[10];
-            tensor_jet(t, b, N);
+            tensor_jet(t0, t1, t2, b, N);
             nurbs_param(N, cp[STR * b + 3], W, R);
             phys_jet(R, Gi, X2);
 #pragma unroll
@@ -274,16 +282,17 @@ __global__ __launch_bounds__(256) void k_residual_gather(MeshK M, const double *
 //                        per point splitting the 27 basis functions
 //   stage B  (16 chunks of QC = 4 Gauss points)
 //     B1  physical basis jets R_q[10][27] -> LDS
-//     B2  pointwise tangent D_q = w_q C_q (30x30) -> LDS
+//     B2  pointwise tangent D_q = w_q C_q (30x30, the blocks i <= j) -> LDS
 //     B3  K^{ij} += sum_{q,a} R_q[a][b] * (sum_b' D_q[(i,a),(j,b')] R_q[b'][c])
-//         on V_MFMA_{F64,F32}_16X16X4: M = b, N = c (27 -> 32), K = (q,a);
-//         wave w owns the (b,c) tile pair (w>>1, w&1) of all nine (i,j)
-//         blocks and forms its B operand on the VALU from D and R.
+//         for the six blocks i <= j (K_e is symmetric; K^{ji} = K^{ij}^T is
+//         written by mirroring) on V_MFMA_{F64,F32}_16X16X4: M = b, N = c
+//         (27 -> 32), K = (q,a).  Wave w owns column tile w>>1 and blocks
+//         3(w&1)..3(w&1)+2 for both row tiles, so each B operand it forms on
+//         the VALU from D and R feeds two matrix-core products.  For the
+//         static physics the value slot a = 0 of D is zero and is skipped.
 // ============================================================================
 constexpr int QC = 4;
 constexpr int GEO = 38;   // per Gauss point: wgt, Gi[9], X2[18], W[10]
-constexpr int STW = 63;   // State: F[9] H[27] S[9] G[9] N[9]
-
 // Operands of the 16x16x4 products: lane l supplies A[l&15][l>>4] and
 // B[l>>4][l&15].  Result register r of lane l holds D[row(l>>4, r)][l&15];
 // the row map of each instruction is pinned by igap_gpu_selftest_mfma.
@@ -305,22 +314,19 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int kr, int r) { return 4 * kr + r; }
 };
 
-__device__ __forceinline__ void load_state(const double *s, State &st)
-{
-#pragma unroll
-    for (int i = 0; i < 9; i++) st.F[i] = s[i];
-#pragma unroll
-    for (int i = 0; i < 27; i++) st.H[i] = s[9 + i];
-#pragma unroll
-    for (int i = 0; i < 9; i++) { st.S[i] = s[36 + i]; st.G[i] = s[45 + i]; st.N[i] = s[54 + i]; }
-}
+__constant__ const int kBlkI[6] = {0, 0, 0, 1, 1, 2};
+__constant__ const int kBlkJ[6] = {0, 1, 2, 1, 2, 2};
 
 template <typename T, int NH>
-__global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
-                                                        T *__restrict__ Ke, int nel)
+__global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh,
+                                                                         int64_t ndof, T *__restrict__ Ke, int nel)
 {
     constexpr int STR = 4 + 3 * NH;
     constexpr int NNODE = (NH == 1) ? 1 : 8;
+    constexpr int A0 = (NH == 1) ? 1 : 0;     // first jet slot in the contraction
+    constexpr int NA = 10 - A0;               // jet slots per Gauss point in K
+    constexpr int KSTEPS = QC * NA / 4;       // 4-deep k-steps per chunk
+    static_assert((QC * NA) % 4 == 0, "chunk K dimension must be a multiple of 4");
     __shared__ double sT[3][4][3][3];          // 1-D tables [d][q_d][order][i]
     __shared__ double sCP[27 * STR];
     __shared__ double sGeo[64 * GEO];
@@ -364,49 +370,46 @@ __global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const
     {
         const int q = tid >> 2, part = tid & 3;
         const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
-        double t[3][3][3];
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int o = 0; o < 3; o++)
-#pragma unroll
-                for (int i = 0; i < 3; i++) t[d][o][i] = sT[d][qd[d]][o][i];
+        const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
         double W[10];
 #pragma unroll
         for (int a = 0; a < 10; a++) W[a] = 0.0;
         for (int b = part; b < 27; b += 4) {
             double N[10];
-            tensor_jet(t, b, N);
+            tensor_jet(t0, t1, t2, b, N);
             double wb = sCP[STR * b + 3];
 #pragma unroll
             for (int a = 0; a < 10; a++) W[a] += wb * N[a];
         }
 #pragma unroll
         for (int a = 0; a < 10; a++) { W[a] += __shfl_xor(W[a], 1, 64); W[a] += __shfl_xor(W[a], 2, 64); }
-        double Xj[30];
+        double Gi[9], X2[18], detG;
+        {
+            double Xj[30];
 #pragma unroll
-        for (int a = 0; a < 30; a++) Xj[a] = 0.0;
-        for (int b = part; b < 27; b += 4) {
-            double N[10], R[10];
-            tensor_jet(t, b, N);
-            nurbs_param(N, sCP[STR * b + 3], W, R);
+            for (int a = 0; a < 30; a++) Xj[a] = 0.0;
+            for (int b = part; b < 27; b += 4) {
+                double N[10], R[10];
+                tensor_jet(t0, t1, t2, b, N);
+                nurbs_param(N, sCP[STR * b + 3], W, R);
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+#pragma unroll
+                    for (int a = 0; a < 10; a++) Xj[10 * d + a] += sCP[STR * b + d] * R[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 30; a++) { Xj[a] += __shfl_xor(Xj[a], 1, 64); Xj[a] += __shfl_xor(Xj[a], 2, 64); }
+            double G[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) G[3 * i + j] = Xj[10 * i + 1 + j];
 #pragma unroll
             for (int d = 0; d < 3; d++)
 #pragma unroll
-                for (int a = 0; a < 10; a++) Xj[10 * d + a] += sCP[STR * b + d] * R[a];
+                for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = Xj[10 * d + 4 + s2];
+            detG = inv3(G, Gi);
         }
-#pragma unroll
-        for (int a = 0; a < 30; a++) { Xj[a] += __shfl_xor(Xj[a], 1, 64); Xj[a] += __shfl_xor(Xj[a], 2, 64); }
-        double G[9], Gi[9], X2[18];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) G[3 * i + j] = Xj[10 * i + 1 + j];
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int s = 0; s < 6; s++) X2[6 * d + s] = Xj[10 * d + 4 + s];
-        double detG = inv3(G, Gi);
         double xq[3], dX[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -421,7 +424,7 @@ __global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const
             for (int a = 0; a < 30; a++) u[a] = 0.0;
             for (int b = part; b < 27; b += 4) {
                 double N[10], R[10];
-                tensor_jet(t, b, N);
+                tensor_jet(t0, t1, t2, b, N);
                 nurbs_param(N, sCP[STR * b + 3], W, R);
                 phys_jet(R, Gi, X2);
 #pragma unroll
@@ -433,12 +436,7 @@ __global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const
             }
 #pragma unroll
             for (int a = 0; a < 30; a++) { u[a] += __shfl_xor(u[a], 1, 64); u[a] += __shfl_xor(u[a], 2, 64); }
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                u[10 * k] += M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
-#pragma unroll
-                for (int d = 0; d < 3; d++) u[10 * k + 1 + d] += M.par_periodic[3 * k + d];
-            }
+            add_periodic(M, xq, u);
             if (part == 0) {
 #pragma unroll
                 for (int a = 0; a < 30; a++) sJet[q * 30 * NH + 30 * h + a] = u[a];
@@ -458,10 +456,13 @@ __global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const
     __syncthreads();
 
     typedef typename Mfma<T>::acc_t acc_t;
-    acc_t acc[9];
+    acc_t acc[3][2];
 #pragma unroll
-    for (int i = 0; i < 9; i++) acc[i] = (acc_t){0, 0, 0, 0};
-    const int bt = wv >> 1, ctile = wv & 1, m = lane & 15, kr = lane >> 4;
+    for (int t = 0; t < 3; t++) { acc[t][0] = (acc_t){0, 0, 0, 0}; acc[t][1] = (acc_t){0, 0, 0, 0}; }
+    const int ctile = wv >> 1, blk0 = 3 * (wv & 1), m = lane & 15, kr = lane >> 4;
+    int bi[3], bj[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) { bi[t] = kBlkI[blk0 + t]; bj[t] = kBlkJ[blk0 + t]; }
 
     for (int chunk = 0; chunk < 64 / QC; chunk++) {
         const int q0 = chunk * QC;
@@ -470,16 +471,10 @@ __global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const
             int ql = tid >> 5, b = tid & 31, q = q0 + ql;
             if (b < 27) {
                 const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
-                double t[3][3][3];
-#pragma unroll
-                for (int d = 0; d < 3; d++)
-#pragma unroll
-                    for (int o = 0; o < 3; o++)
-#pragma unroll
-                        for (int i = 0; i < 3; i++) t[d][o][i] = sT[d][qd[d]][o][i];
+                const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
                 const double *g = sGeo + q * GEO;
                 double N[10], R[10];
-                tensor_jet(t, b, N);
+                tensor_jet(t0, t1, t2, b, N);
                 nurbs_param(N, sCP[STR * b + 3], g + 28, R);
                 phys_jet(R, g + 1, g + 10);
 #pragma unroll
@@ -504,92 +499,84 @@ __global__ __launch_bounds__(256, 1) void k_tangent_elem(MeshK M, PhysK P, const
             phys_state(us, P, st);
             double *s = sSt + (ql * NNODE + j) * STW;
 #pragma unroll
-            for (int i = 0; i < 9; i++) s[i] = st.F[i];
+            for (int i = 0; i < 9; i++) s[ST_F + i] = st.F[i];
 #pragma unroll
-            for (int i = 0; i < 27; i++) s[9 + i] = st.H[i];
+            for (int i = 0; i < 27; i++) s[ST_H + i] = st.H[i];
 #pragma unroll
-            for (int i = 0; i < 9; i++) { s[36 + i] = st.S[i]; s[45 + i] = st.G[i]; s[54 + i] = st.N[i]; }
+            for (int i = 0; i < 9; i++) { s[ST_S + i] = st.S[i]; s[ST_G + i] = st.G[i]; s[ST_N + i] = st.N[i]; }
         }
         __syncthreads();
-        // ---- B2b: D_q = w_q C_q, wave wv handles Gauss point q0 + wv
+        // ---- B2b: D_q = w_q C_q on the blocks i <= j; wave wv handles Gauss point q0 + wv
         {
             const int ql = wv, q = q0 + ql;
             const double wgt = sGeo[q * GEO];
+            const double *S0 = sSt + ql * NNODE * STW;
             T *D = sD[ql];
-            for (int f = lane; f < 81; f += 64) {
-                int k = f / 27, L = (f / 9) % 3, l = (f / 3) % 3, Mi = f % 3;
+            for (int f = lane; f < 54; f += 64) {
+                int p = f / 9, L = (f / 3) % 3, Mi = f % 3, k = kBlkI[p], l = kBlkJ[p];
                 double v = 0.0;
-                for (int j = 0; j < NNODE; j++) {
-                    State st;
-                    load_state(sSt + (ql * NNODE + j) * STW, st);
-                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_ff(st, P, k, L, l, Mi);
-                }
+                for (int j = 0; j < NNODE; j++) v += (NH == 1 ? 1.0 : P.ct[j]) * c_ff(S0 + j * STW, P, k, L, l, Mi);
                 D[(10 * k + 1 + L) * 30 + 10 * l + 1 + Mi] = (T)(wgt * v);
             }
             for (int f = lane; f < 162; f += 64) {
-                int k = f / 54, L = (f / 18) % 3, l = (f / 6) % 3, p = f % 6;
-                int I = kPairI[p], K = kPairK[p];
+                int k = f / 54, L = (f / 18) % 3, l = (f / 6) % 3, pp = f % 6;
+                int I = kPairI[pp], K = kPairK[pp];
                 double v = 0.0;
-                for (int j = 0; j < NNODE; j++) {
-                    State st;
-                    load_state(sSt + (ql * NNODE + j) * STW, st);
-                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_fh(st, P, k, L, l, I, K);
-                }
+                for (int j = 0; j < NNODE; j++) v += (NH == 1 ? 1.0 : P.ct[j]) * c_fh(S0 + j * STW, P, k, L, l, I, K);
                 T tv = (T)(wgt * v);
-                D[(10 * k + 1 + L) * 30 + 10 * l + 4 + p] = tv;
-                D[(10 * l + 4 + p) * 30 + 10 * k + 1 + L] = tv;
+                if (k <= l) D[(10 * k + 1 + L) * 30 + 10 * l + 4 + pp] = tv;
+                if (k >= l) D[(10 * l + 4 + pp) * 30 + 10 * k + 1 + L] = tv;
             }
-            for (int f = lane; f < 324; f += 64) {
-                int k = f / 108, p = (f / 18) % 6, l = (f / 6) % 3, p2 = f % 6;
+            for (int f = lane; f < 216; f += 64) {
+                int p = f / 36, pp = (f / 6) % 6, pp2 = f % 6, k = kBlkI[p], l = kBlkJ[p];
                 double v = 0.0;
-                for (int j = 0; j < NNODE; j++) {
-                    State st;
-                    load_state(sSt + (ql * NNODE + j) * STW, st);
-                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_hh(st, P, k, kPairI[p], kPairK[p], l, kPairI[p2], kPairK[p2]);
-                }
-                D[(10 * k + 4 + p) * 30 + 10 * l + 4 + p2] = (T)(wgt * v);
+                for (int j = 0; j < NNODE; j++)
+                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_hh(S0 + j * STW, P, k, kPairI[pp], kPairK[pp], l, kPairI[pp2], kPairK[pp2]);
+                D[(10 * k + 4 + pp) * 30 + 10 * l + 4 + pp2] = (T)(wgt * v);
             }
             if (NH != 1 && lane < 3)
                 D[(10 * lane) * 30 + 10 * lane] = (T)(wgt * (P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt)));
         }
         __syncthreads();
         // ---- B3: matrix-core contraction, K-dim = (q, a) of this chunk
-#pragma unroll 2
-        for (int s = 0; s < QC * 10 / 4; s++) {
-            const int kk = 4 * s + kr, ql = kk / 10, a = kk % 10;
-            const T A = sR[ql][a][16 * bt + m];
-            T Rc[10];
+#pragma unroll 3
+        for (int s = 0; s < KSTEPS; s++) {
+            const int kk = 4 * s + kr, ql = kk / NA, a = A0 + kk % NA;
+            const T Av0 = sR[ql][a][m], Av1 = sR[ql][a][16 + m];
+            T Rc[NA];
 #pragma unroll
-            for (int bb = 0; bb < 10; bb++) Rc[bb] = sR[ql][bb][16 * ctile + m];
-            const T *Drow = sD[ql] + a * 30;
+            for (int bb = 0; bb < NA; bb++) Rc[bb] = sR[ql][A0 + bb][16 * ctile + m];
+            const T *Drow = sD[ql] + a * 30 + A0;
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+            for (int t = 0; t < 3; t++) {
+                const T *d = Drow + 300 * bi[t] + 10 * bj[t];
+                T y = d[0] * Rc[0];
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const T *d = Drow + 300 * i + 10 * j;
-                    T y = d[0] * Rc[0];
-#pragma unroll
-                    for (int bb = 1; bb < 10; bb++) y += d[bb] * Rc[bb];
-                    acc[3 * i + j] = Mfma<T>::run(A, y, acc[3 * i + j]);
-                }
+                for (int bb = 1; bb < NA; bb++) y += d[bb] * Rc[bb];
+                acc[t][0] = Mfma<T>::run(Av0, y, acc[t][0]);
+                acc[t][1] = Mfma<T>::run(Av1, y, acc[t][1]);
+            }
         }
         __syncthreads();
     }
 
-    // ---- write K_e (row 3b+i, column 3c+j): acc[r] holds tile row Mfma<T>::row(kr, r), column m
+    // ---- write K_e (row 3b+i, column 3c+j) and the mirrored block (3c+j, 3b+i)
     T *out = Ke + (int64_t)e * 6561;
     const int c = 16 * ctile + m;
     if (c < 27) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int b = 16 * bt + Mfma<T>::row(kr, r);
-            if (b < 27) {
+        for (int t = 0; t < 3; t++)
 #pragma unroll
-                for (int i = 0; i < 3; i++)
+            for (int bt = 0; bt < 2; bt++)
 #pragma unroll
-                    for (int j = 0; j < 3; j++) out[(3 * b + i) * 81 + 3 * c + j] = acc[3 * i + j][r];
-            }
-        }
+                for (int r = 0; r < 4; r++) {
+                    const int b = 16 * bt + Mfma<T>::row(kr, r);
+                    if (b < 27) {
+                        T v = acc[t][bt][r];
+                        out[(3 * b + bi[t]) * 81 + 3 * c + bj[t]] = v;
+                        if (bi[t] != bj[t]) out[(3 * c + bj[t]) * 81 + 3 * b + bi[t]] = v;
+                    }
+                }
     }
 }
 

@@ -1,0 +1,523 @@
+// igap_solver.hip -- device-resident Newton-Krylov solve of the assembled system.
+//
+// The reference drives PETSc SNES NEWTONLS with a backtracking line search,
+// KSP GMRES (right preconditioned, modified Gram-Schmidt, restart 10000) and
+// PCASM/ILU(0) (src/interface.c:207-250), one SNESSolve per outer sub-
+// iteration until ||F|| < 1e-10 (src/interface.c:302-328).  On one GPU the
+// same algorithm runs with every vector in HBM:
+//   * Newton with the BT line search (Armijo on |F|^2/2, quadratic then cubic
+//     backtracking, lambda in [0.1, 0.5] of the previous step),
+//   * restarted GMRES(m), right preconditioned, classical Gram-Schmidt with
+//     one reorthogonalisation (two batched GEMVs per step instead of j
+//     dependent dot products; m is bounded because 10000 Krylov vectors of
+//     the large patches do not fit),
+//   * ILU(0) of the assembled tangent (rocSPARSE csrilu0 + csrsv) in the
+//     tangent's precision, FP64 or FP32,
+//   * a wave-per-row CSR SpMV of this file.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsparse/rocsparse.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "igap_ctx.h"
+#include "igap_launch.h"
+
+namespace igap {
+
+struct Solver {
+    int m = 0;                      // GMRES restart
+    int64_t n = 0;
+    rocsparse_handle sp = nullptr;
+    rocsparse_mat_descr dA = nullptr, dL = nullptr, dU = nullptr;
+    rocsparse_mat_info info[2] = {nullptr, nullptr};   // per precision
+    void *ilu[2] = {nullptr, nullptr};                 // factor values (double / float)
+    void *buf[2] = {nullptr, nullptr};
+    int analysed[2] = {0, 0};
+    int32_t *rowptr32 = nullptr;
+    rocblas_handle bl = nullptr;
+    double *V = nullptr;            // (m+1) x n, column j = Krylov vector j
+    double *w = nullptr, *z = nullptr, *t = nullptr, *d = nullptr, *x = nullptr, *xt = nullptr, *F = nullptr, *Ft = nullptr;
+    double *h = nullptr;            // m+1 device
+    float *fa = nullptr, *fb = nullptr;
+    double *diag = nullptr;         // Jacobi
+};
+
+void solver_destroy(Solver *s)
+{
+    if (!s) return;
+    if (s->sp) {
+        for (int p = 0; p < 2; p++)
+            if (s->info[p]) rocsparse_destroy_mat_info(s->info[p]);
+        if (s->dA) rocsparse_destroy_mat_descr(s->dA);
+        if (s->dL) rocsparse_destroy_mat_descr(s->dL);
+        if (s->dU) rocsparse_destroy_mat_descr(s->dU);
+        rocsparse_destroy_handle(s->sp);
+    }
+    if (s->bl) rocblas_destroy_handle(s->bl);
+    void *bufs[] = {s->ilu[0], s->ilu[1], s->buf[0], s->buf[1], s->rowptr32, s->V, s->w, s->z, s->t, s->d, s->x,
+                    s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    delete s;
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                              const T *__restrict__ val, const double *__restrict__ x, double *__restrict__ y)
+{
+    int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) s += (double)val[j] * x[col[j]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[row] = s;
+}
+
+template <typename T>
+__global__ void k_diag_inv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const T *__restrict__ val,
+                           double *__restrict__ dinv)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double dv = 1.0;
+    for (int64_t j = rp[r]; j < rp[r + 1]; j++)
+        if (col[j] == r) dv = (double)val[j];
+    dinv[r] = dv != 0.0 ? 1.0 / dv : 1.0;
+}
+
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void k_waxpy(int64_t n, double a, const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ w)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) w[i] = y[i] + a * x[i];
+}
+
+__global__ void k_mul(int64_t n, const double *__restrict__ a, const double *__restrict__ x, double *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) y[i] = a[i] * x[i];
+}
+
+__global__ void k_d2f(int64_t n, const double *__restrict__ x, float *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) y[i] = (float)x[i];
+}
+
+__global__ void k_f2d(int64_t n, const float *__restrict__ x, double *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) y[i] = (double)x[i];
+}
+
+__global__ void k_i64_to_i32(int64_t n, const int64_t *__restrict__ a, int32_t *__restrict__ b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (int32_t)a[i];
+}
+
+static inline dim3 grid1(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+#define SPCHK(x)                                                                     \
+    do {                                                                             \
+        rocsparse_status s_ = (x);                                                   \
+        if (s_ != rocsparse_status_success) return fail(IGAP_ERR_HIP, "%s: rocsparse status %d", #x, (int)s_); \
+    } while (0)
+#define BLCHK(x)                                                                     \
+    do {                                                                             \
+        rocblas_status s_ = (x);                                                     \
+        if (s_ != rocblas_status_success) return fail(IGAP_ERR_HIP, "%s: rocblas status %d", #x, (int)s_); \
+    } while (0)
+
+}  // namespace igap
+
+using namespace igap;
+
+// ---------------------------------------------------------------- set-up
+static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
+{
+    Solver *s = c->solver;
+    if (s && s->m != m) {
+        solver_destroy(s);
+        c->solver = s = nullptr;
+    }
+    if (!s) {
+        s = new Solver();
+        c->solver = s;
+        s->m = m;
+        s->n = c->ndof;
+        int64_t n = c->ndof;
+        if (c->nnz > 0x7fffffffLL || n > 0x7fffffffLL)
+            return fail(IGAP_ERR_ARG, "ILU(0) through rocSPARSE needs nnz < 2^31 (nnz = %lld)", (long long)c->nnz);
+        HIPCHK(hipMalloc(&s->V, sizeof(double) * (size_t)(m + 1) * n));
+        double **vecs[] = {&s->w, &s->z, &s->t, &s->d, &s->x, &s->xt, &s->F, &s->Ft, &s->diag};
+        for (double **v : vecs) HIPCHK(hipMalloc(v, sizeof(double) * n));
+        HIPCHK(hipMalloc(&s->h, sizeof(double) * (m + 2)));
+        HIPCHK(hipMalloc(&s->fa, sizeof(float) * n));
+        HIPCHK(hipMalloc(&s->fb, sizeof(float) * n));
+        HIPCHK(hipMalloc(&s->rowptr32, sizeof(int32_t) * (n + 1)));
+        hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, n + 1, c->d_rowptr,
+                           s->rowptr32);
+        HIPCHK(hipGetLastError());
+        SPCHK(rocsparse_create_handle(&s->sp));
+        SPCHK(rocsparse_set_stream(s->sp, c->stream));
+        SPCHK(rocsparse_create_mat_descr(&s->dA));
+        SPCHK(rocsparse_create_mat_descr(&s->dL));
+        SPCHK(rocsparse_set_mat_fill_mode(s->dL, rocsparse_fill_mode_lower));
+        SPCHK(rocsparse_set_mat_diag_type(s->dL, rocsparse_diag_type_unit));
+        SPCHK(rocsparse_create_mat_descr(&s->dU));
+        SPCHK(rocsparse_set_mat_fill_mode(s->dU, rocsparse_fill_mode_upper));
+        SPCHK(rocsparse_set_mat_diag_type(s->dU, rocsparse_diag_type_non_unit));
+        BLCHK(rocblas_create_handle(&s->bl));
+        BLCHK(rocblas_set_stream(s->bl, c->stream));
+        BLCHK(rocblas_set_pointer_mode(s->bl, rocblas_pointer_mode_host));
+    }
+    *out = s;
+    return IGAP_OK;
+}
+
+// ILU(0) numeric factorisation of the current tangent values in precision p
+static int pc_ilu_setup(igap_gpu_ctx *c, Solver *s, int p)
+{
+    int n = (int)c->ndof, nnz = (int)c->nnz;
+    if (!s->ilu[p]) {
+        size_t es = p == IGAP_F64 ? sizeof(double) : sizeof(float);
+        HIPCHK(hipMalloc(&s->ilu[p], es * (size_t)nnz));
+    }
+    if (p == IGAP_F64) HIPCHK(hipMemcpyAsync(s->ilu[p], c->d_val64, sizeof(double) * nnz, hipMemcpyDeviceToDevice, c->stream));
+    else HIPCHK(hipMemcpyAsync(s->ilu[p], c->d_val32, sizeof(float) * nnz, hipMemcpyDeviceToDevice, c->stream));
+    if (!s->analysed[p]) {
+        SPCHK(rocsparse_create_mat_info(&s->info[p]));
+        size_t b0 = 0, b1 = 0, b2 = 0;
+        if (p == IGAP_F64) {
+            double *v = (double *)s->ilu[p];
+            SPCHK(rocsparse_dcsrilu0_buffer_size(s->sp, n, nnz, s->dA, v, s->rowptr32, c->d_col, s->info[p], &b0));
+            SPCHK(rocsparse_dcsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->rowptr32, c->d_col, s->info[p], &b1));
+            SPCHK(rocsparse_dcsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->rowptr32, c->d_col, s->info[p], &b2));
+        } else {
+            float *v = (float *)s->ilu[p];
+            SPCHK(rocsparse_scsrilu0_buffer_size(s->sp, n, nnz, s->dA, v, s->rowptr32, c->d_col, s->info[p], &b0));
+            SPCHK(rocsparse_scsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->rowptr32, c->d_col, s->info[p], &b1));
+            SPCHK(rocsparse_scsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->rowptr32, c->d_col, s->info[p], &b2));
+        }
+        size_t bs = b0 > b1 ? b0 : b1;
+        bs = bs > b2 ? bs : b2;
+        HIPCHK(hipMalloc(&s->buf[p], bs > 0 ? bs : 1));
+        if (p == IGAP_F64) {
+            double *v = (double *)s->ilu[p];
+            SPCHK(rocsparse_dcsrilu0_analysis(s->sp, n, nnz, s->dA, v, s->rowptr32, c->d_col, s->info[p], rocsparse_analysis_policy_reuse,
+                                              rocsparse_solve_policy_auto, s->buf[p]));
+            SPCHK(rocsparse_dcsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->rowptr32, c->d_col, s->info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->buf[p]));
+            SPCHK(rocsparse_dcsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->rowptr32, c->d_col, s->info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->buf[p]));
+        } else {
+            float *v = (float *)s->ilu[p];
+            SPCHK(rocsparse_scsrilu0_analysis(s->sp, n, nnz, s->dA, v, s->rowptr32, c->d_col, s->info[p], rocsparse_analysis_policy_reuse,
+                                              rocsparse_solve_policy_auto, s->buf[p]));
+            SPCHK(rocsparse_scsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->rowptr32, c->d_col, s->info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->buf[p]));
+            SPCHK(rocsparse_scsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->rowptr32, c->d_col, s->info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->buf[p]));
+        }
+        s->analysed[p] = 1;
+    }
+    if (p == IGAP_F64)
+        SPCHK(rocsparse_dcsrilu0(s->sp, n, nnz, s->dA, (double *)s->ilu[p], s->rowptr32, c->d_col, s->info[p], rocsparse_solve_policy_auto,
+                                 s->buf[p]));
+    else
+        SPCHK(rocsparse_scsrilu0(s->sp, n, nnz, s->dA, (float *)s->ilu[p], s->rowptr32, c->d_col, s->info[p], rocsparse_solve_policy_auto,
+                                 s->buf[p]));
+    rocsparse_int zp = -1;
+    rocsparse_status st = rocsparse_csrilu0_zero_pivot(s->sp, s->info[p], &zp);
+    if (st == rocsparse_status_zero_pivot) return fail(IGAP_ERR_STATE, "ILU(0): zero pivot in row %d", (int)zp);
+    return IGAP_OK;
+}
+
+// y = M^{-1} x (x and y must differ from s->t)
+static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, const double *x, double *y)
+{
+    int64_t n = c->ndof;
+    if (pc == IGAP_PC_JACOBI) {
+        hipLaunchKernelGGL(k_mul, grid1(n), dim3(256), 0, c->stream, n, s->diag, x, y);
+        HIPCHK(hipGetLastError());
+        return IGAP_OK;
+    }
+    if (pc == IGAP_PC_NONE) {
+        HIPCHK(hipMemcpyAsync(y, x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        return IGAP_OK;
+    }
+    int nn = (int)n, nnz = (int)c->nnz;
+    if (p == IGAP_F64) {
+        const double one = 1.0;
+        SPCHK(rocsparse_dcsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dL, (double *)s->ilu[p], s->rowptr32, c->d_col,
+                                     s->info[p], x, s->t, rocsparse_solve_policy_auto, s->buf[p]));
+        SPCHK(rocsparse_dcsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dU, (double *)s->ilu[p], s->rowptr32, c->d_col,
+                                     s->info[p], s->t, y, rocsparse_solve_policy_auto, s->buf[p]));
+    } else {
+        const float one = 1.0f;
+        hipLaunchKernelGGL(k_d2f, grid1(n), dim3(256), 0, c->stream, n, x, s->fa);
+        SPCHK(rocsparse_scsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dL, (float *)s->ilu[p], s->rowptr32, c->d_col,
+                                     s->info[p], s->fa, s->fb, rocsparse_solve_policy_auto, s->buf[p]));
+        SPCHK(rocsparse_scsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dU, (float *)s->ilu[p], s->rowptr32, c->d_col,
+                                     s->info[p], s->fb, s->fa, rocsparse_solve_policy_auto, s->buf[p]));
+        hipLaunchKernelGGL(k_f2d, grid1(n), dim3(256), 0, c->stream, n, s->fa, y);
+        HIPCHK(hipGetLastError());
+    }
+    return IGAP_OK;
+}
+
+static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y)
+{
+    int64_t n = c->ndof;
+    dim3 g((unsigned)((n + 3) / 4));
+    if (p == IGAP_F64) hipLaunchKernelGGL(k_spmv<double>, g, dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, x, y);
+    else hipLaunchKernelGGL(k_spmv<float>, g, dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val32, x, y);
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
+static int nrm2(igap_gpu_ctx *c, Solver *s, const double *x, double *out)
+{
+    BLCHK(rocblas_dnrm2(s->bl, (rocblas_int)c->ndof, x, 1, out));
+    return IGAP_OK;
+}
+
+// GMRES(m), right preconditioned: solve A d = b (d starts at 0)
+static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres)
+{
+    const int64_t n = c->ndof;
+    const int m = s->m;
+    const rocblas_int N = (rocblas_int)n;
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hv(m + 2), y(m);
+    HIPCHK(launch_fill_f64(d, 0.0, n, c->stream));
+    double bnorm = 0;
+    int rc = nrm2(c, s, b, &bnorm);
+    if (rc) return rc;
+    *its = 0;
+    *relres = 0;
+    if (bnorm == 0.0) return IGAP_OK;
+    double rnorm = bnorm;
+    int total = 0;
+    // r = b - A d; first cycle d = 0 so r = b
+    HIPCHK(hipMemcpyAsync(s->w, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    while (total < o->ksp_maxit) {
+        double beta = rnorm;
+        // v0 = r / beta
+        hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0 / beta, s->w, 0.0, s->V);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && total < o->ksp_maxit; j++) {
+            double *vj = s->V + (size_t)j * n, *vn = s->V + (size_t)(j + 1) * n;
+            rc = pc_apply(c, s, o->pc, p, vj, s->z);
+            if (rc) return rc;
+            rc = spmv(c, p, s->z, vn);
+            if (rc) return rc;
+            // classical Gram-Schmidt, twice
+            const double one = 1.0, zero = 0.0, mone = -1.0;
+            for (int k = 0; k <= j; k++) hv[k] = 0.0;
+            for (int pass = 0; pass < 2; pass++) {
+                BLCHK(rocblas_dgemv(s->bl, rocblas_operation_transpose, N, j + 1, &one, s->V, N, vn, 1, &zero, s->h, 1));
+                BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j + 1, &mone, s->V, N, s->h, 1, &one, vn, 1));
+                std::vector<double> hp(j + 1);
+                HIPCHK(hipMemcpyAsync(hp.data(), s->h, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                for (int k = 0; k <= j; k++) hv[k] += hp[k];
+            }
+            double hn = 0;
+            rc = nrm2(c, s, vn, &hn);
+            if (rc) return rc;
+            if (hn > 0) hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0 / hn, vn, 0.0, vn);
+            // Givens
+            for (int k = 0; k < j; k++) {
+                double t = cs[k] * hv[k] + sn[k] * hv[k + 1];
+                hv[k + 1] = -sn[k] * hv[k] + cs[k] * hv[k + 1];
+                hv[k] = t;
+            }
+            double den = std::sqrt(hv[j] * hv[j] + hn * hn);
+            cs[j] = den > 0 ? hv[j] / den : 1.0;
+            sn[j] = den > 0 ? hn / den : 0.0;
+            hv[j] = den;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = hv[k];
+            total++;
+            rnorm = std::fabs(g[j + 1]);
+            if (rnorm <= o->ksp_rtol * bnorm || rnorm <= o->ksp_atol || hn == 0.0) { j++; break; }
+        }
+        // y = H^{-1} g ; d += M^{-1} V y
+        for (int k = j - 1; k >= 0; k--) {
+            double t = g[k];
+            for (int l = k + 1; l < j; l++) t -= H[(size_t)k * m + l] * y[l];
+            y[k] = t / H[(size_t)k * m + k];
+        }
+        HIPCHK(hipMemcpyAsync(s->h, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
+        const double one = 1.0, zero = 0.0;
+        BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j, &one, s->V, N, s->h, 1, &zero, s->w, 1));
+        rc = pc_apply(c, s, o->pc, p, s->w, s->z);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0, s->z, 1.0, d);
+        if (rnorm <= o->ksp_rtol * bnorm || rnorm <= o->ksp_atol) break;
+        // true residual for the restart: w = b - A d
+        rc = spmv(c, p, d, s->w);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0, b, -1.0, s->w);
+        rc = nrm2(c, s, s->w, &rnorm);
+        if (rc) return rc;
+        if (rnorm <= o->ksp_rtol * bnorm) break;
+    }
+    *its = total;
+    *relres = rnorm / bnorm;
+    return IGAP_OK;
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" int igap_gpu_solver_default_opts(igap_solver_opts *o)
+{
+    if (!o) return fail(IGAP_ERR_ARG, "null argument");
+    std::memset(o, 0, sizeof *o);
+    o->snes_atol = 1e-10;      // SNESSetTolerances(abstol 1e-10, ...) src/interface.c:220
+    o->snes_rtol = 0.0;
+    o->snes_maxit = 50;
+    o->ksp_rtol = 1e-5;        // PETSc default rtol (KSPSetTolerances(PETSC_DEFAULT,...) :226)
+    o->ksp_atol = 1e-50;
+    o->ksp_maxit = 2000;
+    o->ksp_restart = 200;
+    o->pc = IGAP_PC_ILU0;
+    o->tangent_prec = IGAP_F64;
+    o->linesearch = 1;
+    return IGAP_OK;
+}
+
+extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, igap_solver_stats *st)
+{
+    using clk = std::chrono::steady_clock;
+    if (!c || !o || !st) return fail(IGAP_ERR_ARG, "null argument");
+    if (o->ksp_restart < 1 || o->ksp_restart > 2000) return fail(IGAP_ERR_ARG, "ksp_restart %d out of [1, 2000]", o->ksp_restart);
+    if (o->tangent_prec != IGAP_F64 && o->tangent_prec != IGAP_F32) return fail(IGAP_ERR_ARG, "tangent_prec");
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(st, 0, sizeof *st);
+    auto T0 = clk::now();
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    Solver *s = nullptr;
+    int rc = solver_get(c, o->ksp_restart, &s);
+    if (rc) return rc;
+    const int64_t n = c->ndof;
+    const int p = o->tangent_prec;
+    if (p == IGAP_F64 && !igap_gpu_csr_val64(c)) return IGAP_ERR_NOMEM;
+    if (p == IGAP_F32 && !igap_gpu_csr_val32(c)) return IGAP_ERR_NOMEM;
+    // x = current iterate (U_hist level 0, the reference's solv->U)
+    HIPCHK(hipMemcpyAsync(s->x, c->d_Uh, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    auto t = clk::now();
+    rc = igap_gpu_residual(c, s->x, s->F);
+    if (rc) return rc;
+    double fnorm = 0;
+    rc = nrm2(c, s, s->F, &fnorm);
+    if (rc) return rc;
+    st->t_residual_ms += ms(t, clk::now());
+    st->n_residual++;
+    st->fnorm0 = fnorm;
+    st->converged = 0;
+    for (int it = 0;; it++) {
+        st->fnorm = fnorm;
+        if (!std::isfinite(fnorm)) { st->converged = -4; break; }   // DIVERGED_FNORM_NAN
+        if (fnorm <= o->snes_atol) { st->converged = 2; break; }    // CONVERGED_FNORM_ABS
+        if (o->snes_rtol > 0 && fnorm <= o->snes_rtol * st->fnorm0) { st->converged = 3; break; }   // CONVERGED_FNORM_RELATIVE
+        if (it >= o->snes_maxit) { st->converged = -5; break; }     // DIVERGED_MAX_IT
+        // tangent + preconditioner
+        t = clk::now();
+        rc = igap_gpu_tangent(c, s->x, nullptr, p);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        st->t_tangent_ms += ms(t, clk::now());
+        t = clk::now();
+        if (o->pc == IGAP_PC_ILU0) {
+            rc = pc_ilu_setup(c, s, p);
+            if (rc) return rc;
+        } else if (o->pc == IGAP_PC_JACOBI) {
+            if (p == IGAP_F64) hipLaunchKernelGGL(k_diag_inv<double>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, s->diag);
+            else hipLaunchKernelGGL(k_diag_inv<float>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val32, s->diag);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        st->t_pc_ms += ms(t, clk::now());
+        // solve K d = -F
+        t = clk::now();
+        hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, -1.0, s->F, 0.0, s->Ft);
+        int kits = 0;
+        double relres = 0;
+        rc = gmres(c, s, o, p, s->Ft, s->d, &kits, &relres);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        st->t_ksp_ms += ms(t, clk::now());
+        st->ksp_its += kits;
+        st->last_ksp_relres = relres;
+        // line search
+        t = clk::now();
+        double lambda = 1.0, f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
+        double lprev = 0, fprev = 0, ftrial = 0, fn_trial = 0;
+        for (int ls = 0;; ls++) {
+            hipLaunchKernelGGL(k_waxpy, grid1(n), dim3(256), 0, c->stream, n, lambda, s->d, s->x, s->xt);
+            rc = igap_gpu_residual(c, s->xt, s->Ft);
+            if (rc) return rc;
+            st->n_residual++;
+            rc = nrm2(c, s, s->Ft, &fn_trial);
+            if (rc) return rc;
+            ftrial = 0.5 * fn_trial * fn_trial;
+            if (!o->linesearch || (std::isfinite(ftrial) && ftrial <= f0 + 1e-4 * lambda * slope) || lambda < 1e-12 || ls >= 40) break;
+            double lnew;
+            if (ls == 0 || !std::isfinite(ftrial)) {
+                lnew = std::isfinite(ftrial) ? -slope / (2.0 * (ftrial - f0 - slope)) : 0.1;
+            } else {
+                double t1 = ftrial - f0 - lambda * slope, t2 = fprev - f0 - lprev * slope;
+                double a = (t1 / (lambda * lambda) - t2 / (lprev * lprev)) / (lambda - lprev);
+                double b = (-lprev * t1 / (lambda * lambda) + lambda * t2 / (lprev * lprev)) / (lambda - lprev);
+                if (a == 0.0) lnew = -slope / (2.0 * b);
+                else {
+                    double disc = b * b - 3.0 * a * slope;
+                    lnew = disc < 0 ? 0.5 * lambda : (-b + std::sqrt(disc)) / (3.0 * a);
+                }
+            }
+            if (!(lnew == lnew)) lnew = 0.5 * lambda;
+            if (lnew > 0.5 * lambda) lnew = 0.5 * lambda;
+            if (lnew < 0.1 * lambda) lnew = 0.1 * lambda;
+            lprev = lambda;
+            fprev = ftrial;
+            lambda = lnew;
+            st->ls_backtracks++;
+        }
+        std::swap(s->x, s->xt);
+        std::swap(s->F, s->Ft);
+        fnorm = fn_trial;
+        st->t_residual_ms += ms(t, clk::now());
+        st->newton_its++;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_Uh, s->x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_R, s->F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    st->t_total_ms = ms(T0, clk::now());
+    return IGAP_OK;
+}

@@ -135,6 +135,37 @@ double *igap_gpu_rvec(igap_gpu_ctx *ctx);
  * vals may be NULL for the context's own arrays. */
 int igap_gpu_tangent(igap_gpu_ctx *ctx, const double *U, void *vals, int prec);
 
+/* ---- Newton-Krylov solve (IGAPSolvInit / IGAPSolvSolve, src/interface.c:
+ * 207-250, 302-328: SNES NEWTONLS with BT line search, KSP GMRES right
+ * preconditioned, PC ILU(0)) with every vector in HBM. ---------------------- */
+enum { IGAP_PC_ILU0 = 0, IGAP_PC_JACOBI = 1, IGAP_PC_NONE = 2 };
+
+typedef struct {
+    double snes_atol;     /* stop when ||R|| <= snes_atol (reference: 1e-10, interface.c:220) */
+    double snes_rtol;     /* and/or when ||R|| <= snes_rtol ||R0|| (0 disables)             */
+    int    snes_maxit;
+    double ksp_rtol;      /* GMRES relative tolerance (PETSc default 1e-5)                  */
+    double ksp_atol;
+    int    ksp_maxit;     /* Krylov iterations per Newton step                              */
+    int    ksp_restart;   /* GMRES restart (reference 10000; bounded to fit HBM)            */
+    int    pc;            /* IGAP_PC_ILU0 (the reference's ASM/ILU(0) on one part), ...     */
+    int    tangent_prec;  /* IGAP_F64 or IGAP_F32 (FP32 matrix-core tangent, FP32 ILU)      */
+    int    linesearch;    /* 1: backtracking (SNESLINESEARCHBT, cubic), 0: full steps       */
+} igap_solver_opts;
+
+typedef struct {
+    int    newton_its, ksp_its, ls_backtracks, n_residual;
+    int    converged;     /* > 0 converged (2 abs, 3 rel), < 0 diverged (-4 NaN, -5 max it) */
+    double fnorm0, fnorm, last_ksp_relres;
+    double t_residual_ms, t_tangent_ms, t_pc_ms, t_ksp_ms, t_total_ms;
+} igap_solver_stats;
+
+int igap_gpu_solver_default_opts(igap_solver_opts *opts);
+/* Solves R(U) = 0 starting from U_hist level 0 (the reference keeps solv->U
+ * between solves, src/interface.c:252-259,319); on return U_hist level 0 holds
+ * the last iterate and the context's R buffer its residual.  Synchronous. */
+int igap_gpu_snes_solve(igap_gpu_ctx *ctx, const igap_solver_opts *opts, igap_solver_stats *stats);
+
 /* ---- helpers ------------------------------------------------------------ */
 int igap_gpu_synchronize(igap_gpu_ctx *ctx);
 int igap_gpu_malloc(igap_gpu_ctx *ctx, void **ptr, int64_t bytes);

@@ -1,0 +1,73 @@
+"""Device Newton-Krylov solve (SNES NEWTONLS / GMRES / ILU(0) on the GPU):
+converges to the reference tolerance, reaches the same displacements as a
+CPU Newton on the oracle's residual and tangent, and the FP32 matrix-core
+tangent reaches the FP64 answer."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+pytestmark = pytest.mark.gpu
+
+NEL = (8, 2, 2)
+TIP = 0.02
+
+
+def cpu_newton(oracle, mesh, ph, bc, U0, tol=1e-13, maxit=30):
+    """Newton on the oracle (exact sparse LU solves): the reference answer."""
+    U = U0.copy()
+    pat = oracle.csr_pattern(mesh)
+    R = oracle.residual(mesh, ph, bc, U)
+    r0 = np.linalg.norm(R)
+    for _ in range(maxit):
+        if np.linalg.norm(R) <= tol * max(r0, 1e-300):
+            break
+        rp, col, val = oracle.tangent(mesh, ph, bc, U, pattern=pat)
+        K = sp.csr_matrix((val, col, rp), shape=(mesh.ndof, mesh.ndof))
+        U = U - spla.spsolve(K.tocsc(), R)
+        R = oracle.residual(mesh, ph, bc, U)
+    return U, np.linalg.norm(R) / r0
+
+
+@pytest.fixture(scope="module")
+def problem(igap, oracle):
+    mg = igap.MeshIGABeam(L=(4, 1, 1), nelem=NEL)
+    mo = oracle.Mesh(NEL, L=(4, 1, 1))
+    return mg, mo
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_newton_matches_cpu_newton(igap, oracle, problem, prec):
+    mg, mo = problem
+    ph_o = oracle.make_phys(0, coef=oracle.SVK_COEF)
+    bc_o = oracle.clamped_tip_bc(tip=TIP)
+    U_ref, rel = cpu_newton(oracle, mo, ph_o, bc_o, np.zeros(mo.ndof))
+    assert rel < 1e-12
+    with igap.Assm(mg, igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCClampedTip(tip=TIP)) as A:
+        A.set_uhist(np.zeros(A.ndof))
+        st = A.snes_solve(snes_rtol=1e-10, snes_atol=1e-30, tangent_prec=prec)
+        U = A.get_uhist()
+    print(f"[newton {prec}] its={st['newton_its']} ksp={st['ksp_its']} |F|/|F0|={st['fnorm'] / st['fnorm0']:.2e}")
+    assert st["converged"] > 0 and st["fnorm"] <= 1e-10 * st["fnorm0"]
+    err = np.max(np.abs(U - U_ref)) / np.max(np.abs(U_ref))
+    print(f"  displacement rel err vs CPU Newton {err:.2e}")
+    assert err < 1e-8
+
+
+def test_load_steps_reference_criterion(igap):
+    """Config-1 shape (16x4x4 beam, tip 0.05 in 10 load steps), each load step
+    to the reference's absolute criterion ||R|| < 1e-10."""
+    from igap4_amd.loadstep import run_load_steps
+    m = igap.MeshIGABeam(L=(4, 1, 1), nelem=(16, 4, 4))
+    with igap.Assm(m, igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCClampedTip(tip=0.0)) as A:
+        A.set_uhist(np.zeros(A.ndof))
+        opts = igap.assm.default_solver_opts(snes_atol=1e-10, snes_rtol=0.0)
+        steps = run_load_steps(A, 10, tip_total=0.05, opts=opts)
+        U = A.get_uhist()
+    for s in steps:
+        print(f"  step {s['step']}: its={s['newton_its']} ksp={s['ksp_its']} |F|={s['fnorm']:.2e} {s['wall_ms']:.1f} ms")
+    assert len(steps) == 10 and all(s["converged"] > 0 for s in steps)
+    # the tip face carries the full displacement
+    nb = m.nb
+    Uz = U.reshape(nb[0], nb[1], nb[2], 3)[..., 2]
+    assert np.allclose(Uz[-1], 0.05 * (1 - 1e-15), atol=1e-12)
