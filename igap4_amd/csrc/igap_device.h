@@ -329,6 +329,79 @@ __device__ __forceinline__ double c_hh(const double *st, const PhysK &P, int k, 
     return 2 * P.le * P.le * v;
 }
 
+// One row of the (k,l) block of C at one state, accumulated as row[b - A0] +=
+// s * C[(k,a),(l,b)] for the slots b = A0..9.  The row slot a (1..9) is the
+// lane's; all column slots are unrolled at compile time.  Value slots (a or
+// b = 0) carry no stored-energy term.
+template <int A0>
+__device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, int a, int l, double s, double *row)
+{
+    const double *F = st + ST_F, *H = st + ST_H;
+    const double g2 = 2 * P.le * P.le;
+    const double Fl0 = F[3 * l], Fl1 = F[3 * l + 1], Fl2 = F[3 * l + 2];
+    const double Fl[3] = {Fl0, Fl1, Fl2};
+    if (a >= 1 && a <= 3) {
+        const int L = a - 1;
+        const double Fk0 = F[3 * k], Fk1 = F[3 * k + 1], Fk2 = F[3 * k + 2];
+        const double FkL = F[3 * k + L], FlL = F[3 * l + L];
+        const double hk0 = H[9 * k + 3 * L], hk1 = H[9 * k + 3 * L + 1], hk2 = H[9 * k + 3 * L + 2];
+        const double fdot = Fk0 * Fl0 + Fk1 * Fl1 + Fk2 * Fl2;
+        const double *NL = st + ST_N + 3 * L, *SL = st + ST_S + 3 * L;
+        const bool dia = (k == l);
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            const double FkM = (M == 0) ? Fk0 : (M == 1 ? Fk1 : Fk2);
+            double v = NL[M] * FkL * Fl[M];
+            v += 0.5 * P.e345 * ((M == L) ? (fdot - FkL * FlL) : FkM * FlL);
+            const double hh = hk0 * H[9 * l + 3 * M] + hk1 * H[9 * l + 3 * M + 1] + hk2 * H[9 * l + 3 * M + 2];
+            v += g2 * ((M == L) ? 2.0 / 3.0 : -1.0 / 3.0) * hh;
+            if (dia) v += SL[M];
+            row[1 + M - A0] += s * v;
+        }
+        const double hk[3] = {hk0, hk1, hk2};
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
+            const int I = PI[pp], K = PK[pp];
+            double v = g2 * ((L == I) ? 2.0 / 3.0 : -1.0 / 3.0) * hk[K] * Fl[I];
+            if (I != K) v += g2 * ((L == K) ? 2.0 / 3.0 : -1.0 / 3.0) * hk[I] * Fl[K];
+            if (dia) {
+                if (L == I) v += st[ST_G + 3 * I + K];
+                if (I != K && L == K) v += st[ST_G + 3 * K + I];
+            }
+            row[4 + pp - A0] += s * v;
+        }
+    } else if (a >= 4) {
+        constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
+        const int I = PI[a - 4], K = PK[a - 4];
+        const double FkI = F[3 * k + I], FkK = F[3 * k + K];
+        const bool dia = (k == l), off = (I != K);
+        // first-gradient columns: C[(k,IK),(l,M)] = C[(l,M),(k,IK)]
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * H[9 * l + 3 * M + K] * FkI;
+            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * H[9 * l + 3 * M + I] * FkK;
+            if (dia) {
+                if (M == I) v += st[ST_G + 3 * I + K];
+                if (off && M == K) v += st[ST_G + 3 * K + I];
+            }
+            row[1 + M - A0] += s * v;
+        }
+        // second-gradient columns (J,K2): pairs (I,K),(K,I) against (J,K2),(K2,J)
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PJ[6] = {0, 0, 0, 1, 1, 2}, PK2[6] = {0, 1, 2, 1, 2, 2};
+            const int J = PJ[pp], K2 = PK2[pp];
+            double v = 0.0;
+            if (K == K2) v += ((I == J) ? 2.0 / 3.0 : -1.0 / 3.0) * FkI * Fl[J];
+            if (J != K2 && K == J) v += ((I == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * FkI * Fl[K2];
+            if (off && I == K2) v += ((K == J) ? 2.0 / 3.0 : -1.0 / 3.0) * FkK * Fl[J];
+            if (off && J != K2 && I == J) v += ((K == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * FkK * Fl[K2];
+            row[4 + pp - A0] += s * g2 * v;
+        }
+    }
+}
+
 __constant__ const int kPairI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kPairK[6] = {0, 1, 2, 1, 2, 2};
 

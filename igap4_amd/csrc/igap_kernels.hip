@@ -317,6 +317,8 @@ template <> struct Mfma<float> {
 __constant__ const int kBlkI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kBlkJ[6] = {0, 1, 2, 1, 2, 2};
 
+__device__ __forceinline__ int blk_index(int k, int l) { return k == 0 ? l : (k == 1 ? 2 + l : 5); }   // k <= l
+
 template <typename T, int NH>
 __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh,
                                                                          int64_t ndof, T *__restrict__ Ke, int nel)
@@ -325,15 +327,20 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     constexpr int NNODE = (NH == 1) ? 1 : 8;
     constexpr int A0 = (NH == 1) ? 1 : 0;     // first jet slot in the contraction
     constexpr int NA = 10 - A0;               // jet slots per Gauss point in K
+    constexpr int NN = NA * NA;               // entries of one (i,j) block of D
     constexpr int KSTEPS = QC * NA / 4;       // 4-deep k-steps per chunk
     static_assert((QC * NA) % 4 == 0, "chunk K dimension must be a multiple of 4");
+    // static: the strain-measure state of all 64 Gauss points is built once in
+    // stage A; transient: the u-jets of the three levels are kept and the
+    // eight rule-node states are built per chunk
+    constexpr int NSTATE = (NH == 1) ? 64 : QC * NNODE;
     __shared__ double sT[3][4][3][3];          // 1-D tables [d][q_d][order][i]
     __shared__ double sCP[27 * STR];
     __shared__ double sGeo[64 * GEO];
-    __shared__ double sJet[64 * 30 * NH];
-    __shared__ double sSt[QC * NNODE * STW];
+    __shared__ double sJet[(NH == 1) ? 1 : 64 * 30 * NH];
+    __shared__ double sSt[NSTATE * STW];
     __shared__ T sR[QC][10][32];
-    __shared__ T sD[QC][900];
+    __shared__ T sD[QC][6 * NN];              // blocks i <= j, slots A0..9
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int e = blockIdx.x;
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
     const int ed[3] = {ex, ey, ez};
 
-    // ---- element data, 1-D tables, zero D
+    // ---- element data, 1-D tables
     if (tid < 27) {
         int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
         int ax = ex + bx, ay = ey + by, az = ez + bz;
@@ -363,7 +370,6 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
         double x = (-c + 1.0) * kV[2] + c * kV[3];
         bspline1d(kV, x, sT[d][q][0], sT[d][q][1], sT[d][q][2]);
     }
-    for (int i = tid; i < QC * 900; i += 256) (&sD[0][0])[i] = (T)0;
     __syncthreads();
 
     // ---- stage A: 64 Gauss points x 4 parts
@@ -437,7 +443,19 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
 #pragma unroll
             for (int a = 0; a < 30; a++) { u[a] += __shfl_xor(u[a], 1, 64); u[a] += __shfl_xor(u[a], 2, 64); }
             add_periodic(M, xq, u);
-            if (part == 0) {
+            if (NH == 1) {
+                if (part == 0) {
+                    State st;
+                    phys_state(u, P, st);
+                    double *sp = sSt + q * STW;
+#pragma unroll
+                    for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+#pragma unroll
+                    for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+#pragma unroll
+                    for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
+                }
+            } else if (part == 0) {
 #pragma unroll
                 for (int a = 0; a < 30; a++) sJet[q * 30 * NH + 30 * h + a] = u[a];
             }
@@ -466,9 +484,10 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
 
     for (int chunk = 0; chunk < 64 / QC; chunk++) {
         const int q0 = chunk * QC;
-        // ---- B1: physical basis jets
-        if (tid < QC * 32) {
-            int ql = tid >> 5, b = tid & 31, q = q0 + ql;
+        // ---- B1 (transient: + the rule-node states): wave wv builds the basis
+        //      jets of Gauss point q0 + wv
+        {
+            const int ql = wv, q = q0 + ql, b = lane;
             if (b < 27) {
                 const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
                 const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
@@ -479,77 +498,58 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                 phys_jet(R, g + 1, g + 10);
 #pragma unroll
                 for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)R[a];
-            } else {
+            } else if (b < 32) {
 #pragma unroll
                 for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)0;
-            }
-        } else if (tid < QC * 32 + QC * NNODE) {
-            // ---- B2a: strain-measure states at the rule nodes
-            int r = tid - QC * 32, ql = r / NNODE, j = r % NNODE, q = q0 + ql;
-            const double *u = sJet + q * 30 * NH;
-            double us[30];
-            if (NH == 1) {
-#pragma unroll
-                for (int i = 0; i < 30; i++) us[i] = u[i];
-            } else {
+            } else if (NH != 1 && b < 32 + NNODE) {
+                const int j = b - 32;
+                const double *u = sJet + q * 30 * NH;
+                double us[30];
 #pragma unroll
                 for (int i = 0; i < 30; i++) us[i] = 0.5 * (u[60 + i] + u[30 + i]) + P.s[j] * (0.5 * (u[i] - u[60 + i]));
+                State st;
+                phys_state(us, P, st);
+                double *sp = sSt + (ql * NNODE + j) * STW;
+#pragma unroll
+                for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+#pragma unroll
+                for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+#pragma unroll
+                for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
             }
-            State st;
-            phys_state(us, P, st);
-            double *s = sSt + (ql * NNODE + j) * STW;
-#pragma unroll
-            for (int i = 0; i < 9; i++) s[ST_F + i] = st.F[i];
-#pragma unroll
-            for (int i = 0; i < 27; i++) s[ST_H + i] = st.H[i];
-#pragma unroll
-            for (int i = 0; i < 9; i++) { s[ST_S + i] = st.S[i]; s[ST_G + i] = st.G[i]; s[ST_N + i] = st.N[i]; }
         }
-        __syncthreads();
-        // ---- B2b: D_q = w_q C_q on the blocks i <= j; wave wv handles Gauss point q0 + wv
+        if (NH != 1) __syncthreads();   // node states are read by the whole wave below
+        // ---- B2: D_q = w_q C_q on the blocks i <= j for Gauss point q0 + wv;
+        //      lane 9p + a' (10p + a' transient) builds row a' of block p
         {
             const int ql = wv, q = q0 + ql;
-            const double wgt = sGeo[q * GEO];
-            const double *S0 = sSt + ql * NNODE * STW;
-            T *D = sD[ql];
-            for (int f = lane; f < 54; f += 64) {
-                int p = f / 9, L = (f / 3) % 3, Mi = f % 3, k = kBlkI[p], l = kBlkJ[p];
-                double v = 0.0;
-                for (int j = 0; j < NNODE; j++) v += (NH == 1 ? 1.0 : P.ct[j]) * c_ff(S0 + j * STW, P, k, L, l, Mi);
-                D[(10 * k + 1 + L) * 30 + 10 * l + 1 + Mi] = (T)(wgt * v);
+            if (lane < 6 * NA) {
+                const int p = lane / NA, ar = lane - p * NA, k = kBlkI[p], l = kBlkJ[p];
+                const double wgt = sGeo[q * GEO];
+                const double *S0 = (NH == 1) ? sSt + q * STW : sSt + ql * NNODE * STW;
+                double row[NA];
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
+                for (int j = 0; j < NNODE; j++) c_row<A0>(S0 + j * STW, P, k, ar + A0, l, (NH == 1 ? 1.0 : P.ct[j]), row);
+                if (NH != 1 && ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
+                T *D = sD[ql] + p * NN + ar * NA;
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
             }
-            for (int f = lane; f < 162; f += 64) {
-                int k = f / 54, L = (f / 18) % 3, l = (f / 6) % 3, pp = f % 6;
-                int I = kPairI[pp], K = kPairK[pp];
-                double v = 0.0;
-                for (int j = 0; j < NNODE; j++) v += (NH == 1 ? 1.0 : P.ct[j]) * c_fh(S0 + j * STW, P, k, L, l, I, K);
-                T tv = (T)(wgt * v);
-                if (k <= l) D[(10 * k + 1 + L) * 30 + 10 * l + 4 + pp] = tv;
-                if (k >= l) D[(10 * l + 4 + pp) * 30 + 10 * k + 1 + L] = tv;
-            }
-            for (int f = lane; f < 216; f += 64) {
-                int p = f / 36, pp = (f / 6) % 6, pp2 = f % 6, k = kBlkI[p], l = kBlkJ[p];
-                double v = 0.0;
-                for (int j = 0; j < NNODE; j++)
-                    v += (NH == 1 ? 1.0 : P.ct[j]) * c_hh(S0 + j * STW, P, k, kPairI[pp], kPairK[pp], l, kPairI[pp2], kPairK[pp2]);
-                D[(10 * k + 4 + pp) * 30 + 10 * l + 4 + pp2] = (T)(wgt * v);
-            }
-            if (NH != 1 && lane < 3)
-                D[(10 * lane) * 30 + 10 * lane] = (T)(wgt * (P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt)));
         }
         __syncthreads();
         // ---- B3: matrix-core contraction, K-dim = (q, a) of this chunk
 #pragma unroll 3
         for (int s = 0; s < KSTEPS; s++) {
-            const int kk = 4 * s + kr, ql = kk / NA, a = A0 + kk % NA;
-            const T Av0 = sR[ql][a][m], Av1 = sR[ql][a][16 + m];
+            const int kk = 4 * s + kr, ql = kk / NA, a = kk % NA;     // slot a + A0
+            const T Av0 = sR[ql][A0 + a][m], Av1 = sR[ql][A0 + a][16 + m];
             T Rc[NA];
 #pragma unroll
             for (int bb = 0; bb < NA; bb++) Rc[bb] = sR[ql][A0 + bb][16 * ctile + m];
-            const T *Drow = sD[ql] + a * 30 + A0;
+            const T *Drow = sD[ql] + a * NA;
 #pragma unroll
             for (int t = 0; t < 3; t++) {
-                const T *d = Drow + 300 * bi[t] + 10 * bj[t];
+                const T *d = Drow + (blk0 + t) * NN;
                 T y = d[0] * Rc[0];
 #pragma unroll
                 for (int bb = 1; bb < NA; bb++) y += d[bb] * Rc[bb];
