@@ -91,18 +91,24 @@ __device__ __forceinline__ void tensor_jet(const double *t0, const double *t1, c
 }
 
 // NURBS quotient rule (src/assmcore_iga.c:153-165): R = N w / W to 2nd order
+// One reciprocal of W replaces the reference's 25 divisions (a rounding-level
+// difference); the update order of the slots is the reference's.
 __device__ __forceinline__ void nurbs_param(const double *N, double wb, const double *W, double *R)
 {
+    const double iw = 1.0 / W[0], f = wb * iw;
+    double w[10];
 #pragma unroll
-    for (int a = 0; a < 10; a++) R[a] = N[a] * wb / W[0];
+    for (int a = 1; a < 10; a++) w[a] = W[a] * iw;
 #pragma unroll
-    for (int a = 1; a < 10; a++) R[a] -= R[0] * W[a] / W[0];
-    R[4] -= (R[1] * W[1] + R[1] * W[1]) / W[0];
-    R[5] -= (R[2] * W[1] + R[1] * W[2]) / W[0];
-    R[6] -= (R[3] * W[1] + R[1] * W[3]) / W[0];
-    R[7] -= (R[2] * W[2] + R[2] * W[2]) / W[0];
-    R[8] -= (R[3] * W[2] + R[2] * W[3]) / W[0];
-    R[9] -= (R[3] * W[3] + R[3] * W[3]) / W[0];
+    for (int a = 0; a < 10; a++) R[a] = N[a] * f;
+#pragma unroll
+    for (int a = 1; a < 10; a++) R[a] -= R[0] * w[a];
+    R[4] -= R[1] * w[1] + R[1] * w[1];
+    R[5] -= R[2] * w[1] + R[1] * w[2];
+    R[6] -= R[3] * w[1] + R[1] * w[3];
+    R[7] -= R[2] * w[2] + R[2] * w[2];
+    R[8] -= R[3] * w[2] + R[2] * w[3];
+    R[9] -= R[3] * w[3] + R[3] * w[3];
 }
 
 // parametric -> physical derivatives (src/assmcore_iga.c:179-208).

@@ -328,8 +328,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     constexpr int A0 = (NH == 1) ? 1 : 0;     // first jet slot in the contraction
     constexpr int NA = 10 - A0;               // jet slots per Gauss point in K
     constexpr int NN = NA * NA;               // entries of one (i,j) block of D
-    constexpr int KSTEPS = QC * NA / 4;       // 4-deep k-steps per chunk
-    static_assert((QC * NA) % 4 == 0, "chunk K dimension must be a multiple of 4");
+    static_assert(QC == 4, "the MFMA k-row is the Gauss point of a 4-point chunk");
     // static: the strain-measure state of all 64 Gauss points is built once in
     // stage A; transient: the u-jets of the three levels are kept and the
     // eight rule-node states are built per chunk
@@ -538,23 +537,29 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             }
         }
         __syncthreads();
-        // ---- B3: matrix-core contraction, K-dim = (q, a) of this chunk
-#pragma unroll 3
-        for (int s = 0; s < KSTEPS; s++) {
-            const int kk = 4 * s + kr, ql = kk / NA, a = kk % NA;     // slot a + A0
-            const T Av0 = sR[ql][A0 + a][m], Av1 = sR[ql][A0 + a][16 + m];
+        // ---- B3: matrix-core contraction over (q, a) of this chunk.  The
+        //      4-deep k-row of each MFMA is the chunk's Gauss point (kr) and
+        //      the k-step is the jet slot, so every LDS address is a lane base
+        //      plus a compile-time offset and the lane's basis column Rc is
+        //      read once per chunk.
+        {
+            const T *Rq = &sR[kr][A0][0];
+            const T *Dq = sD[kr] + blk0 * NN;
             T Rc[NA];
 #pragma unroll
-            for (int bb = 0; bb < NA; bb++) Rc[bb] = sR[ql][A0 + bb][16 * ctile + m];
-            const T *Drow = sD[ql] + a * NA;
+            for (int bb = 0; bb < NA; bb++) Rc[bb] = Rq[32 * bb + 16 * ctile + m];
 #pragma unroll
-            for (int t = 0; t < 3; t++) {
-                const T *d = Drow + (blk0 + t) * NN;
-                T y = d[0] * Rc[0];
+            for (int a = 0; a < NA; a++) {
+                const T Av0 = Rq[32 * a + m], Av1 = Rq[32 * a + 16 + m];
 #pragma unroll
-                for (int bb = 1; bb < NA; bb++) y += d[bb] * Rc[bb];
-                acc[t][0] = Mfma<T>::run(Av0, y, acc[t][0]);
-                acc[t][1] = Mfma<T>::run(Av1, y, acc[t][1]);
+                for (int t = 0; t < 3; t++) {
+                    const T *d = Dq + t * NN + a * NA;
+                    T y = d[0] * Rc[0];
+#pragma unroll
+                    for (int bb = 1; bb < NA; bb++) y += d[bb] * Rc[bb];
+                    acc[t][0] = Mfma<T>::run(Av0, y, acc[t][0]);
+                    acc[t][1] = Mfma<T>::run(Av1, y, acc[t][1]);
+                }
             }
         }
         __syncthreads();
