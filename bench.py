@@ -42,14 +42,79 @@ def parse():
     ap.add_argument("--nelem", default="64,16,16")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--probe", type=int, default=0,
+                    help="profiling run for rocprofv3 --pmc: one 256 MiB device copy (the HBM "
+                         "counter calibration) then N residual+tangent calls; prints no JSON")
+    ap.add_argument("--pmc-json", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn the counter_collection.csv of a FETCH_SIZE and a WRITE_SIZE "
+                         "--probe run into profiles/tangent_elem_<nelem>_pmc.json")
     return ap.parse_args()
 
 
-def traffic_from_profiles():
+COPY_BYTES = 1 << 28   # the calibration copy of --probe
+
+
+def probe(nel, calls):
+    import torch
+    import igap4_amd as ig
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    mesh = ig.MeshIGABeam(L=(4.0, 1.0, 1.0), nelem=nel)
+    with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.05)) as A:
+        A.set_uhist(ig.smooth_state(mesh, amp=0.01))
+        for _ in range(calls):
+            A.residual_dev(None, None)
+            A.tangent_dev(None, None, 0)
+        A.synchronize()
+    print(f"probe done: copy {COPY_BYTES} B, {calls} x (residual + tangent) on {nel}", flush=True)
+
+
+def pmc_json(nel, fetch_csv, write_csv):
+    """HBM bytes per tangent-kernel launch: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) of k_tangent_elem<double,1>, each divided by what the same
+    counter reads per byte of the calibration copy."""
+    import csv
+
+    def per_kernel(path, counter):
+        acc = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                k = r["Kernel_Name"]
+                acc.setdefault(k, {}).setdefault(r["Dispatch_Id"], 0.0)
+                acc[k][r["Dispatch_Id"]] += float(r["Counter_Value"])
+        return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+
+    out = {"workload": f"{nel[0]}x{nel[1]}x{nel[2]} static beam, k_tangent_elem<double,1>",
+           "copy_bytes": COPY_BYTES}
+    total = 0.0
+    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        d = per_kernel(path, name)
+        copy = [v for k, v in d.items() if "copy" in k.lower() or "elementwise" in k.lower()]
+        tang = [v for k, v in d.items() if "k_tangent_elem<double, 1>" in k]
+        if not copy or not tang:
+            raise SystemExit(f"{path}: no copy or tangent dispatch found for {name}")
+        factor = max(copy) * 1024.0 / COPY_BYTES
+        b = tang[0] * 1024.0 / factor
+        out[name] = {"tangent_kib_per_dispatch": tang[0], "copy_kib_per_dispatch": max(copy),
+                     "calibration_factor": factor, "tangent_bytes_per_dispatch": b}
+        total += b
+    out["tangent_elem_bytes_per_launch"] = total
+    path = os.path.join(ROOT, "profiles", f"tangent_elem_{nel[0]}x{nel[1]}x{nel[2]}_pmc.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+def traffic_from_profiles(nel):
     """HBM bytes per tangent-kernel launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE / WRITE_SIZE, KiB per dispatch, scaled by the copy-kernel
     calibration recorded there), or None when no summary is committed."""
-    path = os.path.join(ROOT, "profiles", "tangent_elem_64x16x16_pmc.json")
+    path = os.path.join(ROOT, "profiles", f"tangent_elem_{nel[0]}x{nel[1]}x{nel[2]}_pmc.json")
     if not os.path.exists(path):
         return None
     try:
@@ -87,14 +152,19 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only: the whole patch lives in one "
                                    "MI355X's HBM and there is no multi-GPU path; run with --gpus 1"}))
         sys.exit(2)
+    nel = tuple(int(x) for x in a.nelem.split(","))
+    if a.pmc_json:
+        pmc_json(nel, *a.pmc_json)
+        return
+    if a.probe:
+        probe(nel, a.probe)
+        return
     import igap4_amd as ig
 
-    nel = tuple(int(x) for x in a.nelem.split(","))
     mesh = ig.MeshIGABeam(L=(4.0, 1.0, 1.0), nelem=nel)
     phys = ig.PhysGradElast(coef=ig.SVK_COEF)
     bc = ig.BCClampedTip(tip=0.05)
-    from oracle import oracle as O  # smooth synthetic state only (numpy)
-    U = O.smooth_state(O.Mesh(nel, L=(4, 1, 1)), amp=0.01)
+    U = ig.smooth_state(mesh, amp=0.01)
 
     A = ig.Assm(mesh, phys, bc, device=0)
     A.set_uhist(U)
@@ -137,7 +207,7 @@ def main():
     avg32 = {w: kt32[w][0] / max(1, kt32[w][1]) for w in kt32}
     tk = avg[1] * 1e-3
     achieved = F_K * nelem / tk / 1e12
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles(nel)
 
     line = {
         "metric": "elements assembled/s (R+K)",

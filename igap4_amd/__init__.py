@@ -6,8 +6,8 @@ the benchmark and the Newton driver.
 """
 from .assm import (MFMA_ROW, SVK_COEF, Assm, BC, BCClampedTip, BCDefault, MeshIGABeam, PhysGradElast,
                    PhysGradElastTime, selftest_expected, selftest_mfma,
-                   selftest_unpack)
+                   selftest_unpack, smooth_state)
 
 __all__ = ["Assm", "BC", "BCClampedTip", "BCDefault", "MeshIGABeam", "PhysGradElast",
            "PhysGradElastTime", "SVK_COEF", "MFMA_ROW", "selftest_mfma", "selftest_expected",
-           "selftest_unpack"]
+           "selftest_unpack", "smooth_state"]

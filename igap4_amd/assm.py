@@ -66,6 +66,18 @@ class MeshIGABeam:
         return d
 
 
+def smooth_state(mesh, amp=0.01, levels=1):
+    """Synthetic displacement U_i(X) = amp sin(pi X/Lx) cos(pi Y/Ly) cos(pi Z/Lz) (1, .5, .25)_i
+    at the owned control points of ``mesh`` (a smooth, non-trivial state for
+    throughput runs), ``levels`` copies for the time levels of U_hist."""
+    na, nb = mesh.na, mesh.nb
+    X = mesh.X.reshape(na[0], na[1], na[2], 3)[:nb[0], :nb[1], :nb[2]]
+    Lx, Ly, Lz = mesh.L
+    f = amp * np.sin(np.pi * X[..., 0] / Lx) * np.cos(np.pi * X[..., 1] / Ly) * np.cos(np.pi * X[..., 2] / Lz)
+    U = np.stack([f, 0.5 * f, 0.25 * f], axis=-1).reshape(-1)
+    return np.tile(U, levels)
+
+
 def PhysGradElast(par=None, coef=None):
     """gradelast (static) with the defaults of phys/physgradelast.c:21; ``coef``
     overrides the energy coefficients (e.g. ``SVK_COEF``)."""

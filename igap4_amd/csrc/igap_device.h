@@ -242,6 +242,28 @@ __device__ __forceinline__ void phys_residual_acc(const State &st, double s, dou
 
 // affine macro-field of par_periodic (src/assmcore_iga.c:225-229): the value
 // gets a_kd * xq_d (knot-space coordinate), the first derivatives a_kd
+// par_periodic's affine field on displacement component k (value and first
+// derivative slots of one 10-slot jet)
+__device__ __forceinline__ void add_periodic1(const MeshK &M, const double *xq, int k, double *u)
+{
+    u[0] += M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) u[1 + d] += M.par_periodic[3 * k + d];
+}
+
+// Last (x) step of the sum factorisation: the 10-slot parametric jet of one
+// field at one Gauss point from its y,z-contracted values
+// Bq[96 * bx + s6] (s6 = derivative pair (dy,dz) in the order (0,0) (1,0)
+// (0,1) (2,0) (1,1) (0,2)) and the x table tx[3 * dx + bx].
+__device__ __forceinline__ void sf_jet(const double *Bq, const double *tx, double *J)
+{
+    constexpr int DX[10] = {0, 1, 0, 0, 2, 1, 1, 0, 0, 0};
+    constexpr int S6[10] = {0, 0, 1, 2, 0, 1, 2, 3, 4, 5};
+#pragma unroll
+    for (int a = 0; a < 10; a++)
+        J[a] = Bq[S6[a]] * tx[3 * DX[a]] + Bq[96 + S6[a]] * tx[3 * DX[a] + 1] + Bq[192 + S6[a]] * tx[3 * DX[a] + 2];
+}
+
 __device__ __forceinline__ void add_periodic(const MeshK &M, const double *xq, double *u)
 {
 #pragma unroll

@@ -327,7 +327,11 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     constexpr int NNODE = (NH == 1) ? 1 : 8;
     constexpr int A0 = (NH == 1) ? 1 : 0;     // first jet slot in the contraction
     constexpr int NA = 10 - A0;               // jet slots per Gauss point in K
-    constexpr int NN = NA * NA;               // entries of one (i,j) block of D
+    // rows of D are padded to whole 16-byte LDS vectors so that B3 reads them
+    // with 128-bit loads at compile-time offsets
+    constexpr int VN = 16 / (int)sizeof(T);
+    constexpr int RS = (NA + VN - 1) / VN * VN;   // row stride of D
+    constexpr int NN = NA * RS;                   // stride of one (i,j) block of D
     static_assert(QC == 4, "the MFMA k-row is the Gauss point of a 4-point chunk");
     // static: the strain-measure state of all 64 Gauss points is built once in
     // stage A; transient: the u-jets of the three levels are kept and the
@@ -338,8 +342,13 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     __shared__ double sGeo[64 * GEO];
     __shared__ double sJet[(NH == 1) ? 1 : 64 * 30 * NH];
     __shared__ double sSt[NSTATE * STW];
-    __shared__ T sR[QC][10][32];
-    __shared__ T sD[QC][6 * NN];              // blocks i <= j, slots A0..9
+    // stage A's sum-factorisation buffers share memory with stage B's R and D
+    constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
+    constexpr int WORK_A = NF * (108 + 288) * (int)sizeof(double);
+    constexpr int WORK_B = QC * (10 * 32 + 6 * NN) * (int)sizeof(T);
+    __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_B ? WORK_A : WORK_B];
+    T(*sR)[10][32] = reinterpret_cast<T(*)[10][32]>(sWork);
+    T(*sD)[6 * NN] = reinterpret_cast<T(*)[6 * NN]>(sWork + QC * 10 * 32 * sizeof(T));   // blocks i <= j, slots A0..9
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int e = blockIdx.x;
@@ -371,92 +380,103 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     }
     __syncthreads();
 
-    // ---- stage A: 64 Gauss points x 4 parts
+    // ---- stage A: fields at the 64 Gauss points by sum factorisation.
+    //      The fields P_f = sum_b c_{b,f} N_b with c = w, w X (3), w U (3 per
+    //      level) are contracted one direction at a time (z, then y, then x:
+    //      27 -> 108 -> 288 -> 640 values per field); the rational jets follow
+    //      by the quotient rule on P_f / W and the physical jets by the chain
+    //      rule, both linear in the basis, so the result is the reference's
+    //      sum over b of c_b R_b in another order of summation.
     {
-        const int q = tid >> 2, part = tid & 3;
-        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
-        const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
-        double W[10];
+        double *sA = reinterpret_cast<double *>(sWork), *sB = sA + NF * 108;
+        for (int i = tid; i < NF * 108; i += 256) {            // [f][bx*3+by][qz][dz]
+            const int f = i / 108, r = i - 108 * f, bxy = r / 12, qd = r - 12 * bxy, qz = qd / 3, dz = qd - 3 * qz;
+            const double *tz = &sT[2][qz][dz][0];
+            double acc = 0.0;
 #pragma unroll
-        for (int a = 0; a < 10; a++) W[a] = 0.0;
-        for (int b = part; b < 27; b += 4) {
-            double N[10];
-            tensor_jet(t0, t1, t2, b, N);
-            double wb = sCP[STR * b + 3];
-#pragma unroll
-            for (int a = 0; a < 10; a++) W[a] += wb * N[a];
-        }
-#pragma unroll
-        for (int a = 0; a < 10; a++) { W[a] += __shfl_xor(W[a], 1, 64); W[a] += __shfl_xor(W[a], 2, 64); }
-        double Gi[9], X2[18], detG;
-        {
-            double Xj[30];
-#pragma unroll
-            for (int a = 0; a < 30; a++) Xj[a] = 0.0;
-            for (int b = part; b < 27; b += 4) {
-                double N[10], R[10];
-                tensor_jet(t0, t1, t2, b, N);
-                nurbs_param(N, sCP[STR * b + 3], W, R);
-#pragma unroll
-                for (int d = 0; d < 3; d++)
-#pragma unroll
-                    for (int a = 0; a < 10; a++) Xj[10 * d + a] += sCP[STR * b + d] * R[a];
+            for (int bz = 0; bz < 3; bz++) {
+                const double *c = sCP + STR * (3 * bxy + bz);
+                const double cf = (f == 0) ? c[3] : c[3] * ((f < 4) ? c[f - 1] : c[f]);
+                acc += cf * tz[bz];
             }
-#pragma unroll
-            for (int a = 0; a < 30; a++) { Xj[a] += __shfl_xor(Xj[a], 1, 64); Xj[a] += __shfl_xor(Xj[a], 2, 64); }
-            double G[9];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) G[3 * i + j] = Xj[10 * i + 1 + j];
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-#pragma unroll
-                for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = Xj[10 * d + 4 + s2];
-            detG = inv3(G, Gi);
+            sA[i] = acc;
         }
+        __syncthreads();
+        for (int i = tid; i < NF * 288; i += 256) {            // [f][bx][qy][qz][s6]
+            const int f = i / 288, r = i - 288 * f, s6 = r % 6, t = r / 6, qz = t & 3, qy = (t >> 2) & 3, bx = t >> 4;
+            const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
+            const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
+            const double *ty = &sT[1][qy][dy][0];
+            const double *a = sA + 108 * f + 36 * bx + 3 * qz + dz;
+            sB[i] = a[0] * ty[0] + a[12] * ty[1] + a[24] * ty[2];
+        }
+        __syncthreads();
+
+        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
+        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+        const double *tx = &sT[0][qd[0]][0][0];
+        const int boff = (4 * qd[1] + qd[2]) * 6;
+        double W[10], Jg[3][10];
+        sf_jet(sB + boff, tx, W);
+        // part 0: the three geometry fields; part 1 + k: displacement
+        // component k of every level
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int f = (part == 0) ? 1 + j : 4 + 3 * j + part - 1;
+            if (part == 0 || j < NH) {
+                double Pj[10];
+                sf_jet(sB + 288 * f + boff, tx, Pj);
+                nurbs_param(Pj, 1.0, W, Jg[j]);
+            }
+        }
+        double G[9], Gi[9], X2[18];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) G[3 * i + j] = Jg[i][1 + j];
+        const double detG = inv3(G, Gi);
+#pragma unroll
+        for (int i = 0; i < 9; i++) Gi[i] = __shfl(Gi[i], lq, 64);
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = __shfl(Jg[d][4 + s2], lq, 64);
         double xq[3], dX[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const double *kV = M.knot[d] + ed[d];
-            double c = M.gq[qd[d]];
+            const double c = M.gq[qd[d]];
             xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
             dX[d] = kV[3] - kV[2];
         }
-        for (int h = 0; h < NH; h++) {
-            double u[30];
+        if (part > 0) {
+            const int k = part - 1;
 #pragma unroll
-            for (int a = 0; a < 30; a++) u[a] = 0.0;
-            for (int b = part; b < 27; b += 4) {
-                double N[10], R[10];
-                tensor_jet(t0, t1, t2, b, N);
-                nurbs_param(N, sCP[STR * b + 3], W, R);
-                phys_jet(R, Gi, X2);
+            for (int h = 0; h < NH; h++) {
+                phys_jet(Jg[h], Gi, X2);
+                add_periodic1(M, xq, k, Jg[h]);
+                if (NH != 1) {
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    double Uv = sCP[STR * b + 4 + 3 * h + k];
-#pragma unroll
-                    for (int a = 0; a < 10; a++) u[10 * k + a] += R[a] * Uv;
+                    for (int a = 0; a < 10; a++) sJet[q * 30 * NH + 30 * h + 10 * k + a] = Jg[h][a];
                 }
             }
+        }
+        if (NH == 1) {
+            double u[30];
 #pragma unroll
-            for (int a = 0; a < 30; a++) { u[a] += __shfl_xor(u[a], 1, 64); u[a] += __shfl_xor(u[a], 2, 64); }
-            add_periodic(M, xq, u);
-            if (NH == 1) {
-                if (part == 0) {
-                    State st;
-                    phys_state(u, P, st);
-                    double *sp = sSt + q * STW;
+            for (int k = 0; k < 3; k++)
 #pragma unroll
-                    for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+                for (int a = 0; a < 10; a++) u[10 * k + a] = __shfl(Jg[0][a], lq + 1 + k, 64);
+            if (part == 0) {
+                State st;
+                phys_state(u, P, st);
+                double *sp = sSt + q * STW;
 #pragma unroll
-                    for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+                for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
 #pragma unroll
-                    for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
-                }
-            } else if (part == 0) {
+                for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
 #pragma unroll
-                for (int a = 0; a < 30; a++) sJet[q * 30 * NH + 30 * h + a] = u[a];
+                for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
             }
         }
         if (part == 0) {
@@ -531,7 +551,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
                 for (int j = 0; j < NNODE; j++) c_row<A0>(S0 + j * STW, P, k, ar + A0, l, (NH == 1 ? 1.0 : P.ct[j]), row);
                 if (NH != 1 && ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
-                T *D = sD[ql] + p * NN + ar * NA;
+                T *D = sD[ql] + p * NN + ar * RS;
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
             }
@@ -553,7 +573,15 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                 const T Av0 = Rq[32 * a + m], Av1 = Rq[32 * a + 16 + m];
 #pragma unroll
                 for (int t = 0; t < 3; t++) {
-                    const T *d = Dq + t * NN + a * NA;
+                    typedef T vec_t __attribute__((ext_vector_type(VN)));
+                    const vec_t *dv = reinterpret_cast<const vec_t *>(Dq + t * NN + a * RS);
+                    T d[RS];
+#pragma unroll
+                    for (int v = 0; v < RS / VN; v++) {
+                        const vec_t x = dv[v];
+#pragma unroll
+                        for (int c = 0; c < VN; c++) d[VN * v + c] = x[c];
+                    }
                     T y = d[0] * Rc[0];
 #pragma unroll
                     for (int bb = 1; bb < NA; bb++) y += d[bb] * Rc[bb];
