@@ -1,4 +1,5 @@
-"""ctypes binding of libigap_gpu.so (the C ABI declared in include/igap_gpu.h).
+"""ctypes binding of libigap_gpu.so (the C ABI declared in include/igap_gpu.h and
+include/igap_ws.h).
 
 The library is built in-tree by igap4_amd/build.sh (or ``__graft_entry__.build()``).
 There is no fallback: if the library or a GPU is missing, loading fails loudly.
@@ -46,7 +47,7 @@ class SolverOpts(C.Structure):
     _fields_ = [("snes_atol", C.c_double), ("snes_rtol", C.c_double), ("snes_maxit", C.c_int),
                 ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double), ("ksp_maxit", C.c_int),
                 ("ksp_restart", C.c_int), ("pc", C.c_int), ("tangent_prec", C.c_int),
-                ("linesearch", C.c_int)]
+                ("linesearch", C.c_int), ("pc_sweeps", C.c_int)]
 
 
 class SolverStats(C.Structure):
@@ -78,6 +79,8 @@ _SIGS = {
     "igap_gpu_set_bc": (C.c_int, [C.c_void_p, C.POINTER(BCDesc)]),
     "igap_gpu_sizes": (C.c_int, [C.c_void_p, c_i64_p, c_i64_p, c_i64_p, C.POINTER(C.c_int)]),
     "igap_gpu_uhist": (C.c_void_p, [C.c_void_p]),
+    "igap_gpu_save_solution": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "igap_gpu_load_solution": (C.c_int, [C.c_void_p, C.c_char_p]),
     "igap_gpu_csr_rowptr": (C.c_void_p, [C.c_void_p]),
     "igap_gpu_csr_col": (C.c_void_p, [C.c_void_p]),
     "igap_gpu_csr_val64": (C.c_void_p, [C.c_void_p]),
@@ -99,6 +102,17 @@ _SIGS = {
     "igap_gpu_selftest_mfma": (C.c_int, [c_double_p, c_float_p]),
     "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),
     "igap_gpu_snes_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
+    # include/igap_ws.h
+    "igap_ws_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_uint64,
+                                 C.POINTER(C.c_void_p)]),
+    "igap_ws_destroy": (C.c_int, [C.c_void_p]),
+    "igap_ws_nparams": (C.c_int, [C.c_void_p, c_i64_p]),
+    "igap_ws_get_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "igap_ws_set_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "igap_ws_forward": (C.c_int, [C.c_void_p, c_float_p, c_float_p]),
+    "igap_ws_gradients": (C.c_int, [C.c_void_p, c_float_p, c_float_p, c_double_p, c_float_p]),
+    "igap_ws_train_step": (C.c_int, [C.c_void_p, c_float_p, c_float_p, C.c_float, c_double_p]),
+    "igap_ws_selftest_mfma_bf16": (C.c_int, [c_float_p]),
 }
 
 _LIB = None

@@ -195,6 +195,14 @@ class Assm:
     def get_uhist(self):
         return self.dtoh(self.uhist_ptr(), (self.torder + 1) * self.ndof)
 
+    def save_solution(self, fname):
+        """IGAPSolvSaveSolution: the reference's solution file (universal dof order)."""
+        check(lib().igap_gpu_save_solution(self.ctx, str(fname).encode()))
+
+    def load_solution(self, fname):
+        """IGAPSolvLoadSolution (transient: level 0 = level 1 afterwards)."""
+        check(lib().igap_gpu_load_solution(self.ctx, str(fname).encode()))
+
     def set_bc(self, bc):
         self.bc = bc
         check(lib().igap_gpu_set_bc(self.ctx, C.byref(bc)))

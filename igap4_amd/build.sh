@@ -6,7 +6,8 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 "$HIPCC" -O3 -std=c++17 --offload-arch=gfx950 -fPIC -shared \
     -Xclang -target-feature -Xclang -packed-fp32-ops \
     ${IGAP_EXTRA_FLAGS} \
-    "$HERE/csrc/igap_kernels.hip" "$HERE/csrc/igap_solver.hip" "$HERE/csrc/igap_capi.cpp" \
+    "$HERE/csrc/igap_kernels.hip" "$HERE/csrc/igap_solver.hip" "$HERE/csrc/igap_ws.hip" \
+    "$HERE/csrc/igap_capi.cpp" "$HERE/csrc/igap_ws_capi.cpp" \
     -lrocsparse -lrocblas \
     -o "$HERE/libigap_gpu.so.tmp"
 mv "$HERE/libigap_gpu.so.tmp" "$HERE/libigap_gpu.so"

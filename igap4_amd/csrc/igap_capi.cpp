@@ -514,6 +514,51 @@ extern "C" int igap_gpu_sizes(const igap_gpu_ctx *c, int64_t *ndof, int64_t *nnz
     return IGAP_OK;
 }
 extern "C" double *igap_gpu_uhist(igap_gpu_ctx *c) { return c ? c->d_Uh : nullptr; }
+
+// levels [h0, h1) of U_hist that a solution file holds (src/interface.c:271-300)
+static void file_levels(const igap_gpu_ctx *c, int &h0, int &h1)
+{
+    h0 = c->torder == 0 ? 0 : 1;
+    h1 = c->torder + 1;
+}
+
+extern "C" int igap_gpu_save_solution(igap_gpu_ctx *c, const char *fname)
+{
+    if (!c || !fname) return fail(IGAP_ERR_ARG, "null argument");
+    int h0, h1;
+    file_levels(c, h0, h1);
+    const size_t n = (size_t)(h1 - h0) * (size_t)c->ndof;
+    std::vector<double> buf(n);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(buf.data(), c->d_Uh + (size_t)h0 * c->ndof, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    FILE *f = fopen(fname, "wb");
+    if (!f) return fail(IGAP_ERR_ARG, "cannot open %s for writing", fname);
+    const size_t w = fwrite(buf.data(), sizeof(double), n, f);
+    const int cl = fclose(f);
+    if (w != n || cl != 0) return fail(IGAP_ERR_ARG, "short write to %s", fname);
+    return IGAP_OK;
+}
+
+extern "C" int igap_gpu_load_solution(igap_gpu_ctx *c, const char *fname)
+{
+    if (!c || !fname) return fail(IGAP_ERR_ARG, "null argument");
+    int h0, h1;
+    file_levels(c, h0, h1);
+    const size_t n = (size_t)(h1 - h0) * (size_t)c->ndof;
+    std::vector<double> buf(n);
+    FILE *f = fopen(fname, "rb");
+    if (!f) return fail(IGAP_ERR_ARG, "cannot open %s", fname);
+    const size_t r = fread(buf.data(), sizeof(double), n, f);
+    fclose(f);
+    if (r != n) return fail(IGAP_ERR_ARG, "%s holds %zu of the %zu values of %d level(s)", fname, r, n, h1 - h0);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->d_Uh + (size_t)h0 * c->ndof, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if (h0 == 1)
+        HIPCHK(hipMemcpyAsync(c->d_Uh, c->d_Uh + c->ndof, sizeof(double) * c->ndof, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IGAP_OK;
+}
 extern "C" double *igap_gpu_rvec(igap_gpu_ctx *c) { return c ? c->d_R : nullptr; }
 extern "C" const int64_t *igap_gpu_csr_rowptr(const igap_gpu_ctx *c) { return c ? c->d_rowptr : nullptr; }
 extern "C" const int32_t *igap_gpu_csr_col(const igap_gpu_ctx *c) { return c ? c->d_col : nullptr; }

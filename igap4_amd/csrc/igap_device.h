@@ -135,6 +135,59 @@ __device__ __forceinline__ void phys_jet(double *R, const double *Gi, const doub
     }
 }
 
+// Transpose of phys_jet (for fixed Gi, X2): given the adjoint o of the
+// physical jet, returns in place the adjoint of the parametric jet.  The
+// projection sum_a r_a R_a(b) of a residual jet onto a basis function is the
+// pairing of the adjoint with the basis function's parametric jet.
+__device__ __forceinline__ void phys_jet_adj(double *o, const double *Gi, const double *X2)
+{
+    const int pi_[6] = {0, 0, 0, 1, 1, 2}, pj_[6] = {0, 1, 2, 1, 2, 2};
+    double nb[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const int p = pi_[t], q = pj_[t];
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            const int i = pi_[s], j = pj_[s];
+            double m = Gi[3 * p + i] * Gi[3 * q + j];
+            if (p != q) m += Gi[3 * q + i] * Gi[3 * p + j];
+            acc += o[4 + s] * m;
+        }
+        nb[t] = acc;
+    }
+    double rb[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double acc = o[1 + k];
+#pragma unroll
+        for (int t = 0; t < 6; t++) acc -= nb[t] * X2[6 * k + t];
+        rb[k] = acc;
+    }
+#pragma unroll
+    for (int p = 0; p < 3; p++) o[1 + p] = rb[0] * Gi[3 * p] + rb[1] * Gi[3 * p + 1] + rb[2] * Gi[3 * p + 2];
+#pragma unroll
+    for (int t = 0; t < 6; t++) o[4 + t] = nb[t];
+}
+
+// Transpose of nurbs_param with wb = 1 (for fixed W), in place.
+__device__ __forceinline__ void nurbs_param_adj(double *R, const double *W)
+{
+    const double iw = 1.0 / W[0];
+    double w[10];
+#pragma unroll
+    for (int a = 1; a < 10; a++) w[a] = W[a] * iw;
+    R[1] -= 2.0 * w[1] * R[4] + w[2] * R[5] + w[3] * R[6];
+    R[2] -= w[1] * R[5] + 2.0 * w[2] * R[7] + w[3] * R[8];
+    R[3] -= w[1] * R[6] + w[2] * R[8] + 2.0 * w[3] * R[9];
+    double s = 0.0;
+#pragma unroll
+    for (int a = 1; a < 10; a++) s += w[a] * R[a];
+    R[0] -= s;
+#pragma unroll
+    for (int a = 0; a < 10; a++) R[a] *= iw;
+}
+
 __device__ __forceinline__ double inv3(const double *G, double *Gi)
 {
     double det = G[0] * (G[4] * G[8] - G[5] * G[7]) - G[1] * (G[3] * G[8] - G[5] * G[6]) + G[2] * (G[3] * G[7] - G[4] * G[6]);

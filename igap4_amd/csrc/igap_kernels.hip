@@ -52,174 +52,249 @@ __device__ __forceinline__ void rs_track(int N, bool up, int &base, int &cnt)
 }
 
 // ============================================================================
-// residual: one wave per element, lane = Gauss point (4x4x4 = 64)
+// residual: one workgroup (4 waves) per element.
+//   A  the Gauss-point fields by sum factorisation (as in the tangent):
+//      geometry, the physical u-jets of every history level -> LDS
+//   P  pointwise residual jets r_q (static: one lane per Gauss point on one
+//      wave; transient: four lanes per point, two rule nodes each)
+//   T  projection R_e[b][k] = sum_q w_q sum_a r_q[k][a] R_a(q, b), written as
+//      w_b sum_q (Q_q^T Phi_q^T w_q r_q[k]) . N_b(q): the transposes of the
+//      chain rule (phys_jet_adj) and of the quotient rule (nurbs_param_adj)
+//      per Gauss point, then the transposed sum factorisation x -> y -> z.
 // ============================================================================
 template <int NH>
 __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                        double *__restrict__ Re, int nel)
 {
-    constexpr int STR = 4 + 3 * NH;   // X(3) w(1) U(3*NH) per control point
-    __shared__ double sCP[4][27 * STR];
-    __shared__ double sTab[4][3][4][9];   // per wave: 1-D tables [d][q_d][3*order + i]
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int e = blockIdx.x * 4 + wv;
-    const bool valid = e < nel;
-    const int ee = valid ? e : nel - 1;
+    constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
+    constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
+    constexpr int WA = NF * (108 + 288);       // stage-A buffers
+    constexpr int WT = 1920 + 864 + 324;       // unused + x^T and y^T buffers (rho lives in sJet)
+    constexpr int GEO_R = 38;                  // wgt, Gi[9], X2[18], W[10]
+    __shared__ double sT[3][4][3][3];
+    __shared__ double sCP[27 * STR];
+    __shared__ double sWork[WA > WT ? WA : WT];
+    __shared__ double sJet[64 * 30 * NH];      // physical u-jets [q][h][k][a]; then r_q / rho_q [q][k][a]
+    __shared__ double sGeo[64 * GEO_R];
+
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x;
     const int nz = M.nelem[2], ny = M.nelem[1];
-    const int ex = ee / (ny * nz), ey = (ee / nz) % ny, ez = ee % nz;
-    double *cp = sCP[wv];
-    if (lane < 27) {
-        int bx = lane / 9, by = (lane / 3) % 3, bz = lane % 3;
+    const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
+    const int ed[3] = {ex, ey, ez};
+    if (tid < 27) {
+        int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
         int ax = ex + bx, ay = ey + by, az = ez + bz;
         int64_t id = ((int64_t)M.na[1] * M.na[2]) * ax + (int64_t)M.na[2] * ay + az;
-        cp[STR * lane + 0] = M.X[3 * id + 0];
-        cp[STR * lane + 1] = M.X[3 * id + 1];
-        cp[STR * lane + 2] = M.X[3 * id + 2];
-        cp[STR * lane + 3] = M.w[id];
+        sCP[STR * tid + 0] = M.X[3 * id + 0];
+        sCP[STR * tid + 1] = M.X[3 * id + 1];
+        sCP[STR * tid + 2] = M.X[3 * id + 2];
+        sCP[STR * tid + 3] = M.w[id];
         int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ax, M.nb[0], M.periodic[0])
                    + (int64_t)M.nb[2] * own1(ay, M.nb[1], M.periodic[1]) + own1(az, M.nb[2], M.periodic[2]);
 #pragma unroll
         for (int h = 0; h < NH; h++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) cp[STR * lane + 4 + 3 * h + k] = Uh[h * ndof + 3 * oc + k];
-    }
-    const int ed[3] = {ex, ey, ez};
-    if (lane >= 32 && lane < 44) {
-        int d = (lane - 32) >> 2, q = (lane - 32) & 3;
+            for (int k = 0; k < 3; k++) sCP[STR * tid + 4 + 3 * h + k] = Uh[h * ndof + 3 * oc + k];
+    } else if (tid >= 64 && tid < 64 + 12) {
+        int d = (tid - 64) / 4, q = (tid - 64) % 4;
         const double *kV = M.knot[d] + ed[d];
         double c = M.gq[q];
-        double *tb = sTab[wv][d][q];
-        bspline1d(kV, (-c + 1.0) * kV[2] + c * kV[3], tb, tb + 3, tb + 6);
+        double x = (-c + 1.0) * kV[2] + c * kV[3];
+        bspline1d(kV, x, sT[d][q][0], sT[d][q][1], sT[d][q][2]);
     }
     __syncthreads();
 
-    const int qd[3] = {lane >> 4, (lane >> 2) & 3, lane & 3};
-    const double *t0 = sTab[wv][0][qd[0]], *t1 = sTab[wv][1][qd[1]], *t2 = sTab[wv][2][qd[2]];
-    double xq[3], dX[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double *kV = M.knot[d] + ed[d];
-        double X0 = kV[2], X1 = kV[3], c = M.gq[qd[d]];
-        xq[d] = (-c + 1.0) * X0 + c * X1;
-        dX[d] = X1 - X0;
-    }
-    double W[10];
-#pragma unroll
-    for (int a = 0; a < 10; a++) W[a] = 0.0;
-    for (int b = 0; b < 27; b++) {
-        double N[10];
-        tensor_jet(t0, t1, t2, b, N);
-        double wb = cp[STR * b + 3];
-#pragma unroll
-        for (int a = 0; a < 10; a++) W[a] += wb * N[a];
-    }
-    double Gi[9], X2[18], detG;
+    // ---- A: fields at the Gauss points (z, then y, then x)
     {
-        double Xj[30];
+        double *sA = sWork, *sB = sWork + NF * 108;
+        for (int i = tid; i < NF * 108; i += 256) {            // [f][bx*3+by][qz][dz]
+            const int f = i / 108, r = i - 108 * f, bxy = r / 12, qd = r - 12 * bxy, qz = qd / 3, dz = qd - 3 * qz;
+            const double *tz = &sT[2][qz][dz][0];
+            double acc = 0.0;
 #pragma unroll
-        for (int a = 0; a < 30; a++) Xj[a] = 0.0;
-        for (int b = 0; b < 27; b++) {
-            double N[10], R[10];
-            tensor_jet(t0, t1, t2, b, N);
-            nurbs_param(N, cp[STR * b + 3], W, R);
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-#pragma unroll
-                for (int a = 0; a < 10; a++) Xj[10 * d + a] += cp[STR * b + d] * R[a];
+            for (int bz = 0; bz < 3; bz++) {
+                const double *c = sCP + STR * (3 * bxy + bz);
+                const double cf = (f == 0) ? c[3] : c[3] * ((f < 4) ? c[f - 1] : c[f]);
+                acc += cf * tz[bz];
+            }
+            sA[i] = acc;
         }
-        double G[9];
+        __syncthreads();
+        for (int i = tid; i < NF * 288; i += 256) {            // [f][bx][qy][qz][s6]
+            const int f = i / 288, r = i - 288 * f, s6 = r % 6, t = r / 6, qz = t & 3, qy = (t >> 2) & 3, bx = t >> 4;
+            const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
+            const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
+            const double *ty = &sT[1][qy][dy][0];
+            const double *a = sA + 108 * f + 36 * bx + 3 * qz + dz;
+            sB[i] = a[0] * ty[0] + a[12] * ty[1] + a[24] * ty[2];
+        }
+        __syncthreads();
+
+        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
+        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+        const double *tx = &sT[0][qd[0]][0][0];
+        const int boff = (4 * qd[1] + qd[2]) * 6;
+        double W[10], Jg[3][10];
+        sf_jet(sB + boff, tx, W);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int f = (part == 0) ? 1 + j : 4 + 3 * j + part - 1;
+            if (part == 0 || j < NH) {
+                double Pj[10];
+                sf_jet(sB + 288 * f + boff, tx, Pj);
+                nurbs_param(Pj, 1.0, W, Jg[j]);
+            }
+        }
+        double G[9], Gi[9], X2[18];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) G[3 * i + j] = Xj[10 * i + 1 + j];
+            for (int j = 0; j < 3; j++) G[3 * i + j] = Jg[i][1 + j];
+        const double detG = inv3(G, Gi);
+#pragma unroll
+        for (int i = 0; i < 9; i++) Gi[i] = __shfl(Gi[i], lq, 64);
 #pragma unroll
         for (int d = 0; d < 3; d++)
 #pragma unroll
-            for (int s = 0; s < 6; s++) X2[6 * d + s] = Xj[10 * d + 4 + s];
-        detG = inv3(G, Gi);
+            for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = __shfl(Jg[d][4 + s2], lq, 64);
+        if (part > 0) {
+            const int k = part - 1;
+            double xq[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double *kV = M.knot[d] + ed[d];
+                const double c = M.gq[qd[d]];
+                xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
+            }
+#pragma unroll
+            for (int h = 0; h < NH; h++) {
+                phys_jet(Jg[h], Gi, X2);
+                add_periodic1(M, xq, k, Jg[h]);
+#pragma unroll
+                for (int a = 0; a < 10; a++) sJet[q * 30 * NH + 30 * h + 10 * k + a] = Jg[h][a];
+            }
+        } else {
+            double *g = sGeo + q * GEO_R;
+            double dX[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) dX[d] = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
+            g[0] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
+#pragma unroll
+            for (int i = 0; i < 9; i++) g[1 + i] = Gi[i];
+#pragma unroll
+            for (int i = 0; i < 18; i++) g[10 + i] = X2[i];
+#pragma unroll
+            for (int i = 0; i < 10; i++) g[28 + i] = W[i];
+        }
     }
+    __syncthreads();
 
-    double rr[30];
+    // ---- P: pointwise residual jets r_q -> sJet[q * 30 * NH + 10 k + a]
     if (NH == 1) {
-        double u[30];
+        if ((tid >> 6) == (e & 3)) {                            // one wave, lane = Gauss point
+            const int q = tid & 63;
+            double u[30], r[30];
 #pragma unroll
-        for (int a = 0; a < 30; a++) u[a] = 0.0;
-        for (int b = 0; b < 27; b++) {
-            double N[10], R[10];
-            tensor_jet(t0, t1, t2, b, N);
-            nurbs_param(N, cp[STR * b + 3], W, R);
-            phys_jet(R, Gi, X2);
+            for (int a = 0; a < 30; a++) u[a] = sJet[q * 30 + a];
+            phys_residual_static(P, u, r);
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double Uv = cp[STR * b + 4 + k];
-#pragma unroll
-                for (int a = 0; a < 10; a++) u[10 * k + a] += R[a] * Uv;
-            }
+            for (int a = 0; a < 30; a++) sJet[q * 30 + a] = r[a];
         }
-        add_periodic(M, xq, u);
-        phys_residual_static(P, u, rr);
     } else {
-        // midpoint m0 = (u0+u1)/2 and increment d = (u-u0)/2 jets, plus the
-        // value slots of the three levels for the inertia term
-        double m0[30], dl[30], uv[9];
+        // lanes 4q..4q+3 evaluate two rule nodes each at s_j: m0 + s_j d with
+        // m0 = (u^1 + u^0)/2, d = (u - u^0)/2, then sum over the four lanes
+        const int q = tid >> 2, part = tid & 3;
+        const double *J = sJet + q * 30 * NH;
+        double r[30];
 #pragma unroll
-        for (int a = 0; a < 30; a++) { m0[a] = 0.0; dl[a] = 0.0; }
-#pragma unroll
-        for (int a = 0; a < 9; a++) uv[a] = 0.0;
-        for (int b = 0; b < 27; b++) {
-            double N[10], R[10];
-            tensor_jet(t0, t1, t2, b, N);
-            nurbs_param(N, cp[STR * b + 3], W, R);
-            phys_jet(R, Gi, X2);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double U0 = cp[STR * b + 4 + k], U1 = cp[STR * b + 7 + k], U2 = cp[STR * b + 10 + k];
-                double mv = 0.5 * (U2 + U1), dv = 0.5 * (U0 - U2);
-#pragma unroll
-                for (int a = 0; a < 10; a++) { m0[10 * k + a] += R[a] * mv; dl[10 * k + a] += R[a] * dv; }
-                uv[k] += R[0] * U0; uv[3 + k] += R[0] * U1; uv[6 + k] += R[0] * U2;
-            }
-        }
-        // par_periodic adds the same affine field to every level: it enters m0 only
-        add_periodic(M, xq, m0);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double pv = M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
-            uv[k] += pv; uv[3 + k] += pv; uv[6 + k] += pv;
-        }
-        phys_residual_transient(P, m0, dl, uv, rr);
-    }
-    const double wgt = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
-
-    // projection onto the 27 basis functions in three groups of nine, each
-    // group's 27 values summed over the 64 lanes by a reduce-scatter
+        for (int a = 0; a < 30; a++) r[a] = 0.0;
 #pragma unroll 1
-    for (int grp = 0; grp < 3; grp++) {
-        double val[27];
+        for (int gg = 0; gg < 2; gg++) {
+            const int g = 2 * part + gg;
+            double us[30];
 #pragma unroll
-        for (int bb = 0; bb < 9; bb++) {
-            const int b = 9 * grp + bb;
-            double N[10], R[10];
-            tensor_jet(t0, t1, t2, b, N);
-            nurbs_param(N, cp[STR * b + 3], W, R);
-            phys_jet(R, Gi, X2);
+            for (int a = 0; a < 30; a++) us[a] = 0.5 * (J[30 + a] + J[60 + a]) + P.s[g] * (0.5 * (J[a] - J[60 + a]));
+            State st;
+            phys_state(us, P, st);
+            phys_residual_acc(st, P.cr[g], r);
+        }
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double s = 0.0;
+        for (int a = 0; a < 30; a++) {
+            r[a] += __shfl_xor(r[a], 1, 64);
+            r[a] += __shfl_xor(r[a], 2, 64);
+        }
+        if (part == 0) {
 #pragma unroll
-                for (int a = 0; a < 10; a++) s += R[a] * rr[10 * k + a];
-                val[3 * bb + k] = wgt * s;
+            for (int k = 0; k < 3; k++)
+                r[10 * k] = (P.cii * (J[10 * k] - J[60 + 10 * k])) / (2. * P.dt)
+                          + (P.rho * (J[10 * k] + J[60 + 10 * k] - 2 * J[30 + 10 * k])) / (P.dt * P.dt);
+#pragma unroll
+            for (int a = 0; a < 30; a++) sJet[q * 30 * NH + a] = r[a];
+        }
+    }
+    __syncthreads();
+
+    // ---- T: per (Gauss point, component) adjoint of the basis maps, weighted
+    if ((tid & 3) != 0) {
+        const int q = tid >> 2, k = (tid & 3) - 1;
+        const double *g = sGeo + q * GEO_R;
+        double o[10];
+        double *rq = sJet + q * 30 * NH + 10 * k;
+#pragma unroll
+        for (int a = 0; a < 10; a++) o[a] = g[0] * rq[a];
+        phys_jet_adj(o, g + 1, g + 10);
+        nurbs_param_adj(o, g + 28);
+#pragma unroll
+        for (int a = 0; a < 10; a++) rq[a] = o[a];
+    }
+    __syncthreads();
+    //      x^T: C1[k][bx][qy][qz][s6] = sum_qx sum_{a: S6(a) = s6} rho[q][k][a] tx[qx][DX(a)][bx]
+    double *C1 = sWork, *C2 = sWork + 864;
+    for (int i = tid; i < 864; i += 256) {
+        const int s6 = i % 6, t = i / 6, qz = t & 3, qy = (t >> 2) & 3, bx = (t >> 4) % 3, k = t / 48;
+        double acc = 0.0;
+#pragma unroll
+        for (int qx = 0; qx < 4; qx++) {
+            const double *rq = sJet + (16 * qx + 4 * qy + qz) * 30 * NH + 10 * k;
+            const double *tx = &sT[0][qx][0][0];   // tx[3 * dx + bx]
+            switch (s6) {
+                case 0: acc += rq[0] * tx[bx] + rq[1] * tx[3 + bx] + rq[4] * tx[6 + bx]; break;
+                case 1: acc += rq[2] * tx[bx] + rq[5] * tx[3 + bx]; break;
+                case 2: acc += rq[3] * tx[bx] + rq[6] * tx[3 + bx]; break;
+                case 3: acc += rq[7] * tx[bx]; break;
+                case 4: acc += rq[8] * tx[bx]; break;
+                default: acc += rq[9] * tx[bx]; break;
             }
         }
-        double v14[14], v7[7], v4[4], v2[2], v1a[1], v1b[1];
-        int base = 0, cnt = 27;
-        rs_step<27>(val, v14, 32, lane & 32); rs_track(27, lane & 32, base, cnt);
-        rs_step<14>(v14, v7, 16, lane & 16);  rs_track(14, lane & 16, base, cnt);
-        rs_step<7>(v7, v4, 8, lane & 8);      rs_track(7, lane & 8, base, cnt);
-        rs_step<4>(v4, v2, 4, lane & 4);      rs_track(4, lane & 4, base, cnt);
-        rs_step<2>(v2, v1a, 2, lane & 2);     rs_track(2, lane & 2, base, cnt);
-        rs_step<1>(v1a, v1b, 1, lane & 1);    rs_track(1, lane & 1, base, cnt);
-        if (valid && cnt > 0) Re[(int64_t)e * 81 + 27 * grp + base] = v1b[0];
+        C1[i] = acc;   // [k][bx][qy][qz][s6]
+    }
+    __syncthreads();
+    //      y^T: C2[k][bx][by][qz][dz] = sum_qy sum_{s6: dz(s6) = dz} C1[..][s6] ty[qy][dy(s6)][by]
+    for (int i = tid; i < 324; i += 256) {
+        const int dz = i % 3, t = i / 3, qz = t & 3, u = t >> 2, by = u % 3, bx = (u / 3) % 3, k = u / 9;
+        double acc = 0.0;
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const double *c = C1 + (((k * 3 + bx) * 4 + qy) * 4 + qz) * 6;
+            const double *ty = &sT[1][qy][0][0];   // ty[3 * dy + by]
+            if (dz == 0) acc += c[0] * ty[by] + c[1] * ty[3 + by] + c[3] * ty[6 + by];
+            else if (dz == 1) acc += c[2] * ty[by] + c[4] * ty[3 + by];
+            else acc += c[5] * ty[by];
+        }
+        C2[i] = acc;   // [k][bx][by][qz][dz]
+    }
+    __syncthreads();
+    //      z^T: R_e[3 b + k] = w_b sum_qz sum_dz C2[k][bx][by][qz][dz] tz[qz][dz][bz]
+    if (tid < 81) {
+        const int k = tid % 3, b = tid / 3, bx = b / 9, by = (b / 3) % 3, bz = b % 3;
+        const double *c = C2 + ((k * 3 + bx) * 3 + by) * 12;
+        double acc = 0.0;
+#pragma unroll
+        for (int qz = 0; qz < 4; qz++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) acc += c[3 * qz + dz] * sT[2][qz][dz][bz];
+        Re[(int64_t)e * 81 + tid] = sCP[STR * b + 3] * acc;
     }
 }
 
@@ -770,7 +845,7 @@ namespace igap {
 
 hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Re, int nel, hipStream_t s)
 {
-    dim3 grid((nel + 3) / 4), block(256);
+    dim3 grid(nel), block(256);
     if (P.torder == 0) hipLaunchKernelGGL(k_residual_elem<1>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     else hipLaunchKernelGGL(k_residual_elem<3>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     return hipGetLastError();

@@ -44,6 +44,8 @@ struct Solver {
     double *h = nullptr;            // m+1 device
     float *fa = nullptr, *fb = nullptr;
     double *diag = nullptr;         // Jacobi
+    double *t2 = nullptr;           // second sweep buffer
+    int64_t *dpos = nullptr;        // CSR position of each row's diagonal
 };
 
 void solver_destroy(Solver *s)
@@ -59,7 +61,7 @@ void solver_destroy(Solver *s)
     }
     if (s->bl) rocblas_destroy_handle(s->bl);
     void *bufs[] = {s->ilu[0], s->ilu[1], s->buf[0], s->buf[1], s->rowptr32, s->V, s->w, s->z, s->t, s->d, s->x,
-                    s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag};
+                    s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag, s->t2, s->dpos};
     for (void *b : bufs)
         if (b) hipFree(b);
     delete s;
@@ -78,6 +80,39 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) y[row] = s;
+}
+
+// one Jacobi sweep of an ILU(0) triangle stored in the combined CSR factor:
+//   lower (unit diagonal):  yout = b - L_strict yin
+//   upper:                  yout = (b - U_strict yin) / U_diag   (yin = null: 0)
+// one wave per row
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_tri_sweep(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                   const int64_t *__restrict__ dpos, const T *__restrict__ val,
+                                                   const double *__restrict__ b, const double *__restrict__ yin,
+                                                   double *__restrict__ yout)
+{
+    int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const int64_t dp = dpos[row];
+    const int64_t j0 = LOWER ? rp[row] : dp + 1, j1 = LOWER ? dp : rp[row + 1];
+    double s = 0.0;
+    if (yin)
+        for (int64_t j = j0 + lane; j < j1; j += 64) s += (double)val[j] * yin[col[j]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) yout[row] = LOWER ? b[row] - s : (b[row] - s) / (double)val[dp];
+}
+
+__global__ void k_diagpos(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t *__restrict__ dpos)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int64_t d = rp[r];
+    for (int64_t j = rp[r]; j < rp[r + 1]; j++)
+        if (col[j] == r) d = j;
+    dpos[r] = d;
 }
 
 template <typename T>
@@ -173,7 +208,7 @@ static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
         if (c->nnz > 0x7fffffffLL || n > 0x7fffffffLL)
             return fail(IGAP_ERR_ARG, "ILU(0) through rocSPARSE needs nnz < 2^31 (nnz = %lld)", (long long)c->nnz);
         HIPCHK(hipMalloc(&s->V, sizeof(double) * (size_t)(m + 1) * n));
-        double **vecs[] = {&s->w, &s->z, &s->t, &s->d, &s->x, &s->xt, &s->F, &s->Ft, &s->diag};
+        double **vecs[] = {&s->w, &s->z, &s->t, &s->t2, &s->d, &s->x, &s->xt, &s->F, &s->Ft, &s->diag};
         for (double **v : vecs) HIPCHK(hipMalloc(v, sizeof(double) * n));
         HIPCHK(hipMalloc(&s->h, sizeof(double) * (m + 2)));
         HIPCHK(hipMalloc(&s->fa, sizeof(float) * n));
@@ -181,6 +216,9 @@ static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
         HIPCHK(hipMalloc(&s->rowptr32, sizeof(int32_t) * (n + 1)));
         hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, n + 1, c->d_rowptr,
                            s->rowptr32);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMalloc(&s->dpos, sizeof(int64_t) * n));
+        hipLaunchKernelGGL(k_diagpos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, s->dpos);
         HIPCHK(hipGetLastError());
         SPCHK(rocsparse_create_handle(&s->sp));
         SPCHK(rocsparse_set_stream(s->sp, c->stream));
@@ -257,10 +295,43 @@ static int pc_ilu_setup(igap_gpu_ctx *c, Solver *s, int p)
     return IGAP_OK;
 }
 
-// y = M^{-1} x (x and y must differ from s->t)
-static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, const double *x, double *y)
+template <typename T>
+static int ilu_sweeps(igap_gpu_ctx *c, Solver *s, int sw, const T *f, const double *x, double *y)
+{
+    const int64_t n = c->ndof;
+    const dim3 g((unsigned)((n + 3) / 4)), b(256);
+    double *A = s->t, *B = s->t2;
+    // L y = x from y_0 = x
+    const double *yin = x;
+    double *yout = A;
+    for (int k = 0; k < sw; k++) {
+        hipLaunchKernelGGL((k_tri_sweep<T, true>), g, b, 0, c->stream, n, c->d_rowptr, c->d_col, s->dpos, f, x, yin, yout);
+        yin = yout;
+        yout = (yout == A) ? B : A;
+    }
+    const double *yl = yin;
+    double *other = (yl == A) ? B : A;
+    // U z = yl from z_0 = D^{-1} yl; the last of the sw + 1 launches writes y
+    const int nl = sw + 1;
+    double *out = (nl % 2 == 1) ? y : other;
+    const double *zin = nullptr;
+    for (int k = 0; k < nl; k++) {
+        hipLaunchKernelGGL((k_tri_sweep<T, false>), g, b, 0, c->stream, n, c->d_rowptr, c->d_col, s->dpos, f, yl, zin, out);
+        zin = out;
+        out = (out == y) ? other : y;
+    }
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
+// y = M^{-1} x (x and y must differ from s->t and s->t2)
+static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int sweeps, const double *x, double *y)
 {
     int64_t n = c->ndof;
+    if (pc == IGAP_PC_ILU0 && sweeps > 0) {
+        if (p == IGAP_F64) return ilu_sweeps<double>(c, s, sweeps, (const double *)s->ilu[p], x, y);
+        return ilu_sweeps<float>(c, s, sweeps, (const float *)s->ilu[p], x, y);
+    }
     if (pc == IGAP_PC_JACOBI) {
         hipLaunchKernelGGL(k_mul, grid1(n), dim3(256), 0, c->stream, n, s->diag, x, y);
         HIPCHK(hipGetLastError());
@@ -333,7 +404,7 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
         int j = 0;
         for (; j < m && total < o->ksp_maxit; j++) {
             double *vj = s->V + (size_t)j * n, *vn = s->V + (size_t)(j + 1) * n;
-            rc = pc_apply(c, s, o->pc, p, vj, s->z);
+            rc = pc_apply(c, s, o->pc, p, o->pc_sweeps, vj, s->z);
             if (rc) return rc;
             rc = spmv(c, p, s->z, vn);
             if (rc) return rc;
@@ -378,7 +449,7 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
         HIPCHK(hipMemcpyAsync(s->h, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
         const double one = 1.0, zero = 0.0;
         BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j, &one, s->V, N, s->h, 1, &zero, s->w, 1));
-        rc = pc_apply(c, s, o->pc, p, s->w, s->z);
+        rc = pc_apply(c, s, o->pc, p, o->pc_sweeps, s->w, s->z);
         if (rc) return rc;
         hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0, s->z, 1.0, d);
         if (rnorm <= o->ksp_rtol * bnorm || rnorm <= o->ksp_atol) break;
@@ -410,6 +481,7 @@ extern "C" int igap_gpu_solver_default_opts(igap_solver_opts *o)
     o->pc = IGAP_PC_ILU0;
     o->tangent_prec = IGAP_F64;
     o->linesearch = 1;
+    o->pc_sweeps = 0;
     return IGAP_OK;
 }
 

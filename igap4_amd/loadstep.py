@@ -21,14 +21,18 @@ def tip_bc(tip_total, s, tip_dof=2):
 
 
 def run_load_steps(A: Assm, nsteps, tip_total=0.05, guess="previous", opts=None, tip_dof=2,
-                   history=None, callback=None):
+                   history=None, callback=None, max_steps=None, start=1, prev=None):
     """Ramp the tip displacement over ``nsteps`` load steps, solving each to the
-    tolerances of ``opts``.  ``guess`` = "previous" (the reference's behaviour)
-    or "extrapolate" (2 U_k - U_{k-1}).  Returns a list of per-step dicts."""
+    tolerances of ``opts``.  ``guess`` = "previous" (the reference's behaviour),
+    "extrapolate" (2 U_k - U_{k-1}) or a callable ``guess(k, prev)`` returning
+    the initial iterate from the list of previous solutions.  ``start`` and
+    ``prev`` (solutions up to step start-1, last one current) resume a ramp;
+    ``max_steps`` stops after that step.  Returns a list of per-step dicts."""
     out = []
     n = A.ndof
-    prev = [A.get_uhist()[:n].copy()]
-    for k in range(1, nsteps + 1):
+    prev = [A.get_uhist()[:n].copy()] if prev is None else [np.asarray(u)[:n].copy() for u in prev]
+    last = nsteps if max_steps is None else min(nsteps, max_steps)
+    for k in range(start, last + 1):
         s = k / nsteps
         A.set_bc(tip_bc(tip_total, s, tip_dof))
         if guess == "extrapolate" and len(prev) >= 2:
@@ -53,3 +57,56 @@ def run_load_steps(A: Assm, nsteps, tip_total=0.05, guess="previous", opts=None,
         if st["converged"] <= 0:
             break
     return out
+
+
+def run_time_steps(A: Assm, nsteps, opts=None, guess="previous", callback=None):
+    """Transient steps of gradelasttime: before each step the history levels
+    shift (two back <- previous <- last converged) and level 0 gets the initial
+    iterate ("previous": the last converged state, the reference's behaviour;
+    "extrapolate": 2 U_n - U_{n-1}).  U_hist must hold [U_n, U_n, U_{n-1}] or
+    any state whose levels 1 and 2 are the last two time levels.  Returns a
+    list of per-step dicts."""
+    n = A.ndof
+    H = A.get_uhist().reshape(-1, n)
+    if H.shape[0] < 3:
+        raise ValueError("run_time_steps needs a transient context (three history levels)")
+    u1, u0 = H[1].copy(), H[2].copy()
+    out = []
+    for k in range(1, nsteps + 1):
+        if k > 1:
+            u0, u1 = u1, H[0].copy()
+        U = u1.copy() if guess == "previous" else 2.0 * u1 - u0
+        A.set_uhist(np.concatenate([U, u1, u0]))
+        t0 = time.perf_counter()
+        st = A.snes_solve(opts)
+        st["wall_ms"] = (time.perf_counter() - t0) * 1e3
+        st["step"] = k
+        H = A.get_uhist().reshape(-1, n)
+        if callback:
+            callback(k, H, st)
+        out.append(st)
+        if st["converged"] <= 0:
+            break
+    return out
+
+
+def initial_velocity_modes(X, L, dt, amp=0.01, nmodes=8, seed=20261202):
+    """The transient workload's start: zero displacement and the velocity
+    v0 = amp * sum_m d_m sin(pi k_m . X / L + phi_m) over ``nmodes`` modes with
+    integer wave vectors k_m in {1,2}^3, random unit directions d_m and phases
+    phi_m (numpy default_rng(seed)); returns the history levels (u1, u0) with
+    u1 = 0 and u0 = u1 - dt v0 at the control points X[ncp][3]."""
+    rng = np.random.default_rng(seed)
+    X = np.asarray(X).reshape(-1, 3)
+    L = np.asarray(L, dtype=np.float64)
+    v = np.zeros_like(X)
+    for _ in range(nmodes):
+        kv = rng.integers(1, 3, size=3)
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        phi = rng.uniform(0.0, 2.0 * np.pi)
+        v += np.outer(np.sin(np.pi * (X / L) @ kv + phi), d)
+    v *= amp
+    u1 = np.zeros(X.size)
+    u0 = u1 - dt * v.reshape(-1)
+    return u1, u0

@@ -112,6 +112,15 @@ int igap_gpu_sizes(const igap_gpu_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t
 /* core->U_hist (include/coreimpl.h:17): (torder+1)*ndof doubles, level h at
  * offset h*ndof, 0 = current iterate (src/interface.c:127). */
 double *igap_gpu_uhist(igap_gpu_ctx *ctx);
+/* Solution files (IGAPSolvSaveSolution / IGAPSolvLoadSolution,
+ * src/interface.c:271-300, write_solution / read_solution src/ioutil.c:90-159):
+ * raw FP64 in the reference's universal dof order (on one GPU the owned order
+ * 3*cp + i), holding U_hist level 0 for the static physics and levels
+ * 1..torder for the transient one.  Loading a transient file also sets level
+ * 0 = level 1 (the next step's initial iterate).  Synchronous; a short or
+ * missing file is an error, not an abort. */
+int igap_gpu_save_solution(igap_gpu_ctx *ctx, const char *fname);
+int igap_gpu_load_solution(igap_gpu_ctx *ctx, const char *fname);
 /* device CSR pattern (rows = owned dofs dof = 3*cp+i, columns sorted) */
 const int64_t *igap_gpu_csr_rowptr(const igap_gpu_ctx *ctx);
 const int32_t *igap_gpu_csr_col(const igap_gpu_ctx *ctx);
@@ -151,6 +160,9 @@ typedef struct {
     int    pc;            /* IGAP_PC_ILU0 (the reference's ASM/ILU(0) on one part), ...     */
     int    tangent_prec;  /* IGAP_F64 or IGAP_F32 (FP32 matrix-core tangent, FP32 ILU)      */
     int    linesearch;    /* 1: backtracking (SNESLINESEARCHBT, cubic), 0: full steps       */
+    int    pc_sweeps;     /* ILU(0) triangular solves: 0 exact (level-scheduled), k > 0    */
+                          /* k Jacobi sweeps per triangle (an approximation of the same     */
+                          /* factors that trades exactness for parallelism)                 */
 } igap_solver_opts;
 
 typedef struct {

@@ -1,0 +1,131 @@
+"""Learned warm start on the GPU (include/igap_ws.h): the BF16 matrix-core
+layout, forward values and gradients against the CPU oracle with the same
+BF16 rounding (oracle/convnet.py), Adam against the oracle's Adam, training
+that lowers the loss, and the load-step ablation: the learned guess is used
+only when its residual beats extrapolation, and on a nonlinear ramp it does."""
+import numpy as np
+import pytest
+
+from oracle import convnet as cn
+
+pytestmark = pytest.mark.gpu
+
+GRID = (10, 6, 5)
+CIN, COUT = 9, 3
+DIL = (1, 2, 4, 1)
+
+
+def random_params(rng):
+    p = np.zeros(cn.NL * cn.LAYER_P, dtype=np.float32)
+    for l in range(cn.NL):
+        o = l * cn.LAYER_P
+        fan = 27 * (CIN if l == 0 else cn.CH)
+        W = rng.uniform(-1, 1, (27, cn.CH, cn.CH)) * np.sqrt(6.0 / fan)
+        if l == 0:
+            W[:, :, CIN:] = 0.0
+        if l == cn.NL - 1:
+            W[:, COUT:, :] = 0.0
+        p[o:o + cn.LAYER_W] = W.reshape(-1)
+        p[o + cn.LAYER_W:o + cn.LAYER_P] = rng.standard_normal(cn.CH) * 0.05
+    return p
+
+
+@pytest.fixture(scope="module")
+def ws(igap):
+    from igap4_amd import warmstart
+    return warmstart
+
+
+def test_bf16_mfma_layout(ws):
+    """Result register r of lane l holds D[4(l>>4)+r][l&15] (f32 16x16 rule)."""
+    out = ws.selftest_mfma_bf16()
+    A = np.array([[1 + i + 0.25 * k for k in range(16)] for i in range(16)])
+    B = np.array([[2 - 0.5 * n + k for n in range(16)] for k in range(16)])
+    D = A @ B
+    exp = np.array([[D[4 * (l >> 4) + r, l & 15] for l in range(64)] for r in range(4)])
+    assert np.array_equal(out, exp.astype(np.float32))
+
+
+def test_forward_matches_oracle(ws):
+    rng = np.random.default_rng(0)
+    p = random_params(rng)
+    V = int(np.prod(GRID))
+    x = rng.standard_normal((V, CIN)).astype(np.float32)
+    with ws.WarmStartNet(GRID, CIN, COUT, DIL) as net:
+        net.set_params(p)
+        y = net.forward(x)
+    y_ref, _ = cn.forward(p, x, GRID, CIN, COUT, DIL, emulate=True)
+    err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    print(f"  forward rel err {err:.2e}")
+    assert err < 5e-3
+
+
+def test_gradients_match_oracle(ws):
+    rng = np.random.default_rng(1)
+    p = random_params(rng)
+    V = int(np.prod(GRID))
+    x = rng.standard_normal((V, CIN)).astype(np.float32)
+    t = rng.standard_normal((V, COUT)).astype(np.float32)
+    with ws.WarmStartNet(GRID, CIN, COUT, DIL) as net:
+        net.set_params(p)
+        loss, g = net.gradients(x, t)
+    loss_ref, g_ref = cn.gradients(p, x, t, GRID, CIN, COUT, DIL, emulate=True)
+    assert abs(loss - loss_ref) < 1e-3 * loss_ref
+    for l in range(cn.NL):
+        o = l * cn.LAYER_P
+        for name, sl in (("W", slice(o, o + cn.LAYER_W)), ("b", slice(o + cn.LAYER_W, o + cn.LAYER_P))):
+            a, b = g[sl], g_ref[sl]
+            err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            print(f"  layer {l} d{name} rel err {err:.2e}")
+            assert err < 2e-2
+
+
+def test_adam_step_matches_oracle(ws):
+    rng = np.random.default_rng(2)
+    p = random_params(rng)
+    V = int(np.prod(GRID))
+    x = rng.standard_normal((V, CIN)).astype(np.float32)
+    t = rng.standard_normal((V, COUT)).astype(np.float32)
+    with ws.WarmStartNet(GRID, CIN, COUT, DIL) as net:
+        net.set_params(p)
+        _, g = net.gradients(x, t)
+        net.train_step(x, t, 1e-3)
+        p1 = net.get_params()
+    z = np.zeros_like(p, dtype=np.float64)
+    p_ref, _, _ = cn.adam(p.astype(np.float64), g.astype(np.float64), z, z, 1, 1e-3)
+    assert np.max(np.abs(p1 - p_ref)) < 1e-6
+
+
+def test_training_lowers_the_loss(ws):
+    """Fit a smooth linear map of the inputs (what a second-order
+    extrapolation is): the loss falls by more than 10x."""
+    rng = np.random.default_rng(3)
+    V = int(np.prod(GRID))
+    xs, ts = [], []
+    for _ in range(8):
+        x = rng.standard_normal((V, CIN)).astype(np.float32)
+        xs.append(x)
+        ts.append((0.5 * (x[:, 0:3] - x[:, 3:6])).astype(np.float32))
+    with ws.WarmStartNet(GRID, CIN, COUT, DIL, seed=5) as net:
+        first = None
+        for ep in range(150):
+            tot = sum(net.train_step(x, t, 3e-3) for x, t in zip(xs, ts)) / len(xs)
+            first = tot if first is None else first
+        print(f"  loss {first:.3e} -> {tot:.3e}")
+        assert tot < 0.1 * first
+
+
+def test_warm_start_ablation_small_beam(ws, igap):
+    """Config-4 scenario at the config-1 size (16x4x4, a ramp of 60 steps to a
+    large tip displacement): trained on steps 1..30, evaluated on 31..60 with
+    and without the network.  The accepted guesses beat extrapolation by
+    construction; the network must be accepted on most steps and both runs
+    must reach the same converged answer."""
+    s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=80, lr=2e-3, verbose=False)
+    print({k: s[k] for k in ("learned", "extrapolation", "accepted_fraction",
+                             "median_residual_ratio_net_over_extrapolation", "loss_first", "loss_last")})
+    assert s["learned"]["all_converged"] and s["extrapolation"]["all_converged"]
+    assert s["loss_last"] < s["loss_first"]
+    assert s["accepted_fraction"] > 0.5
+    assert s["median_residual_ratio_net_over_extrapolation"] < 1.0
+    assert s["final_rel_diff"] < 1e-8
