@@ -12,15 +12,22 @@
 //     dependent dot products; m is bounded because 10000 Krylov vectors of
 //     the large patches do not fit),
 //   * ILU(0) of the assembled tangent (rocSPARSE csrilu0 + csrsv) in the
-//     tangent's precision, FP64 or FP32,
+//     tangent's precision, FP64 or FP32, in the natural dof order (the
+//     reference's) or in a 27-colour order of the control points (colour
+//     9 (i mod 3) + 3 (j mod 3) + (k mod 3), points of one colour share no
+//     stencil entry, the three dofs of a point stay together): the natural
+//     order's triangular solves have ~9 n_x levels of dependent rows, the
+//     coloured order's at most 81,
 //   * a wave-per-row CSR SpMV of this file.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsparse/rocsparse.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "igap_ctx.h"
@@ -44,16 +51,27 @@ struct Solver {
     double *h = nullptr;            // m+1 device
     float *fa = nullptr, *fb = nullptr;
     double *diag = nullptr;         // Jacobi
-    double *t2 = nullptr;           // second sweep buffer
-    int64_t *dpos = nullptr;        // CSR position of each row's diagonal
+    double *t2 = nullptr;           // second permuted work vector
+    // coloured ordering: perm[new] = old dof, permuted CSR and, per permuted
+    // nonzero, its position in the natural CSR
+    int32_t *mc_perm = nullptr, *mc_rowptr = nullptr, *mc_col = nullptr, *mc_src = nullptr, *mc_dpos = nullptr;
+    int mc_cstart[28] = {};         // first row of each colour (rows of a colour are contiguous)
+    int mc_sweep_ok = 0;            // no two points of one colour are coupled (false for a periodic
+                                    // direction whose size is not a multiple of 3): colour sweeps valid
+    rocsparse_mat_info mc_info[2] = {nullptr, nullptr};
+    void *mc_ilu[2] = {nullptr, nullptr};
+    void *mc_buf[2] = {nullptr, nullptr};
+    int mc_analysed[2] = {0, 0};
 };
 
 void solver_destroy(Solver *s)
 {
     if (!s) return;
     if (s->sp) {
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < 2; p++) {
             if (s->info[p]) rocsparse_destroy_mat_info(s->info[p]);
+            if (s->mc_info[p]) rocsparse_destroy_mat_info(s->mc_info[p]);
+        }
         if (s->dA) rocsparse_destroy_mat_descr(s->dA);
         if (s->dL) rocsparse_destroy_mat_descr(s->dL);
         if (s->dU) rocsparse_destroy_mat_descr(s->dU);
@@ -61,7 +79,8 @@ void solver_destroy(Solver *s)
     }
     if (s->bl) rocblas_destroy_handle(s->bl);
     void *bufs[] = {s->ilu[0], s->ilu[1], s->buf[0], s->buf[1], s->rowptr32, s->V, s->w, s->z, s->t, s->d, s->x,
-                    s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag, s->t2, s->dpos};
+                    s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag, s->t2, s->mc_perm, s->mc_rowptr,
+                    s->mc_col, s->mc_src, s->mc_dpos, s->mc_ilu[0], s->mc_ilu[1], s->mc_buf[0], s->mc_buf[1]};
     for (void *b : bufs)
         if (b) hipFree(b);
     delete s;
@@ -82,49 +101,95 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
     if (lane == 0) y[row] = s;
 }
 
-// one Jacobi sweep of an ILU(0) triangle stored in the combined CSR factor:
-//   lower (unit diagonal):  yout = b - L_strict yin
-//   upper:                  yout = (b - U_strict yin) / U_diag   (yin = null: 0)
-// one wave per row
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_tri_sweep(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
-                                                   const int64_t *__restrict__ dpos, const T *__restrict__ val,
-                                                   const double *__restrict__ b, const double *__restrict__ yin,
-                                                   double *__restrict__ yout)
+// y[i] = x[idx[i]]  (values or vectors into the coloured order)
+// (grid-stride: launched with grid1)
+template <typename T, typename U>
+__global__ void k_gather(int64_t n, const int32_t *__restrict__ idx, const T *__restrict__ x, U *__restrict__ y)
 {
-    int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    const int64_t dp = dpos[row];
-    const int64_t j0 = LOWER ? rp[row] : dp + 1, j1 = LOWER ? dp : rp[row + 1];
-    double s = 0.0;
-    if (yin)
-        for (int64_t j = j0 + lane; j < j1; j += 64) s += (double)val[j] * yin[col[j]];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) yout[row] = LOWER ? b[row] - s : (b[row] - s) / (double)val[dp];
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) y[i] = (U)x[idx[i]];
 }
 
-__global__ void k_diagpos(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t *__restrict__ dpos)
+// y[idx[i]] = x[i]  (back to the natural order; grid-stride)
+__global__ void k_scatter(int64_t n, const int32_t *__restrict__ idx, const double *__restrict__ x, double *__restrict__ y)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    int64_t d = rp[r];
-    for (int64_t j = rp[r]; j < rp[r + 1]; j++)
-        if (col[j] == r) d = j;
-    dpos[r] = d;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) y[idx[i]] = x[i];
+}
+
+// Triangular solves of the ILU(0) factor in the coloured order, one colour
+// per launch: points of one colour share no stencil entry, so every point's
+// three rows depend only on earlier colours (lower) or later ones (upper) and
+// on the point's own 3x3 diagonal block, solved exactly by lane 0.  One wave
+// per control point; rows r = r0 + 3 qi + d, d = 0..2.  The factor holds L
+// (unit diagonal, strictly lower) and U (upper with diagonal) in place;
+// dpos[r] is the position of row r's diagonal and the point's own columns
+// sit next to it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mc_lower(int r0, int npts, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const int32_t *__restrict__ dpos, const T *__restrict__ f,
+                                                  const double *__restrict__ b, double *__restrict__ y)
+{
+    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (qi >= npts) return;
+    const int r = r0 + 3 * qi;
+    double s[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double a = 0.0;
+        for (int j = rp[r + d] + lane; j < dpos[r + d] - d; j += 64) a += (double)f[j] * y[col[j]];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        s[d] = a;
+    }
+    if (lane == 0) {
+        const double y0 = b[r] - s[0];
+        const double y1 = b[r + 1] - s[1] - (double)f[dpos[r + 1] - 1] * y0;
+        const double y2 = b[r + 2] - s[2] - (double)f[dpos[r + 2] - 2] * y0 - (double)f[dpos[r + 2] - 1] * y1;
+        y[r] = y0;
+        y[r + 1] = y1;
+        y[r + 2] = y2;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mc_upper(int r0, int npts, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const int32_t *__restrict__ dpos, const T *__restrict__ f,
+                                                  const double *__restrict__ y, double *__restrict__ z)
+{
+    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (qi >= npts) return;
+    const int r = r0 + 3 * qi;
+    double s[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double a = 0.0;
+        for (int j = dpos[r + d] + 3 - d + lane; j < rp[r + d + 1]; j += 64) a += (double)f[j] * z[col[j]];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        s[d] = a;
+    }
+    if (lane == 0) {
+        const double z2 = (y[r + 2] - s[2]) / (double)f[dpos[r + 2]];
+        const double z1 = (y[r + 1] - s[1] - (double)f[dpos[r + 1] + 1] * z2) / (double)f[dpos[r + 1]];
+        const double z0 = (y[r] - s[0] - (double)f[dpos[r] + 1] * z1 - (double)f[dpos[r] + 2] * z2) / (double)f[dpos[r]];
+        z[r] = z0;
+        z[r + 1] = z1;
+        z[r + 2] = z2;
+    }
 }
 
 template <typename T>
 __global__ void k_diag_inv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const T *__restrict__ val,
                            double *__restrict__ dinv)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double dv = 1.0;
-    for (int64_t j = rp[r]; j < rp[r + 1]; j++)
-        if (col[j] == r) dv = (double)val[j];
-    dinv[r] = dv != 0.0 ? 1.0 / dv : 1.0;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;   // grid-stride: launched with grid1
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += st) {
+        double dv = 1.0;
+        for (int64_t j = rp[r]; j < rp[r + 1]; j++)
+            if (col[j] == r) dv = (double)val[j];
+        dinv[r] = dv != 0.0 ? 1.0 / dv : 1.0;
+    }
 }
 
 __global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
@@ -217,9 +282,6 @@ static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
         hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, n + 1, c->d_rowptr,
                            s->rowptr32);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMalloc(&s->dpos, sizeof(int64_t) * n));
-        hipLaunchKernelGGL(k_diagpos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, s->dpos);
-        HIPCHK(hipGetLastError());
         SPCHK(rocsparse_create_handle(&s->sp));
         SPCHK(rocsparse_set_stream(s->sp, c->stream));
         SPCHK(rocsparse_create_mat_descr(&s->dA));
@@ -234,6 +296,142 @@ static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
         BLCHK(rocblas_set_pointer_mode(s->bl, rocblas_pointer_mode_host));
     }
     *out = s;
+    return IGAP_OK;
+}
+
+// the coloured ordering and the permuted CSR pattern (host, once per context)
+static int mc_build(igap_gpu_ctx *c, Solver *s)
+{
+    if (s->mc_perm) return IGAP_OK;
+    const int64_t n = c->ndof, nnz = c->nnz, ncp = n / 3;
+    const int nb1 = c->nb[1], nb2 = c->nb[2];
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> col(nnz);
+    HIPCHK(hipMemcpyAsync(rp.data(), c->d_rowptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(col.data(), c->d_col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // control points by (colour, index); dofs 3 cp + i keep their point's place
+    std::vector<int32_t> cps(ncp);
+    std::iota(cps.begin(), cps.end(), 0);
+    auto colour = [&](int64_t a) {
+        const int64_t i = a / ((int64_t)nb1 * nb2), j = (a / nb2) % nb1, k = a % nb2;
+        return (int)(9 * (i % 3) + 3 * (j % 3) + (k % 3));
+    };
+    std::stable_sort(cps.begin(), cps.end(), [&](int32_t a, int32_t b) { return colour(a) < colour(b); });
+    std::vector<int32_t> perm(n), iperm(n);
+    for (int64_t q = 0; q < ncp; q++)
+        for (int d = 0; d < 3; d++) {
+            perm[3 * q + d] = 3 * cps[q] + d;
+            iperm[3 * cps[q] + d] = (int32_t)(3 * q + d);
+        }
+    std::vector<int32_t> prp(n + 1), pcol(nnz), psrc(nnz);
+    std::vector<std::pair<int32_t, int32_t>> row;
+    prp[0] = 0;
+    for (int64_t r = 0; r < n; r++) {
+        const int64_t o = perm[r];
+        row.clear();
+        for (int64_t j = rp[o]; j < rp[o + 1]; j++) row.emplace_back(iperm[col[j]], (int32_t)j);
+        std::sort(row.begin(), row.end());
+        const int64_t b = prp[r];
+        for (size_t t = 0; t < row.size(); t++) {
+            pcol[b + t] = row[t].first;
+            psrc[b + t] = row[t].second;
+        }
+        prp[r + 1] = (int32_t)(b + (int64_t)row.size());
+    }
+    std::vector<int32_t> pdpos(n);
+    for (int64_t r = 0; r < n; r++)
+        for (int32_t j = prp[r]; j < prp[r + 1]; j++)
+            if (pcol[j] == r) pdpos[r] = j;
+    for (int cc = 0; cc <= 27; cc++) s->mc_cstart[cc] = 0;
+    {
+        int64_t q = 0;
+        for (int cc = 0; cc < 27; cc++) {
+            s->mc_cstart[cc] = (int)(3 * q);
+            while (q < ncp && colour(cps[q]) == cc) q++;
+        }
+        s->mc_cstart[27] = (int)(3 * q);
+    }
+    s->mc_sweep_ok = 1;
+    for (int64_t r = 0; r < n && s->mc_sweep_ok; r++) {
+        const int64_t q0 = 3 * (r / 3);
+        for (int32_t j = prp[r]; j < prp[r + 1]; j++) {
+            const int32_t cj = pcol[j];
+            if ((cj < q0 || cj > q0 + 2) && colour(perm[cj] / 3) == colour(perm[r] / 3)) { s->mc_sweep_ok = 0; break; }
+        }
+    }
+    HIPCHK(hipMalloc(&s->mc_dpos, sizeof(int32_t) * n));
+    HIPCHK(hipMemcpyAsync(s->mc_dpos, pdpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMalloc(&s->mc_perm, sizeof(int32_t) * n));
+    HIPCHK(hipMalloc(&s->mc_rowptr, sizeof(int32_t) * (n + 1)));
+    HIPCHK(hipMalloc(&s->mc_col, sizeof(int32_t) * nnz));
+    HIPCHK(hipMalloc(&s->mc_src, sizeof(int32_t) * nnz));
+    HIPCHK(hipMemcpyAsync(s->mc_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(s->mc_rowptr, prp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(s->mc_col, pcol.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(s->mc_src, psrc.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IGAP_OK;
+}
+
+// ILU(0) of the tangent in the coloured order, precision p
+static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p)
+{
+    int rc = mc_build(c, s);
+    if (rc) return rc;
+    const int n = (int)c->ndof, nnz = (int)c->nnz;
+    if (!s->mc_ilu[p]) HIPCHK(hipMalloc(&s->mc_ilu[p], (p == IGAP_F64 ? sizeof(double) : sizeof(float)) * (size_t)nnz));
+    if (p == IGAP_F64)
+        hipLaunchKernelGGL((k_gather<double, double>), grid1(nnz), dim3(256), 0, c->stream, (int64_t)nnz, s->mc_src, c->d_val64,
+                           (double *)s->mc_ilu[p]);
+    else
+        hipLaunchKernelGGL((k_gather<float, float>), grid1(nnz), dim3(256), 0, c->stream, (int64_t)nnz, s->mc_src, c->d_val32,
+                           (float *)s->mc_ilu[p]);
+    HIPCHK(hipGetLastError());
+    if (!s->mc_analysed[p]) {
+        SPCHK(rocsparse_create_mat_info(&s->mc_info[p]));
+        size_t b0 = 0, b1 = 0, b2 = 0;
+        if (p == IGAP_F64) {
+            double *v = (double *)s->mc_ilu[p];
+            SPCHK(rocsparse_dcsrilu0_buffer_size(s->sp, n, nnz, s->dA, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b0));
+            SPCHK(rocsparse_dcsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b1));
+            SPCHK(rocsparse_dcsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b2));
+        } else {
+            float *v = (float *)s->mc_ilu[p];
+            SPCHK(rocsparse_scsrilu0_buffer_size(s->sp, n, nnz, s->dA, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b0));
+            SPCHK(rocsparse_scsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b1));
+            SPCHK(rocsparse_scsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b2));
+        }
+        size_t bs = std::max(b0, std::max(b1, b2));
+        HIPCHK(hipMalloc(&s->mc_buf[p], bs > 0 ? bs : 1));
+        if (p == IGAP_F64) {
+            double *v = (double *)s->mc_ilu[p];
+            SPCHK(rocsparse_dcsrilu0_analysis(s->sp, n, nnz, s->dA, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                              rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->mc_buf[p]));
+            SPCHK(rocsparse_dcsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->mc_buf[p]));
+            SPCHK(rocsparse_dcsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->mc_buf[p]));
+        } else {
+            float *v = (float *)s->mc_ilu[p];
+            SPCHK(rocsparse_scsrilu0_analysis(s->sp, n, nnz, s->dA, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                              rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->mc_buf[p]));
+            SPCHK(rocsparse_scsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dL, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->mc_buf[p]));
+            SPCHK(rocsparse_scsrsv_analysis(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                            rocsparse_analysis_policy_reuse, rocsparse_solve_policy_auto, s->mc_buf[p]));
+        }
+        s->mc_analysed[p] = 1;
+    }
+    if (p == IGAP_F64)
+        SPCHK(rocsparse_dcsrilu0(s->sp, n, nnz, s->dA, (double *)s->mc_ilu[p], s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                 rocsparse_solve_policy_auto, s->mc_buf[p]));
+    else
+        SPCHK(rocsparse_scsrilu0(s->sp, n, nnz, s->dA, (float *)s->mc_ilu[p], s->mc_rowptr, s->mc_col, s->mc_info[p],
+                                 rocsparse_solve_policy_auto, s->mc_buf[p]));
+    rocsparse_int zp = -1;
+    rocsparse_status st = rocsparse_csrilu0_zero_pivot(s->sp, s->mc_info[p], &zp);
+    if (st == rocsparse_status_zero_pivot) return fail(IGAP_ERR_STATE, "ILU(0), coloured order: zero pivot in row %d", (int)zp);
     return IGAP_OK;
 }
 
@@ -295,42 +493,62 @@ static int pc_ilu_setup(igap_gpu_ctx *c, Solver *s, int p)
     return IGAP_OK;
 }
 
-template <typename T>
-static int ilu_sweeps(igap_gpu_ctx *c, Solver *s, int sw, const T *f, const double *x, double *y)
-{
-    const int64_t n = c->ndof;
-    const dim3 g((unsigned)((n + 3) / 4)), b(256);
-    double *A = s->t, *B = s->t2;
-    // L y = x from y_0 = x
-    const double *yin = x;
-    double *yout = A;
-    for (int k = 0; k < sw; k++) {
-        hipLaunchKernelGGL((k_tri_sweep<T, true>), g, b, 0, c->stream, n, c->d_rowptr, c->d_col, s->dpos, f, x, yin, yout);
-        yin = yout;
-        yout = (yout == A) ? B : A;
-    }
-    const double *yl = yin;
-    double *other = (yl == A) ? B : A;
-    // U z = yl from z_0 = D^{-1} yl; the last of the sw + 1 launches writes y
-    const int nl = sw + 1;
-    double *out = (nl % 2 == 1) ? y : other;
-    const double *zin = nullptr;
-    for (int k = 0; k < nl; k++) {
-        hipLaunchKernelGGL((k_tri_sweep<T, false>), g, b, 0, c->stream, n, c->d_rowptr, c->d_col, s->dpos, f, yl, zin, out);
-        zin = out;
-        out = (out == y) ? other : y;
-    }
-    HIPCHK(hipGetLastError());
-    return IGAP_OK;
-}
-
 // y = M^{-1} x (x and y must differ from s->t and s->t2)
-static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int sweeps, const double *x, double *y)
+static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int order, const double *x, double *y)
 {
     int64_t n = c->ndof;
-    if (pc == IGAP_PC_ILU0 && sweeps > 0) {
-        if (p == IGAP_F64) return ilu_sweeps<double>(c, s, sweeps, (const double *)s->ilu[p], x, y);
-        return ilu_sweeps<float>(c, s, sweeps, (const float *)s->ilu[p], x, y);
+    if (pc == IGAP_PC_ILU0 && order == 1) {
+        hipLaunchKernelGGL((k_gather<double, double>), grid1(n), dim3(256), 0, c->stream, n, s->mc_perm, x, s->t);
+        if (!s->mc_sweep_ok) {   // generic level-scheduled solves of the same factor
+            int nn = (int)n, nnz = (int)c->nnz;
+            if (p == IGAP_F64) {
+                const double one = 1.0;
+                SPCHK(rocsparse_dcsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dL, (double *)s->mc_ilu[p],
+                                             s->mc_rowptr, s->mc_col, s->mc_info[p], s->t, s->t2, rocsparse_solve_policy_auto,
+                                             s->mc_buf[p]));
+                SPCHK(rocsparse_dcsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dU, (double *)s->mc_ilu[p],
+                                             s->mc_rowptr, s->mc_col, s->mc_info[p], s->t2, s->t, rocsparse_solve_policy_auto,
+                                             s->mc_buf[p]));
+            } else {
+                const float one = 1.0f;
+                hipLaunchKernelGGL(k_d2f, grid1(n), dim3(256), 0, c->stream, n, s->t, s->fa);
+                SPCHK(rocsparse_scsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dL, (float *)s->mc_ilu[p],
+                                             s->mc_rowptr, s->mc_col, s->mc_info[p], s->fa, s->fb, rocsparse_solve_policy_auto,
+                                             s->mc_buf[p]));
+                SPCHK(rocsparse_scsrsv_solve(s->sp, rocsparse_operation_none, nn, nnz, &one, s->dU, (float *)s->mc_ilu[p],
+                                             s->mc_rowptr, s->mc_col, s->mc_info[p], s->fb, s->fa, rocsparse_solve_policy_auto,
+                                             s->mc_buf[p]));
+                hipLaunchKernelGGL(k_f2d, grid1(n), dim3(256), 0, c->stream, n, s->fa, s->t);
+            }
+            hipLaunchKernelGGL(k_scatter, grid1(n), dim3(256), 0, c->stream, n, s->mc_perm, s->t, y);
+            HIPCHK(hipGetLastError());
+            return IGAP_OK;
+        }
+        for (int cc = 0; cc < 27; cc++) {
+            const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
+            if (np == 0) continue;
+            const dim3 g((unsigned)((np + 3) / 4));
+            if (p == IGAP_F64)
+                hipLaunchKernelGGL(k_mc_lower<double>, g, dim3(256), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col, s->mc_dpos,
+                                   (const double *)s->mc_ilu[p], s->t, s->t2);
+            else
+                hipLaunchKernelGGL(k_mc_lower<float>, g, dim3(256), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col, s->mc_dpos,
+                                   (const float *)s->mc_ilu[p], s->t, s->t2);
+        }
+        for (int cc = 26; cc >= 0; cc--) {
+            const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
+            if (np == 0) continue;
+            const dim3 g((unsigned)((np + 3) / 4));
+            if (p == IGAP_F64)
+                hipLaunchKernelGGL(k_mc_upper<double>, g, dim3(256), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col, s->mc_dpos,
+                                   (const double *)s->mc_ilu[p], s->t2, s->t);
+            else
+                hipLaunchKernelGGL(k_mc_upper<float>, g, dim3(256), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col, s->mc_dpos,
+                                   (const float *)s->mc_ilu[p], s->t2, s->t);
+        }
+        hipLaunchKernelGGL(k_scatter, grid1(n), dim3(256), 0, c->stream, n, s->mc_perm, s->t, y);
+        HIPCHK(hipGetLastError());
+        return IGAP_OK;
     }
     if (pc == IGAP_PC_JACOBI) {
         hipLaunchKernelGGL(k_mul, grid1(n), dim3(256), 0, c->stream, n, s->diag, x, y);
@@ -404,7 +622,7 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
         int j = 0;
         for (; j < m && total < o->ksp_maxit; j++) {
             double *vj = s->V + (size_t)j * n, *vn = s->V + (size_t)(j + 1) * n;
-            rc = pc_apply(c, s, o->pc, p, o->pc_sweeps, vj, s->z);
+            rc = pc_apply(c, s, o->pc, p, o->pc_order, vj, s->z);
             if (rc) return rc;
             rc = spmv(c, p, s->z, vn);
             if (rc) return rc;
@@ -449,7 +667,7 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
         HIPCHK(hipMemcpyAsync(s->h, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
         const double one = 1.0, zero = 0.0;
         BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j, &one, s->V, N, s->h, 1, &zero, s->w, 1));
-        rc = pc_apply(c, s, o->pc, p, o->pc_sweeps, s->w, s->z);
+        rc = pc_apply(c, s, o->pc, p, o->pc_order, s->w, s->z);
         if (rc) return rc;
         hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0, s->z, 1.0, d);
         if (rnorm <= o->ksp_rtol * bnorm || rnorm <= o->ksp_atol) break;
@@ -481,7 +699,7 @@ extern "C" int igap_gpu_solver_default_opts(igap_solver_opts *o)
     o->pc = IGAP_PC_ILU0;
     o->tangent_prec = IGAP_F64;
     o->linesearch = 1;
-    o->pc_sweeps = 0;
+    o->pc_order = 0;
     return IGAP_OK;
 }
 
@@ -491,6 +709,7 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     if (!c || !o || !st) return fail(IGAP_ERR_ARG, "null argument");
     if (o->ksp_restart < 1 || o->ksp_restart > 2000) return fail(IGAP_ERR_ARG, "ksp_restart %d out of [1, 2000]", o->ksp_restart);
     if (o->tangent_prec != IGAP_F64 && o->tangent_prec != IGAP_F32) return fail(IGAP_ERR_ARG, "tangent_prec");
+    if (o->pc_order != 0 && o->pc_order != 1) return fail(IGAP_ERR_ARG, "pc_order must be 0 (natural) or 1 (coloured)");
     HIPCHK(hipSetDevice(c->device));
     std::memset(st, 0, sizeof *st);
     auto T0 = clk::now();
@@ -528,7 +747,7 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         st->t_tangent_ms += ms(t, clk::now());
         t = clk::now();
         if (o->pc == IGAP_PC_ILU0) {
-            rc = pc_ilu_setup(c, s, p);
+            rc = o->pc_order == 1 ? pc_ilu_setup_mc(c, s, p) : pc_ilu_setup(c, s, p);
             if (rc) return rc;
         } else if (o->pc == IGAP_PC_JACOBI) {
             if (p == IGAP_F64) hipLaunchKernelGGL(k_diag_inv<double>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, s->diag);

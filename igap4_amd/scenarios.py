@@ -38,7 +38,7 @@ def _agg(steps):
             "t_pc_ms": float(sum(s["t_pc_ms"] for s in steps)), "t_ksp_ms": float(sum(s["t_ksp_ms"] for s in steps))}
 
 
-def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_sweeps=0, dt=1e-3, rtol=1e-10,
+def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=0, dt=1e-3, rtol=1e-10,
               guess="previous", device=0, verbose=True):
     """gradelasttime with the reference parameters except dt; relative
     Newton tolerance rtol per step."""
@@ -49,7 +49,7 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_sweeps
     A = am.Assm(mesh, ph, am.BCDefault(), device=device)
     u1, u0 = initial_velocity_modes(_owned_X(mesh), (4, 1, 1), dt)
     A.set_uhist(np.concatenate([u1, u1, u0]))
-    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_sweeps=pc_sweeps)
+    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_order=pc_order)
 
     def cb(k, H, st):
         if verbose:
@@ -60,17 +60,17 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_sweeps
     steps = run_time_steps(A, nsteps, opts=opts, guess=guess, callback=cb)
     tt = time.perf_counter() - t0
     out = {"scenario": "transient", "nelem": list(nelem), "ndof": A.ndof, "nnz": A.nnz, "dt": dt, "tangent": prec,
-           "pc": pc, "pc_sweeps": pc_sweeps, "rtol": rtol, "total_s": tt, **_agg(steps)}
+           "pc": pc, "pc_order": pc_order, "rtol": rtol, "total_s": tt, **_agg(steps)}
     A.close()
     return out
 
 
-def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_sweeps=0, rtol=1e-10, guess="previous",
+def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_order=0, rtol=1e-10, guess="previous",
          max_steps=None, device=0, verbose=True):
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
-    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_sweeps=pc_sweeps)
+    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_order=pc_order)
 
     def cb(k, U, st):
         if verbose:
@@ -80,7 +80,7 @@ def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_swee
     steps = run_load_steps(A, nsteps, tip_total=tip, guess=guess, opts=opts, callback=cb, max_steps=max_steps)
     tt = time.perf_counter() - t0
     out = {"scenario": "ramp", "nelem": list(nelem), "ndof": A.ndof, "nnz": A.nnz, "tip": tip, "load_steps": nsteps,
-           "tangent": prec, "pc": pc, "pc_sweeps": pc_sweeps, "rtol": rtol, "guess": guess, "total_s": tt,
+           "tangent": prec, "pc": pc, "pc_order": pc_order, "rtol": rtol, "guess": guess, "total_s": tt,
            **_agg(steps)}
     A.close()
     return out
@@ -95,17 +95,17 @@ def main():
     ap.add_argument("--tip", type=float, default=0.05)
     ap.add_argument("--prec", default=None)
     ap.add_argument("--pc", default=None)
-    ap.add_argument("--pc-sweeps", type=int, default=0)
+    ap.add_argument("--pc-order", type=int, default=0)
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--guess", default="previous")
     a = ap.parse_args()
     if a.which == "transient":
         nel = tuple(int(x) for x in (a.nelem or "128,32,32").split(","))
-        r = transient(nel, a.steps, a.prec or "f32", a.pc or "jacobi", a.pc_sweeps, a.dt, a.rtol, a.guess)
+        r = transient(nel, a.steps, a.prec or "f32", a.pc or "jacobi", a.pc_order, a.dt, a.rtol, a.guess)
     else:
         nel = tuple(int(x) for x in (a.nelem or "64,16,16").split(","))
-        r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_sweeps, a.rtol, a.guess, a.max_steps)
+        r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_order, a.rtol, a.guess, a.max_steps)
     print(json.dumps(r), flush=True)
 
 

@@ -185,7 +185,7 @@ class LearnedGuess:
 
 
 # ------------------------------------------------------------------ driver
-def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f64", pc_sweeps=0,
+def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f64", pc_order=0,
         seed=0, device=0, verbose=True):
     """Config-4 scenario: steps 1..ntrain generate the training trajectory
     (extrapolation guesses), the network is trained on it, then steps
@@ -200,7 +200,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
 
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     grid = mesh.nb
-    opts = am.default_solver_opts(snes_rtol=1e-10, snes_atol=1e-30, tangent_prec=prec, pc_sweeps=pc_sweeps)
+    opts = am.default_solver_opts(snes_rtol=1e-10, snes_atol=1e-30, tangent_prec=prec, pc_order=pc_order)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
     U, G = [A.get_uhist()[:A.ndof].copy()], [A.dirichlet()[1]]
@@ -270,10 +270,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--prec", default="f64")
-    ap.add_argument("--pc-sweeps", type=int, default=0)
+    ap.add_argument("--pc-order", type=int, default=0)
     a = ap.parse_args()
     nel = tuple(int(x) for x in a.nelem.split(","))
-    s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_sweeps)
+    s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order)
     print(json.dumps(s), flush=True)
 
 

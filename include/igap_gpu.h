@@ -160,9 +160,10 @@ typedef struct {
     int    pc;            /* IGAP_PC_ILU0 (the reference's ASM/ILU(0) on one part), ...     */
     int    tangent_prec;  /* IGAP_F64 or IGAP_F32 (FP32 matrix-core tangent, FP32 ILU)      */
     int    linesearch;    /* 1: backtracking (SNESLINESEARCHBT, cubic), 0: full steps       */
-    int    pc_sweeps;     /* ILU(0) triangular solves: 0 exact (level-scheduled), k > 0    */
-                          /* k Jacobi sweeps per triangle (an approximation of the same     */
-                          /* factors that trades exactness for parallelism)                 */
+    int    pc_order;      /* ILU(0) ordering: 0 natural dof order (the reference's), 1     */
+                          /* 27 colours of control points (colour 9(i%3)+3(j%3)+k%3, the   */
+                          /* three dofs of a point together): ~25x fewer dependent levels  */
+                          /* in the triangular solves, a different incomplete factor       */
 } igap_solver_opts;
 
 typedef struct {
