@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--nelem", default="64,16,16")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--newton-steps", type=int, default=3,
+                    help="load steps of the Newton ramp timed after the assembly bench (0: skip)")
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling run for rocprofv3 --pmc: one 256 MiB device copy (the HBM "
                          "counter calibration) then N residual+tangent calls; prints no JSON")
@@ -237,11 +239,33 @@ def main():
                          "tangent_elem_tflops": F_K * nelem / (avg32[1] * 1e-3) / 1e12,
                          "peak": PEAK_FP32_TFLOPS},
     }
+    A.close()
+    if a.newton_steps > 0:
+        line["newton_load_step"] = newton_load_steps(nel, a.newton_steps)
     if a.cpu_baseline:
         th = a.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(th)
-    A.close()
     print(json.dumps(line), flush=True)
+
+
+def newton_load_steps(nel, k, nsteps=20, tip=0.05):
+    """The second measure of the workload: wall time per Newton load step of
+    the clamped-tip ramp (tip 0.05 in 20 load steps) to ||R||/||R0|| <= 1e-10,
+    device Newton with GMRES and ILU(0) in the 27-colour order, FP64 tangent;
+    the first k steps of the ramp (the first includes one-time set-up)."""
+    import igap4_amd as ig
+    from igap4_amd.loadstep import run_load_steps
+    mesh = ig.MeshIGABeam(L=(4.0, 1.0, 1.0), nelem=nel)
+    with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.0)) as A:
+        A.set_uhist(np.zeros(A.ndof))
+        opts = ig.assm.default_solver_opts(snes_rtol=1e-10, snes_atol=1e-30, tangent_prec="f64", pc_order=1)
+        st = run_load_steps(A, nsteps, tip_total=tip, opts=opts, max_steps=k)
+    w = [s["wall_ms"] for s in st]
+    return {"ms_per_step_median": float(np.median(w)), "ms_per_step": w, "steps_timed": len(st),
+            "newton_its": [s["newton_its"] for s in st], "ksp_its": [s["ksp_its"] for s in st],
+            "converged": bool(all(s["converged"] > 0 for s in st)),
+            "solver": "NEWTONLS+BT, GMRES(200) right-PC, ILU(0) 27-colour order, FP64 tangent, rtol 1e-10",
+            "ramp": f"tip {tip} in {nsteps} load steps"}
 
 
 if __name__ == "__main__":

@@ -79,6 +79,7 @@ _SIGS = {
     "igap_gpu_set_bc": (C.c_int, [C.c_void_p, C.POINTER(BCDesc)]),
     "igap_gpu_sizes": (C.c_int, [C.c_void_p, c_i64_p, c_i64_p, c_i64_p, C.POINTER(C.c_int)]),
     "igap_gpu_uhist": (C.c_void_p, [C.c_void_p]),
+    "igap_gpu_intdensity": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_int)]),
     "igap_gpu_save_solution": (C.c_int, [C.c_void_p, C.c_char_p]),
     "igap_gpu_load_solution": (C.c_int, [C.c_void_p, C.c_char_p]),
     "igap_gpu_csr_rowptr": (C.c_void_p, [C.c_void_p]),

@@ -195,6 +195,13 @@ class Assm:
     def get_uhist(self):
         return self.dtoh(self.uhist_ptr(), (self.torder + 1) * self.ndof)
 
+    def intdensity(self):
+        """iga_intdensity / IGAPPostComputeHH of the current U_hist."""
+        HH = np.zeros(13)
+        n = C.c_int()
+        check(lib().igap_gpu_intdensity(self.ctx, _dp(HH), C.byref(n)))
+        return HH[:n.value]
+
     def save_solution(self, fname):
         """IGAPSolvSaveSolution: the reference's solution file (universal dof order)."""
         check(lib().igap_gpu_save_solution(self.ctx, str(fname).encode()))

@@ -485,7 +485,7 @@ extern "C" int igap_gpu_destroy(igap_gpu_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (int d = 0; d < 3; d++) hipFree(c->d_knot[d]);
     void *bufs[] = {c->d_w, c->d_X, c->d_Uh, c->d_R, c->d_Re, c->d_Ke[0], c->d_Ke[1], c->d_rowptr, c->d_col, c->d_val64,
-                    c->d_val32, c->d_mask, c->d_g, c->d_F};
+                    c->d_val32, c->d_mask, c->d_g, c->d_F, c->d_Hd, c->d_H};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->solver) solver_destroy(c->solver);
@@ -514,6 +514,33 @@ extern "C" int igap_gpu_sizes(const igap_gpu_ctx *c, int64_t *ndof, int64_t *nnz
     return IGAP_OK;
 }
 extern "C" double *igap_gpu_uhist(igap_gpu_ctx *c) { return c ? c->d_Uh : nullptr; }
+
+extern "C" int igap_gpu_intdensity(igap_gpu_ctx *c, double *HH, int *ndensity)
+{
+    if (!c || !HH || !ndensity) return fail(IGAP_ERR_ARG, "null argument");
+    const int nd = c->torder == 0 ? 13 : 2;
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->d_Hd) {
+        HIPCHK(hipMalloc(&c->d_Hd, sizeof(double) * (size_t)c->nel * nd));
+        HIPCHK(hipMalloc(&c->d_H, sizeof(double) * nd));
+    }
+    HIPCHK(launch_density(c->M, c->P, c->d_Uh, c->ndof, c->d_Hd, c->d_H, (int)c->nel, c->stream));
+    std::vector<double> H(nd), U((size_t)(c->torder + 1) * c->ndof), F(c->ndof);
+    HIPCHK(hipMemcpyAsync(H.data(), c->d_H, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(U.data(), c->d_Uh, sizeof(double) * U.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(F.data(), c->d_F, sizeof(double) * c->ndof, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // Neumann faces (src/postcore_iga.c:302-371): -int u t with F = -int N t
+    // the residual's load; u = current level (static) or the mean of the two
+    // previous levels (gradelasttime_densityb, phys/gradelasttime.c:9234-9239)
+    double s = 0.0;
+    const int64_t n = c->ndof;
+    for (int64_t i = 0; i < n; i++) s += (c->torder == 0 ? U[i] : 0.5 * (U[n + i] + U[2 * n + i])) * F[i];
+    H[0] += s;
+    for (int i = 0; i < nd; i++) HH[i] = H[i];
+    *ndensity = nd;
+    return IGAP_OK;
+}
 
 // levels [h0, h1) of U_hist that a solution file holds (src/interface.c:271-300)
 static void file_levels(const igap_gpu_ctx *c, int &h0, int &h1)

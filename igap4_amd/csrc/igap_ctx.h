@@ -41,6 +41,7 @@ struct igap_gpu_ctx {
     float *d_val32 = nullptr;
     int32_t *d_mask = nullptr;
     double *d_g = nullptr, *d_F = nullptr;
+    double *d_Hd = nullptr, *d_H = nullptr;   // density integral: per element, total
     // timing
     int timing = 0;
     hipEvent_t ev[4];

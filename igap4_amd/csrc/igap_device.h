@@ -359,6 +359,83 @@ __device__ __forceinline__ void phys_residual_transient(const PhysK &P, const do
         r[10 * k] = (P.cii * (uv[k] - uv[6 + k])) / (2. * P.dt) + (P.rho * (uv[k] + uv[6 + k] - 2 * uv[3 + k])) / (P.dt * P.dt);
 }
 
+// ---------------------------------------------------------------- densities
+// gradelast_density (phys/gradelast.c:1025-1085): hh[0] the stored energy,
+// hh[1..6] = 2 dPsi/dE_IJ (IJ = 11,12,13,22,23,33, each off-diagonal E_IJ
+// taken once), hh[7..12] the phase indicators of the deviatoric strain
+// (e25, e26) and the energy.
+constexpr int ND_STATIC = 13, ND_TRANSIENT = 2;
+
+__device__ __forceinline__ double phys_energy(const double *u, const PhysK &P, double &e25, double &e26)
+{
+    double F[9], E[9], EK[9];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int I = 0; I < 3; I++) F[3 * k + I] = (k == I ? 1.0 : 0.0) + u[10 * k + 1 + I];
+#pragma unroll
+    for (int I = 0; I < 3; I++)
+#pragma unroll
+        for (int J = 0; J < 3; J++)
+            E[3 * I + J] = 0.5 * (F[I] * F[J] + F[3 + I] * F[3 + J] + F[6 + I] * F[6 + J] - (I == J ? 1.0 : 0.0));
+#pragma unroll
+    for (int I = 0; I < 3; I++)
+#pragma unroll
+        for (int K = 0; K < 3; K++)
+            EK[3 * I + K] = F[I] * u[sidx(I, K)] + F[3 + I] * u[10 + sidx(I, K)] + F[6 + I] * u[20 + sidx(I, K)];
+    const double e24 = (E[0] + E[4] + E[8]) * IS3;
+    e25 = (E[0] - E[4]) * IS2;
+    e26 = (E[0] + E[4] - 2 * E[8]) * IS6;
+    const double rho2 = e25 * e25 + e26 * e26;
+    double g2 = 0.0;
+#pragma unroll
+    for (int K = 0; K < 3; K++) {
+        const double a = (EK[K] - EK[3 + K]) * IS2, b = (EK[K] + EK[3 + K] - 2 * EK[6 + K]) * IS6;
+        g2 += a * a + b * b;
+    }
+    return P.c3 * e26 * (e26 * e26 - 3 * e25 * e25) + P.c2 * rho2 + P.c4 * rho2 * rho2 + P.e0 * e24 * e24
+         + P.e345 * (E[1] * E[1] + E[2] * E[2] + E[5] * E[5]) + P.le * P.le * g2;
+}
+
+__device__ __forceinline__ void phys_density_static(const double *u, const PhysK &P, double *hh)
+{
+    double e25, e26;
+    const double psi = phys_energy(u, P, e25, e26);
+    State st;
+    phys_state(u, P, st);
+    hh[0] = psi;
+    hh[1] = 2 * st.S[0];
+    hh[2] = 4 * st.S[1];
+    hh[3] = 4 * st.S[2];
+    hh[4] = 2 * st.S[4];
+    hh[5] = 4 * st.S[5];
+    hh[6] = 2 * st.S[8];
+    hh[7] = (double)(e25 > 0 && e26 > -0.5 * e25);
+    hh[8] = (double)(e25 < 0 && e26 > 0.5 * e25);
+    hh[9] = (double)(e26 < 0.5 * e25 && e26 < -0.5 * e25);
+    hh[10] = (double)(psi < -0.5 && e25 > 0 && e26 > 0);
+    hh[11] = (double)(psi < -0.5 && e25 < 0 && e26 > 0);
+    hh[12] = (double)(psi < -0.5 && e26 < 0);
+}
+
+// gradelasttime_density (phys/gradelasttime.c:9183-9232): the energy at the
+// midpoint of the two previous levels plus rho |u1 - u0|^2 / (2 dt^2), and 1.
+// J = jets of the levels [u, u1, u0] (90 values).
+__device__ __forceinline__ void phys_density_transient(const double *J, const PhysK &P, double *hh)
+{
+    double m[30];
+#pragma unroll
+    for (int i = 0; i < 30; i++) m[i] = 0.5 * (J[30 + i] + J[60 + i]);
+    double e25, e26, kin = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double v = J[30 + 10 * k] - J[60 + 10 * k];
+        kin += v * v;
+    }
+    hh[0] = phys_energy(m, P, e25, e26) + P.rho * kin / (2.0 * P.dt * P.dt);
+    hh[1] = 1.0;
+}
+
 // ---------------------------------------------------------------- tangent entries
 // Entries of d2Psi/du2 at one state, by block type.  M = I - 11^T/3.
 __device__ __forceinline__ double Mdev(int a, int b) { return (a == b ? 2.0 : -1.0) / 3.0; }

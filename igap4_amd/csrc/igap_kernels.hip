@@ -62,23 +62,19 @@ __device__ __forceinline__ void rs_track(int N, bool up, int &base, int &cnt)
 //      chain rule (phys_jet_adj) and of the quotient rule (nurbs_param_adj)
 //      per Gauss point, then the transposed sum factorisation x -> y -> z.
 // ============================================================================
+// Element e's Gauss-point fields for the residual and the density integral
+// (256 threads): the control points and 1-D tables to LDS, then by sum
+// factorisation the physical u-jets of every history level
+// (sJet[q][h][k][a], par_periodic added) and the weight, G^-1, d2x/dxi2
+// and W jets (sGeo[q][38]).  Ends with a block barrier.
 template <int NH>
-__global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
-                                                       double *__restrict__ Re, int nel)
+__device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restrict__ Uh, int64_t ndof, int e,
+                                          double (*sT)[4][3][3], double *sCP, double *sWork, double *sJet, double *sGeo)
 {
-    constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
-    constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
-    constexpr int WA = NF * (108 + 288);       // stage-A buffers
-    constexpr int WT = 1920 + 864 + 324;       // unused + x^T and y^T buffers (rho lives in sJet)
-    constexpr int GEO_R = 38;                  // wgt, Gi[9], X2[18], W[10]
-    __shared__ double sT[3][4][3][3];
-    __shared__ double sCP[27 * STR];
-    __shared__ double sWork[WA > WT ? WA : WT];
-    __shared__ double sJet[64 * 30 * NH];      // physical u-jets [q][h][k][a]; then r_q / rho_q [q][k][a]
-    __shared__ double sGeo[64 * GEO_R];
-
+    constexpr int STR = 4 + 3 * NH;
+    constexpr int NF = 4 + 3 * NH;
+    constexpr int GEO_R = 38;
     const int tid = threadIdx.x;
-    const int e = blockIdx.x;
     const int nz = M.nelem[2], ny = M.nelem[1];
     const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
     const int ed[3] = {ex, ey, ez};
@@ -105,7 +101,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
     }
     __syncthreads();
 
-    // ---- A: fields at the Gauss points (z, then y, then x)
+    // fields at the Gauss points (z, then y, then x)
     {
         double *sA = sWork, *sB = sWork + NF * 108;
         for (int i = tid; i < NF * 108; i += 256) {            // [f][bx*3+by][qz][dz]
@@ -189,6 +185,28 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
         }
     }
     __syncthreads();
+
+}
+
+template <int NH>
+__global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                       double *__restrict__ Re, int nel)
+{
+    constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
+    constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
+    constexpr int WA = NF * (108 + 288);       // stage-A buffers
+    constexpr int WT = 1920 + 864 + 324;       // unused + x^T and y^T buffers (rho lives in sJet)
+    constexpr int GEO_R = 38;                  // wgt, Gi[9], X2[18], W[10]
+    __shared__ double sT[3][4][3][3];
+    __shared__ double sCP[27 * STR];
+    __shared__ double sWork[WA > WT ? WA : WT];
+    __shared__ double sJet[64 * 30 * NH];      // physical u-jets [q][h][k][a]; then r_q / rho_q [q][k][a]
+    __shared__ double sGeo[64 * GEO_R];
+
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x;
+    // ---- A: fields at the Gauss points
+    gp_fields<NH>(M, Uh, ndof, e, sT, sCP, sWork, sJet, sGeo);
 
     // ---- P: pointwise residual jets r_q -> sJet[q * 30 * NH + 10 k + a]
     if (NH == 1) {
@@ -296,6 +314,58 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
             for (int dz = 0; dz < 3; dz++) acc += c[3 * qz + dz] * sT[2][qz][dz][bz];
         Re[(int64_t)e * 81 + tid] = sCP[STR * b + 3] * acc;
     }
+}
+
+// ============================================================================
+// density integral (iga_intdensity volume part, src/postcore_iga.c:84-234):
+// per element sum_q w_q hh(u_q) -> Hd[e][ND]; lane = Gauss point, butterfly
+// sums over the 64 lanes
+// ============================================================================
+template <int NH>
+__global__ __launch_bounds__(256) void k_density_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                      double *__restrict__ Hd, int nel)
+{
+    constexpr int STR = 4 + 3 * NH;
+    constexpr int NF = 4 + 3 * NH;
+    constexpr int GEO_R = 38;
+    constexpr int ND = (NH == 1) ? ND_STATIC : ND_TRANSIENT;
+    __shared__ double sT[3][4][3][3];
+    __shared__ double sCP[27 * STR];
+    __shared__ double sWork[NF * (108 + 288)];
+    __shared__ double sJet[64 * 30 * NH];
+    __shared__ double sGeo[64 * GEO_R];
+    const int e = blockIdx.x;
+    gp_fields<NH>(M, Uh, ndof, e, sT, sCP, sWork, sJet, sGeo);
+    if (threadIdx.x < 64) {
+        const int q = threadIdx.x;
+        double hh[ND];
+        if (NH == 1) phys_density_static(sJet + q * 30, P, hh);
+        else phys_density_transient(sJet + q * 90, P, hh);
+        const double w = sGeo[q * GEO_R];
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            double v = w * hh[i];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (q == 0) Hd[(int64_t)e * ND + i] = v;
+        }
+    }
+}
+
+// H[i] = sum over elements of Hd[e][i], in a fixed order (one block per density)
+__global__ __launch_bounds__(256) void k_density_reduce(int nel, int nd, const double *__restrict__ Hd, double *__restrict__ H)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x;
+    double s = 0.0;
+    for (int e = threadIdx.x; e < nel; e += 256) s += Hd[(int64_t)e * nd + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) H[i] = red[0];
 }
 
 // contributing (element, local index) pairs of owned control point a along one
@@ -842,6 +912,17 @@ __global__ void k_selftest_mfma(double *o64, float *o32)
 #include "igap_launch.h"
 
 namespace igap {
+
+hipError_t launch_density(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Hd, double *H, int nel,
+                          hipStream_t s)
+{
+    dim3 grid(nel), block(256);
+    if (P.torder == 0) hipLaunchKernelGGL(k_density_elem<1>, grid, block, 0, s, M, P, Uh, ndof, Hd, nel);
+    else hipLaunchKernelGGL(k_density_elem<3>, grid, block, 0, s, M, P, Uh, ndof, Hd, nel);
+    const int nd = P.torder == 0 ? ND_STATIC : ND_TRANSIENT;
+    hipLaunchKernelGGL(k_density_reduce, dim3(nd), dim3(256), 0, s, nel, nd, Hd, H);
+    return hipGetLastError();
+}
 
 hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Re, int nel, hipStream_t s)
 {

@@ -8,6 +8,8 @@
 
 namespace igap {
 
+hipError_t launch_density(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Hd, double *H, int nel,
+                          hipStream_t s);
 hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Re, int nel, hipStream_t s);
 hipError_t launch_residual_gather(const MeshK &M, const double *Re, const double *Uh, const double *Fneu, const int32_t *mask,
                                   const double *g, double *R, int64_t ncp, hipStream_t s);

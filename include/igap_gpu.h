@@ -112,6 +112,13 @@ int igap_gpu_sizes(const igap_gpu_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t
 /* core->U_hist (include/coreimpl.h:17): (torder+1)*ndof doubles, level h at
  * offset h*ndof, 0 = current iterate (src/interface.c:127). */
 double *igap_gpu_uhist(igap_gpu_ctx *ctx);
+/* iga_intdensity (src/postcore_iga.c:13-379, IGAPPostComputeHH
+ * src/interface.c:341-346) of the current U_hist: HH[i] = sum over Gauss
+ * points of w_q hh_i(u_q) minus the Neumann face work.  gradelast: 13 values
+ * (energy, 2 dPsi/dE_IJ for 11,12,13,22,23,33, six phase fractions);
+ * gradelasttime: 2 (energy with the kinetic term, volume).  HH is a host
+ * array of 13; *ndensity receives the count.  Synchronous. */
+int igap_gpu_intdensity(igap_gpu_ctx *ctx, double *HH, int *ndensity);
 /* Solution files (IGAPSolvSaveSolution / IGAPSolvLoadSolution,
  * src/interface.c:271-300, write_solution / read_solution src/ioutil.c:90-159):
  * raw FP64 in the reference's universal dof order (on one GPU the owned order

@@ -152,6 +152,53 @@ double oc_static_energy(const double *u, const double *coef)
     return st.psi;
 }
 
+/* gradelast_density, phys/gradelast.c:1025-1085 (energy, 2 dPsi/dE, phases) */
+void oc_static_density(double *hh, const double *u, const double *coef)
+{
+    oc_state st;
+    oc_static_state(u, coef, &st);
+    double E[3][3];
+    for (int I = 0; I < 3; I++)
+        for (int J = 0; J < 3; J++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += st.F[k][I] * st.F[k][J];
+            E[I][J] = 0.5 * (s - (I == J ? 1.0 : 0.0));
+        }
+    const double e25 = (E[0][0] - E[1][1]) * IS2;
+    const double e26 = (E[0][0] + E[1][1] - 2 * E[2][2]) * IS6;
+    hh[0] = st.psi;
+    hh[1] = 2 * st.S[0][0];
+    hh[2] = 4 * st.S[0][1];
+    hh[3] = 4 * st.S[0][2];
+    hh[4] = 2 * st.S[1][1];
+    hh[5] = 4 * st.S[1][2];
+    hh[6] = 2 * st.S[2][2];
+    hh[7] = (double)(e25 > 0 && e26 > -0.5 * e25);
+    hh[8] = (double)(e25 < 0 && e26 > 0.5 * e25);
+    hh[9] = (double)(e26 < 0.5 * e25 && e26 < -0.5 * e25);
+    hh[10] = (double)(st.psi < -0.5 && e25 > 0 && e26 > 0);
+    hh[11] = (double)(st.psi < -0.5 && e25 < 0 && e26 > 0);
+    hh[12] = (double)(st.psi < -0.5 && e26 < 0);
+}
+
+void oc_phys_density(const oc_phys *ph, double *hh, const double *u)
+{
+    if (ph->torder == 0) {
+        oc_static_density(hh, u, ph->coef);
+        return;
+    }
+    /* gradelasttime_density, phys/gradelasttime.c:9183-9232 */
+    double mid[30];
+    for (int i = 0; i < 30; i++) mid[i] = 0.5 * (u[30 + i] + u[60 + i]);
+    double kin = 0;
+    for (int k = 0; k < 3; k++) {
+        const double v = u[30 + 10 * k] - u[60 + 10 * k];
+        kin += v * v;
+    }
+    hh[0] = oc_static_energy(mid, ph->coef) + ph->rho * kin / (2.0 * ph->dt * ph->dt);
+    hh[1] = 1.0;
+}
+
 /* r = dPsi/du (30 jet slots, slot 10k+a) */
 static void oc_static_residual_acc(const oc_state *st, double s, double *r)
 {
@@ -1020,6 +1067,41 @@ double oc_energy(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const dou
                             tot += gp.weight * oc_static_energy(u, ph->coef);
                         }
     return tot;
+}
+
+int oc_intdensity(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist, double *HH)
+{
+    const int nd = ph->torder == 0 ? OC_NDENSITY_STATIC : OC_NDENSITY_TRANSIENT;
+    const int nh = ph->torder + 1;
+    double cq[4], wq[4];
+    oc_gauss4(cq, wq);
+    for (int i = 0; i < nd; i++) HH[i] = 0.0;
+    for (int ex = 0; ex < m->nelem[0]; ex++)
+        for (int ey = 0; ey < m->nelem[1]; ey++)
+            for (int ez = 0; ez < m->nelem[2]; ez++)
+                for (int qx = 0; qx < NQ; qx++)
+                    for (int qy = 0; qy < NQ; qy++)
+                        for (int qz = 0; qz < NQ; qz++) {
+                            oc_gp gp;
+                            oc_gausspoint(m, ex, ey, ez, cq, wq, qx, qy, qz, &gp);
+                            double u[90], hh[OC_NDENSITY_STATIC];
+                            oc_ujet(m, bc, nh, U_hist, ex, ey, ez, &gp, u);
+                            oc_phys_density(ph, hh, u);
+                            for (int i = 0; i < nd; i++) HH[i] += gp.weight * hh[i];
+                        }
+    /* Neumann faces: F = -int N t (the residual's load), so -int u t = U . F
+     * with U the current level (static) or the mean of levels 1 and 2 */
+    const int64_t ndof = oc_ndof(m);
+    double *F = (double *)malloc(sizeof(double) * ndof);
+    oc_neumann(m, bc, F);
+    double s = 0;
+    for (int64_t i = 0; i < ndof; i++) {
+        const double ub = ph->torder == 0 ? U_hist[i] : 0.5 * (U_hist[ndof + i] + U_hist[2 * ndof + i]);
+        s += ub * F[i];
+    }
+    free(F);
+    HH[0] += s;
+    return nd;
 }
 
 double oc_volume(const oc_mesh *m)

@@ -41,6 +41,17 @@ void oc_static_tangent(double *C, const double *u, const double *coef);
 void oc_phys_residual(const oc_phys *ph, double *r, const double *u);
 void oc_phys_tangent(const oc_phys *ph, double *C, const double *u);
 double oc_static_energy(const double *u, const double *coef);
+/* Density callbacks of iga_intdensity (include/physimpl.h:9):
+ * gradelast_density (phys/gradelast.c:1025-1085): 13 values, hh[0] the stored
+ * energy, hh[1..6] = 2 dPsi/dE_IJ for IJ = 11,12,13,22,23,33 (each off-
+ * diagonal E_IJ taken once), hh[7..12] the phase indicators;
+ * gradelasttime_density (phys/gradelasttime.c:9183-9232): 2 values, the
+ * stored energy at the midpoint (u1+u0)/2 plus the kinetic energy
+ * rho |u1-u0|^2 / (2 dt^2), and 1 (the volume).  u = [u, u1, u0] jets. */
+#define OC_NDENSITY_STATIC 13
+#define OC_NDENSITY_TRANSIENT 2
+void oc_static_density(double *hh, const double *u, const double *coef);
+void oc_phys_density(const oc_phys *ph, double *hh, const double *u);
 
 /* ---- IGA kernels ---------------------------------------------------------- */
 void oc_gauss4(double *x, double *w);                     /* quadrature.c:381-429 on [0,1] */
@@ -106,6 +117,10 @@ void oc_element_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc,
 /* total stored energy  sum_e sum_q w_q Psi(u_q) (static physics; used for
  * R = dPi/dU checks) */
 double oc_energy(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist);
+/* iga_intdensity (src/postcore_iga.c:13-379): HH[i] = sum_e sum_q w_q hh_i(u_q)
+ * minus the Neumann face terms (hh[0] = u t, transient (u1+u0)/2 t, the
+ * other densities 0 there).  Returns the number of densities. */
+int oc_intdensity(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist, double *HH);
 /* sum_e sum_q w_q detG  (volume, postcore_iga.c with density 1) */
 double oc_volume(const oc_mesh *m);
 

@@ -277,6 +277,22 @@ def volume(mesh):
     return lib().oc_volume(C.byref(mesh.c))
 
 
+def phys_density(ph, u):
+    """gradelast_density (13 values) or gradelasttime_density (2 values)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    hh = np.zeros(13)
+    lib().oc_phys_density(C.byref(ph), _p(hh), _p(u))
+    return hh[:13 if ph.torder == 0 else 2]
+
+
+def intdensity(mesh, ph, bc, U_hist):
+    """iga_intdensity (src/postcore_iga.c:13-379)."""
+    U = np.ascontiguousarray(U_hist, dtype=np.float64)
+    HH = np.zeros(13)
+    n = lib().oc_intdensity(C.byref(mesh.c), C.byref(ph), C.byref(bc), _p(U), _p(HH))
+    return HH[:n]
+
+
 def element_basis(mesh, e, q):
     out = np.zeros(272)
     lib().oc_element_basis(C.byref(mesh.c), e[0], e[1], e[2], q[0], q[1], q[2], _p(out))

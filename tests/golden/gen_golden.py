@@ -9,8 +9,8 @@ Inputs are seeded random jets and points; outputs are what the reference's own
 functions return for them:
   pointwise_static.npz     gradelast_residual / gradelast_tangent /
                            gradelast_density  (phys/gradelast.c:7-1085)
-  pointwise_transient.npz  gradelasttime_residual / gradelasttime_tangent
-                           (phys/gradelasttime.c:7-9181)
+  pointwise_transient.npz  gradelasttime_residual / gradelasttime_tangent /
+                           gradelasttime_density (phys/gradelasttime.c:7-9232)
   basis.npz                eval_Bspline (src/meshcore_iga.c:243-327), evalN
                            (:329-384), legendre_handle n=4 (src/quadrature.c:
                            381-429), matinv 3x3 (src/mathutil.c:17-64)
@@ -50,8 +50,26 @@ def main():
     U = np.stack([rng.normal(scale=s, size=30) for s in scales])
     r = np.stack([_call(R.gradelast_residual, 30, u, PAR_STATIC) for u in U])
     Cm = np.stack([_call(R.gradelast_tangent, 900, u, PAR_STATIC) for u in U])
-    psi = np.stack([_call(R.gradelast_density, 13, u, PAR_STATIC)[0] for u in U])
-    np.savez_compressed(os.path.join(HERE, "pointwise_static.npz"), u=U, par=PAR_STATIC, r=r, C=Cm, psi=psi)
+    hh = np.stack([_call(R.gradelast_density, 13, u, PAR_STATIC) for u in U])
+    psi = hh[:, 0].copy()
+    # jets in the energy's wells: symmetric stretch F = sqrt(I + 2E) with a
+    # deviatoric E on circles in the (e25, e26) plane, small strain gradients
+    UP = []
+    rngP = np.random.default_rng(777)   # own stream: the other fixtures keep their jets
+    for rad in (0.1, 0.2, 0.3):
+        for th in np.linspace(0.0, 2 * np.pi, 8, endpoint=False):
+            e25, e26 = rad * np.cos(th), rad * np.sin(th)
+            d = np.array([e25 / np.sqrt(2) + e26 / np.sqrt(6), -e25 / np.sqrt(2) + e26 / np.sqrt(6), -2 * e26 / np.sqrt(6)])
+            F = np.diag(np.sqrt(1 + 2 * d))
+            u = rngP.normal(scale=0.01, size=30)
+            for k in range(3):
+                for a in range(3):
+                    u[10 * k + 1 + a] = F[k, a] - (1.0 if k == a else 0.0)
+            UP.append(u)
+    UP = np.stack(UP)
+    hhP = np.stack([_call(R.gradelast_density, 13, u, PAR_STATIC) for u in UP])
+    np.savez_compressed(os.path.join(HERE, "pointwise_static.npz"), u=U, par=PAR_STATIC, r=r, C=Cm, psi=psi, hh=hh,
+                        u_phase=UP, hh_phase=hhP)
 
     # ---- F1 transient: 24 (u, u1, u0) triples with increments 1e-4 .. 0.3
     nT = 24
@@ -63,7 +81,8 @@ def main():
     UT = np.stack(UT)
     rT = np.stack([_call(R.gradelasttime_residual, 30, u, PAR_TRANSIENT) for u in UT])
     CT = np.stack([_call(R.gradelasttime_tangent, 900, u, PAR_TRANSIENT) for u in UT])
-    np.savez_compressed(os.path.join(HERE, "pointwise_transient.npz"), u=UT, par=PAR_TRANSIENT, r=rT, C=CT)
+    hT = np.stack([_call(R.gradelasttime_density, 2, u, PAR_TRANSIENT) for u in UT])
+    np.savez_compressed(os.path.join(HERE, "pointwise_transient.npz"), u=UT, par=PAR_TRANSIENT, r=rT, C=CT, hh=hT)
 
     # ---- F2 basis: open and periodic knot vectors, several spans, the 4 Gauss points
     x = np.zeros(4)

@@ -113,3 +113,24 @@ def test_matinv_matches_reference(oracle):
         oracle.lib().oc_matinv3(oracle._p(m))
         assert np.allclose(m, ai, rtol=1e-14, atol=1e-15)
         assert np.allclose(m.reshape(3, 3) @ a.reshape(3, 3), np.eye(3), atol=1e-13)
+
+
+def test_static_density_matches_reference(oracle):
+    """All 13 gradelast_density outputs: energy, the six stress values and the
+    phase indicators (the well jets exercise every indicator)."""
+    g = _load("pointwise_static.npz")
+    ph = oracle.make_phys(0, par=g["par"])
+    for key_u, key_h in (("u", "hh"), ("u_phase", "hh_phase")):
+        for u, hh in zip(g[key_u], g[key_h]):
+            d = oracle.phys_density(ph, u)
+            assert rel_row_err(d[:7], hh[:7]) < 1e-12
+            assert np.array_equal(d[7:], hh[7:])
+    assert g["hh_phase"][:, 7:].sum(axis=0).min() >= 1   # every indicator is hit
+
+
+def test_transient_density_matches_reference(oracle):
+    g = _load("pointwise_transient.npz")
+    ph = oracle.make_phys(2, par=g["par"])
+    for u, hh in zip(g["u"], g["hh"]):
+        d = oracle.phys_density(ph, u)
+        assert abs(d[0] - hh[0]) <= 1e-12 * max(1.0, abs(hh[0])) and d[1] == hh[1] == 1.0

@@ -198,3 +198,31 @@ def test_neumann_constant_traction(oracle):
     nm[6 * 1 + 0 + 0] = 1.0
     F = oracle.neumann(m, oracle.make_bc(neumann=nm))
     assert abs(F[0::3].sum() + 1.0) < 1e-14 and np.all(F[1::3] == 0) and np.all(F[2::3] == 0)
+
+
+def test_intdensity_volume_energy_and_traction_work(oracle):
+    """iga_intdensity: the transient's second density integrates to the volume
+    (ex1's KAT: 1 on the unit cube), the static energy density integrates to
+    the oracle's energy, and a Neumann traction adds -int u.t, which for a
+    uniform traction on a face with u constant there is -area * u * t."""
+    m = oracle.Mesh((4, 4, 4), L=(1, 1, 1), periodic=(1, 1, 1))
+    pht = oracle.make_phys(2)
+    U = oracle.glibc_rand_state(3 * m.ndof)
+    HH = oracle.intdensity(m, pht, oracle.make_bc(), U)
+    assert HH.size == 2 and abs(HH[1] - 1.0) < 1e-12
+    mb = oracle.Mesh((5, 3, 3), L=(4, 1, 1))
+    ph = oracle.make_phys(0, coef=oracle.SVK_COEF)
+    bc = oracle.make_bc()
+    U = oracle.smooth_state(mb, amp=0.01)
+    HH = oracle.intdensity(mb, ph, bc, U)
+    assert HH.size == 13 and abs(HH[0] - oracle.energy(mb, ph, bc, U)) < 1e-12 * abs(HH[0])
+    # traction t on face 1 (x = L) in z, u_z = c everywhere (a rigid shift):
+    # the boundary term is -area * c * t and the volume terms vanish
+    neu = [0.0] * 36
+    neu[6 * 1 + 3 * 0 + 2] = 0.3          # face 1, order 0, dof z
+    bc_t = oracle.make_bc(neumann=neu)
+    c = 0.02
+    U = np.zeros(oracle.Mesh((5, 3, 3), L=(4, 1, 1)).ndof)
+    U[2::3] = c
+    HH = oracle.intdensity(mb, ph, bc_t, U)
+    assert abs(HH[0] - (-1.0 * c * 0.3)) < 1e-13
