@@ -560,6 +560,99 @@ __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, i
     }
 }
 
+// The transient rule weights the pointwise tangent of eight states
+// us_j = m0 + s_j d.  F and H are linear in the jet, F_j = F0 + s_j Fd and
+// H_j = H0 + s_j Hd, so every product of two of them sums over the nodes to
+//   sum_j ct_j X_j Y_j = mu0 X0 Y0 + mu1 (X0 Yd + Xd Y0) + mu2 Xd Yd
+// (mu_p = sum_j ct_j s_j^p) and the N-weighted ones to the same form with the
+// node sums N_p = sum_j ct_j s_j^p N_j; S and G enter linearly (Sb, Gb).
+// The moment state per Gauss point:
+constexpr int MF0 = 0, MFD = 9, MH0 = 18, MHD = 45, MS = 72, MG = 81, MN0 = 90, MN1 = 99, MN2 = 108, MW = 117;
+
+struct Mom {
+    double m0, m1, m2;
+    __device__ __forceinline__ double mp(double x0, double xd, double y0, double yd) const
+    {
+        return m0 * x0 * y0 + m1 * (x0 * yd + xd * y0) + m2 * xd * yd;
+    }
+};
+
+// c_row for the rule-weighted transient tangent, from the moment state ms
+template <int A0>
+__device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, const Mom &mu, int k, int a, int l, double *row)
+{
+    const double *F0 = ms + MF0, *Fd = ms + MFD, *H0 = ms + MH0, *Hd = ms + MHD;
+    const double g2 = 2 * P.le * P.le;
+    const bool dia = (k == l);
+    if (a >= 1 && a <= 3) {
+        const int L = a - 1;
+        const double FkL0 = F0[3 * k + L], FkLd = Fd[3 * k + L];
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            const double Fl0 = F0[3 * l + M], Fld = Fd[3 * l + M];
+            double v = ms[MN0 + 3 * L + M] * FkL0 * Fl0 + ms[MN1 + 3 * L + M] * (FkL0 * Fld + FkLd * Fl0)
+                     + ms[MN2 + 3 * L + M] * FkLd * Fld;
+            double e = 0.0;
+            if (M == L) {
+#pragma unroll
+                for (int J = 0; J < 3; J++)
+                    if (J != L) e += mu.mp(F0[3 * k + J], Fd[3 * k + J], F0[3 * l + J], Fd[3 * l + J]);
+            } else {
+                e = mu.mp(F0[3 * k + M], Fd[3 * k + M], F0[3 * l + L], Fd[3 * l + L]);
+            }
+            v += 0.5 * P.e345 * e;
+            double hh = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                hh += mu.mp(H0[9 * k + 3 * L + c], Hd[9 * k + 3 * L + c], H0[9 * l + 3 * M + c], Hd[9 * l + 3 * M + c]);
+            v += g2 * ((M == L) ? 2.0 / 3.0 : -1.0 / 3.0) * hh;
+            if (dia) v += ms[MS + 3 * L + M];
+            row[1 + M - A0] += v;
+        }
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
+            const int I = PI[pp], K = PK[pp];
+            double v = g2 * ((L == I) ? 2.0 / 3.0 : -1.0 / 3.0)
+                     * mu.mp(H0[9 * k + 3 * L + K], Hd[9 * k + 3 * L + K], F0[3 * l + I], Fd[3 * l + I]);
+            if (I != K)
+                v += g2 * ((L == K) ? 2.0 / 3.0 : -1.0 / 3.0)
+                   * mu.mp(H0[9 * k + 3 * L + I], Hd[9 * k + 3 * L + I], F0[3 * l + K], Fd[3 * l + K]);
+            if (dia) {
+                if (L == I) v += ms[MG + 3 * I + K];
+                if (I != K && L == K) v += ms[MG + 3 * K + I];
+            }
+            row[4 + pp - A0] += v;
+        }
+    } else if (a >= 4) {
+        constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
+        const int I = PI[a - 4], K = PK[a - 4];
+        const bool off = (I != K);
+        const double FkI0 = F0[3 * k + I], FkId = Fd[3 * k + I], FkK0 = F0[3 * k + K], FkKd = Fd[3 * k + K];
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(H0[9 * l + 3 * M + K], Hd[9 * l + 3 * M + K], FkI0, FkId);
+            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(H0[9 * l + 3 * M + I], Hd[9 * l + 3 * M + I], FkK0, FkKd);
+            if (dia) {
+                if (M == I) v += ms[MG + 3 * I + K];
+                if (off && M == K) v += ms[MG + 3 * K + I];
+            }
+            row[1 + M - A0] += v;
+        }
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PJ[6] = {0, 0, 0, 1, 1, 2}, PK2[6] = {0, 1, 2, 1, 2, 2};
+            const int J = PJ[pp], K2 = PK2[pp];
+            double v = 0.0;
+            if (K == K2) v += ((I == J) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkI0, FkId, F0[3 * l + J], Fd[3 * l + J]);
+            if (J != K2 && K == J) v += ((I == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkI0, FkId, F0[3 * l + K2], Fd[3 * l + K2]);
+            if (off && I == K2) v += ((K == J) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkK0, FkKd, F0[3 * l + J], Fd[3 * l + J]);
+            if (off && J != K2 && I == J) v += ((K == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkK0, FkKd, F0[3 * l + K2], Fd[3 * l + K2]);
+            row[4 + pp - A0] += g2 * v;
+        }
+    }
+}
+
 __constant__ const int kPairI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kPairK[6] = {0, 1, 2, 1, 2, 2};
 

@@ -481,12 +481,12 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     // static: the strain-measure state of all 64 Gauss points is built once in
     // stage A; transient: the u-jets of the three levels are kept and the
     // eight rule-node states are built per chunk
-    constexpr int NSTATE = (NH == 1) ? 64 : QC * NNODE;
+    constexpr int SST = (NH == 1) ? 64 * STW : QC * MW;   // static: states; transient: moment states
     __shared__ double sT[3][4][3][3];          // 1-D tables [d][q_d][order][i]
     __shared__ double sCP[27 * STR];
     __shared__ double sGeo[64 * GEO];
     __shared__ double sJet[(NH == 1) ? 1 : 64 * 30 * NH];
-    __shared__ double sSt[NSTATE * STW];
+    __shared__ double sSt[SST];
     // stage A's sum-factorisation buffers share memory with stage B's R and D
     constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
     constexpr int WORK_A = NF * (108 + 288) * (int)sizeof(double);
@@ -646,6 +646,15 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
 #pragma unroll
     for (int t = 0; t < 3; t++) { bi[t] = kBlkI[blk0 + t]; bj[t] = kBlkJ[blk0 + t]; }
 
+    Mom mu = {0.0, 0.0, 0.0};
+    if (NH != 1) {
+#pragma unroll
+        for (int j = 0; j < NNODE; j++) {
+            mu.m0 += P.ct[j];
+            mu.m1 += P.ct[j] * P.s[j];
+            mu.m2 += P.ct[j] * P.s[j] * P.s[j];
+        }
+    }
     for (int chunk = 0; chunk < 64 / QC; chunk++) {
         const int q0 = chunk * QC;
         // ---- B1 (transient: + the rule-node states): wave wv builds the basis
@@ -666,6 +675,8 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
 #pragma unroll
                 for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)0;
             } else if (NH != 1 && b < 32 + NNODE) {
+                // rule node j: S, G and N at us_j, weighted and summed over
+                // the eight lanes 32..39 into the moment state
                 const int j = b - 32;
                 const double *u = sJet + q * 30 * NH;
                 double us[30];
@@ -673,13 +684,48 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                 for (int i = 0; i < 30; i++) us[i] = 0.5 * (u[60 + i] + u[30 + i]) + P.s[j] * (0.5 * (u[i] - u[60 + i]));
                 State st;
                 phys_state(us, P, st);
-                double *sp = sSt + (ql * NNODE + j) * STW;
+                const double w0 = P.ct[j], w1 = w0 * P.s[j], w2 = w1 * P.s[j];
+                double v[45];
 #pragma unroll
-                for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+                for (int i = 0; i < 9; i++) {
+                    v[i] = w0 * st.S[i];
+                    v[9 + i] = w0 * st.G[i];
+                    v[18 + i] = w0 * st.N[i];
+                    v[27 + i] = w1 * st.N[i];
+                    v[36 + i] = w2 * st.N[i];
+                }
 #pragma unroll
-                for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+                for (int i = 0; i < 45; i++) {
+                    v[i] += __shfl_xor(v[i], 1, 64);
+                    v[i] += __shfl_xor(v[i], 2, 64);
+                    v[i] += __shfl_xor(v[i], 4, 64);
+                }
+                if (j == 0) {
+                    double *ms = sSt + ql * MW;
 #pragma unroll
-                for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
+                    for (int i = 0; i < 45; i++) ms[MS + i] = v[i];
+                }
+            } else if (NH != 1 && b >= 40) {
+                // F0, Fd, H0, Hd of the midpoint m0 = (u1+u0)/2 and the
+                // increment d = (u-u0)/2
+                const double *u = sJet + q * 30 * NH;
+                double *ms = sSt + ql * MW;
+                for (int i = b - 40; i < 72; i += 24) {
+                    const bool isF = i < 18, dpart = isF ? (i >= 9) : (i >= 45);
+                    const int r = isF ? (i % 9) : ((i - 18) % 27);
+                    int slot, k;
+                    if (isF) {
+                        k = r / 3;
+                        slot = 10 * k + 1 + (r % 3);
+                    } else {
+                        k = r / 9;
+                        slot = 10 * k + sidx((r % 9) / 3, r % 3);
+                    }
+                    const double m0v = 0.5 * (u[30 + slot] + u[60 + slot]), dv = 0.5 * (u[slot] - u[60 + slot]);
+                    double val = dpart ? dv : m0v;
+                    if (isF && !dpart && (r / 3) == (r % 3)) val += 1.0;
+                    ms[i] = val;
+                }
             }
         }
         if (NH != 1) __syncthreads();   // node states are read by the whole wave below
@@ -690,11 +736,11 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             if (lane < 6 * NA) {
                 const int p = lane / NA, ar = lane - p * NA, k = kBlkI[p], l = kBlkJ[p];
                 const double wgt = sGeo[q * GEO];
-                const double *S0 = (NH == 1) ? sSt + q * STW : sSt + ql * NNODE * STW;
                 double row[NA];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
-                for (int j = 0; j < NNODE; j++) c_row<A0>(S0 + j * STW, P, k, ar + A0, l, (NH == 1 ? 1.0 : P.ct[j]), row);
+                if (NH == 1) c_row<A0>(sSt + q * STW, P, k, ar + A0, l, 1.0, row);
+                else c_row_mom<A0>(sSt + ql * MW, P, mu, k, ar + A0, l, row);
                 if (NH != 1 && ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
                 T *D = sD[ql] + p * NN + ar * RS;
 #pragma unroll

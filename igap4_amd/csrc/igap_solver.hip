@@ -700,6 +700,7 @@ extern "C" int igap_gpu_solver_default_opts(igap_solver_opts *o)
     o->tangent_prec = IGAP_F64;
     o->linesearch = 1;
     o->pc_order = 0;
+    o->pc_lag = 1;
     return IGAP_OK;
 }
 
@@ -710,6 +711,7 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     if (o->ksp_restart < 1 || o->ksp_restart > 2000) return fail(IGAP_ERR_ARG, "ksp_restart %d out of [1, 2000]", o->ksp_restart);
     if (o->tangent_prec != IGAP_F64 && o->tangent_prec != IGAP_F32) return fail(IGAP_ERR_ARG, "tangent_prec");
     if (o->pc_order != 0 && o->pc_order != 1) return fail(IGAP_ERR_ARG, "pc_order must be 0 (natural) or 1 (coloured)");
+    if (o->pc_lag < 0) return fail(IGAP_ERR_ARG, "pc_lag must be >= 0");
     HIPCHK(hipSetDevice(c->device));
     std::memset(st, 0, sizeof *st);
     auto T0 = clk::now();
@@ -746,10 +748,11 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         HIPCHK(hipStreamSynchronize(c->stream));
         st->t_tangent_ms += ms(t, clk::now());
         t = clk::now();
-        if (o->pc == IGAP_PC_ILU0) {
+        const bool refactor = (it % (o->pc_lag > 1 ? o->pc_lag : 1)) == 0;
+        if (o->pc == IGAP_PC_ILU0 && refactor) {
             rc = o->pc_order == 1 ? pc_ilu_setup_mc(c, s, p) : pc_ilu_setup(c, s, p);
             if (rc) return rc;
-        } else if (o->pc == IGAP_PC_JACOBI) {
+        } else if (o->pc == IGAP_PC_JACOBI && refactor) {
             if (p == IGAP_F64) hipLaunchKernelGGL(k_diag_inv<double>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, s->diag);
             else hipLaunchKernelGGL(k_diag_inv<float>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val32, s->diag);
         }

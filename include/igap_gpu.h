@@ -171,6 +171,9 @@ typedef struct {
                           /* 27 colours of control points (colour 9(i%3)+3(j%3)+k%3, the   */
                           /* three dofs of a point together): ~25x fewer dependent levels  */
                           /* in the triangular solves, a different incomplete factor       */
+    int    pc_lag;        /* refactor the preconditioner every pc_lag-th Newton iteration  */
+                          /* of a solve (1, the reference: every iteration; k > 1 keeps    */
+                          /* the factor of an earlier tangent, SNESSetLagPreconditioner)   */
 } igap_solver_opts;
 
 typedef struct {
