@@ -170,7 +170,10 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
 #pragma unroll
                 for (int a = 0; a < 10; a++) sJet[q * 30 * NH + 30 * h + 10 * k + a] = Jg[h][a];
             }
-        } else {
+        }
+        // sGeo may share memory with the field buffers read above
+        __syncthreads();
+        if (part == 0) {
             double *g = sGeo + q * GEO_R;
             double dX[3];
 #pragma unroll
@@ -185,7 +188,6 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
         }
     }
     __syncthreads();
-
 }
 
 template <int NH>
@@ -195,13 +197,15 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
     constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
     constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
     constexpr int WA = NF * (108 + 288);       // stage-A buffers
-    constexpr int WT = 1920 + 864 + 324;       // unused + x^T and y^T buffers (rho lives in sJet)
     constexpr int GEO_R = 38;                  // wgt, Gi[9], X2[18], W[10]
+    constexpr int WG = 64 * GEO_R;             // the Gauss-point geometry (after stage A)
+    constexpr int WT = 864 + 324;              // x^T and y^T buffers (after the adjoint; rho lives in sJet)
+    constexpr int W1 = WA > WG ? WA : WG;
     __shared__ double sT[3][4][3][3];
     __shared__ double sCP[27 * STR];
-    __shared__ double sWork[WA > WT ? WA : WT];
+    __shared__ double sWork[W1 > WT ? W1 : WT];   // stage A, then sGeo, then x^T / y^T
     __shared__ double sJet[64 * 30 * NH];      // physical u-jets [q][h][k][a]; then r_q / rho_q [q][k][a]
-    __shared__ double sGeo[64 * GEO_R];
+    double *sGeo = sWork;
 
     const int tid = threadIdx.x;
     const int e = blockIdx.x;
@@ -269,8 +273,8 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
     __syncthreads();
     //      x^T: C1[k][bx][qy][qz][s6] = sum_qx sum_{a: S6(a) = s6} rho[q][k][a] tx[qx][DX(a)][bx]
     double *C1 = sWork, *C2 = sWork + 864;
-    for (int i = tid; i < 864; i += 256) {
-        const int s6 = i % 6, t = i / 6, qz = t & 3, qy = (t >> 2) & 3, bx = (t >> 4) % 3, k = t / 48;
+    for (int i = tid; i < 864; i += 256) {   // s6 slowest: the switch below is mostly wave-uniform
+        const int s6 = i / 144, t = i - 144 * s6, qz = t & 3, qy = (t >> 2) & 3, bx = (t >> 4) % 3, k = t / 48;
         double acc = 0.0;
 #pragma unroll
         for (int qx = 0; qx < 4; qx++) {
@@ -285,12 +289,12 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
                 default: acc += rq[9] * tx[bx]; break;
             }
         }
-        C1[i] = acc;   // [k][bx][qy][qz][s6]
+        C1[6 * t + s6] = acc;   // [k][bx][qy][qz][s6]
     }
     __syncthreads();
     //      y^T: C2[k][bx][by][qz][dz] = sum_qy sum_{s6: dz(s6) = dz} C1[..][s6] ty[qy][dy(s6)][by]
-    for (int i = tid; i < 324; i += 256) {
-        const int dz = i % 3, t = i / 3, qz = t & 3, u = t >> 2, by = u % 3, bx = (u / 3) % 3, k = u / 9;
+    for (int i = tid; i < 324; i += 256) {   // dz slowest (mostly wave-uniform branches)
+        const int dz = i / 108, t = i - 108 * dz, qz = t & 3, u = t >> 2, by = u % 3, bx = (u / 3) % 3, k = u / 9;
         double acc = 0.0;
 #pragma unroll
         for (int qy = 0; qy < 4; qy++) {
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const d
             else if (dz == 1) acc += c[2] * ty[by] + c[4] * ty[3 + by];
             else acc += c[5] * ty[by];
         }
-        C2[i] = acc;   // [k][bx][by][qz][dz]
+        C2[3 * t + dz] = acc;   // [k][bx][by][qz][dz]
     }
     __syncthreads();
     //      z^T: R_e[3 b + k] = w_b sum_qz sum_dz C2[k][bx][by][qz][dz] tz[qz][dz][bz]
@@ -331,9 +335,10 @@ __global__ __launch_bounds__(256) void k_density_elem(MeshK M, PhysK P, const do
     constexpr int ND = (NH == 1) ? ND_STATIC : ND_TRANSIENT;
     __shared__ double sT[3][4][3][3];
     __shared__ double sCP[27 * STR];
-    __shared__ double sWork[NF * (108 + 288)];
+    constexpr int WA = NF * (108 + 288), WG = 64 * GEO_R;
+    __shared__ double sWork[WA > WG ? WA : WG];   // stage A, then sGeo
     __shared__ double sJet[64 * 30 * NH];
-    __shared__ double sGeo[64 * GEO_R];
+    double *sGeo = sWork;
     const int e = blockIdx.x;
     gp_fields<NH>(M, Uh, ndof, e, sT, sCP, sWork, sJet, sGeo);
     if (threadIdx.x < 64) {
