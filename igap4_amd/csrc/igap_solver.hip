@@ -58,6 +58,8 @@ struct Solver {
     int mc_cstart[28] = {};         // first row of each colour (rows of a colour are contiguous)
     int mc_sweep_ok = 0;            // no two points of one colour are coupled (false for a periodic
                                     // direction whose size is not a multiple of 3): colour sweeps valid
+    int mc_block_ok = 0;            // point-block pattern (k_mcb_* applicable)
+    double *part = nullptr;         // partial dot products, (m+2) x ceil(n / DOT_CHUNK)
     rocsparse_mat_info mc_info[2] = {nullptr, nullptr};
     void *mc_ilu[2] = {nullptr, nullptr};
     void *mc_buf[2] = {nullptr, nullptr};
@@ -80,7 +82,7 @@ void solver_destroy(Solver *s)
     if (s->bl) rocblas_destroy_handle(s->bl);
     void *bufs[] = {s->ilu[0], s->ilu[1], s->buf[0], s->buf[1], s->rowptr32, s->V, s->w, s->z, s->t, s->d, s->x,
                     s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag, s->t2, s->mc_perm, s->mc_rowptr,
-                    s->mc_col, s->mc_src, s->mc_dpos, s->mc_ilu[0], s->mc_ilu[1], s->mc_buf[0], s->mc_buf[1]};
+                    s->mc_col, s->mc_src, s->mc_dpos, s->part, s->mc_ilu[0], s->mc_ilu[1], s->mc_buf[0], s->mc_buf[1]};
     for (void *b : bufs)
         if (b) hipFree(b);
     delete s;
@@ -179,6 +181,142 @@ __global__ __launch_bounds__(256) void k_mc_upper(int r0, int npts, const int32_
     }
 }
 
+// The same solves on the point-block structure of the pattern: the three rows
+// of a point list the same neighbour points, each as three consecutive
+// columns 3Q..3Q+2, lower neighbours first, then the point itself, then the
+// upper ones (checked on the host when the ordering is built).  One wave per
+// point, lane k takes neighbour points k and k+64: one column load, three
+// vector loads and nine factor entries per neighbour, so the dependent load
+// chain is col -> y once instead of once per 64 entries of each row.
+template <typename T>
+__global__ __launch_bounds__(64) void k_mcb_lower(int r0, int npts, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const int32_t *__restrict__ dpos, const T *__restrict__ f,
+                                                  const double *__restrict__ b, double *__restrict__ y)
+{
+    const int qi = blockIdx.x, lane = threadIdx.x;
+    if (qi >= npts) return;
+    const int r = r0 + 3 * qi;
+    const int b0 = rp[r], b1 = rp[r + 1], b2 = rp[r + 2];
+    const int nl = (dpos[r] - b0) / 3;
+    double l10 = 0, l20 = 0, l21 = 0;
+    if (lane == 0) {
+        l10 = (double)f[b1 + 3 * nl];
+        l20 = (double)f[b2 + 3 * nl];
+        l21 = (double)f[b2 + 3 * nl + 1];
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = lane; k < nl; k += 64) {
+        const int q = col[b0 + 3 * k];
+        const double y0 = y[q], y1 = y[q + 1], y2 = y[q + 2];
+        const int o = 3 * k;
+        s0 += (double)f[b0 + o] * y0 + (double)f[b0 + o + 1] * y1 + (double)f[b0 + o + 2] * y2;
+        s1 += (double)f[b1 + o] * y0 + (double)f[b1 + o + 1] * y1 + (double)f[b1 + o + 2] * y2;
+        s2 += (double)f[b2 + o] * y0 + (double)f[b2 + o + 1] * y1 + (double)f[b2 + o + 2] * y2;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+        const double y0 = b[r] - s0;
+        const double y1 = b[r + 1] - s1 - l10 * y0;
+        const double y2 = b[r + 2] - s2 - l20 * y0 - l21 * y1;
+        y[r] = y0;
+        y[r + 1] = y1;
+        y[r + 2] = y2;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_mcb_upper(int r0, int npts, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const int32_t *__restrict__ dpos, const T *__restrict__ f,
+                                                  const double *__restrict__ y, double *__restrict__ z)
+{
+    const int qi = blockIdx.x, lane = threadIdx.x;
+    if (qi >= npts) return;
+    const int r = r0 + 3 * qi;
+    const int b0 = rp[r], b1 = rp[r + 1], b2 = rp[r + 2], e0 = rp[r + 3];
+    const int nl = (dpos[r] - b0) / 3;
+    const int nu = (e0 - b2) / 3 - nl - 1;
+    const int u0 = b0 + 3 * nl + 3, u1 = b1 + 3 * nl + 3, u2 = b2 + 3 * nl + 3;   // first upper entry per row
+    double d00 = 1, d01 = 0, d02 = 0, d11 = 1, d12 = 0, d22 = 1;
+    if (lane == 0) {
+        d00 = (double)f[u0 - 3];
+        d01 = (double)f[u0 - 2];
+        d02 = (double)f[u0 - 1];
+        d11 = (double)f[u1 - 2];
+        d12 = (double)f[u1 - 1];
+        d22 = (double)f[u2 - 1];
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = lane; k < nu; k += 64) {
+        const int q = col[u0 + 3 * k];
+        const double z0 = z[q], z1 = z[q + 1], z2 = z[q + 2];
+        const int o = 3 * k;
+        s0 += (double)f[u0 + o] * z0 + (double)f[u0 + o + 1] * z1 + (double)f[u0 + o + 2] * z2;
+        s1 += (double)f[u1 + o] * z0 + (double)f[u1 + o + 1] * z1 + (double)f[u1 + o + 2] * z2;
+        s2 += (double)f[u2 + o] * z0 + (double)f[u2 + o + 1] * z1 + (double)f[u2 + o + 2] * z2;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+        const double z2 = (y[r + 2] - s2) / d22;
+        const double z1 = (y[r + 1] - s1 - d12 * z2) / d11;
+        const double z0 = (y[r] - s0 - d01 * z1 - d02 * z2) / d00;
+        z[r] = z0;
+        z[r + 1] = z1;
+        z[r + 2] = z2;
+    }
+}
+
+// Partial dot products of the Krylov vectors with w: part[k][ch] =
+// sum over rows of chunk ch of V[k ld + i] w[i]; one workgroup per (chunk,
+// vector), fixed summation order (deterministic).
+constexpr int DOT_CHUNK = 4096;
+__global__ __launch_bounds__(256) void k_dot_part(int64_t n, int64_t ld, const double *__restrict__ V, const double *__restrict__ w,
+                                                  double *__restrict__ part)
+{
+    __shared__ double red[4];
+    const int ch = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+    const int64_t i0 = (int64_t)ch * DOT_CHUNK;
+    const int64_t i1 = i0 + DOT_CHUNK < n ? i0 + DOT_CHUNK : n;
+    const double *v = V + (int64_t)k * ld;
+    double s = 0;
+    for (int64_t i = i0 + tid; i < i1; i += 256) s += v[i] * w[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) part[(int64_t)k * gridDim.x + ch] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// h[k] = sum_ch part[k][ch] (one wave per vector; fixed order)
+__global__ __launch_bounds__(64) void k_dot_final(int nch, const double *__restrict__ part, double *__restrict__ h)
+{
+    const int k = blockIdx.x, lane = threadIdx.x;
+    double s = 0;
+    for (int c = lane; c < nch; c += 64) s += part[(int64_t)k * nch + c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) h[k] = s;
+}
+
+// x *= 1/sqrt(*nrm2) when the squared norm is positive
+__global__ void k_scale_rsqrt(int64_t n, const double *__restrict__ nrm2, double *__restrict__ x)
+{
+    const double q = *nrm2;
+    if (!(q > 0.0)) return;
+    const double a = 1.0 / sqrt(q);
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) x[i] *= a;
+}
+
 template <typename T>
 __global__ void k_diag_inv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const T *__restrict__ val,
                            double *__restrict__ dinv)
@@ -275,7 +413,8 @@ static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
         HIPCHK(hipMalloc(&s->V, sizeof(double) * (size_t)(m + 1) * n));
         double **vecs[] = {&s->w, &s->z, &s->t, &s->t2, &s->d, &s->x, &s->xt, &s->F, &s->Ft, &s->diag};
         for (double **v : vecs) HIPCHK(hipMalloc(v, sizeof(double) * n));
-        HIPCHK(hipMalloc(&s->h, sizeof(double) * (m + 2)));
+        HIPCHK(hipMalloc(&s->h, sizeof(double) * 3 * (m + 2)));
+        HIPCHK(hipMalloc(&s->part, sizeof(double) * (size_t)(m + 2) * (size_t)((n + DOT_CHUNK - 1) / DOT_CHUNK)));
         HIPCHK(hipMalloc(&s->fa, sizeof(float) * n));
         HIPCHK(hipMalloc(&s->fb, sizeof(float) * n));
         HIPCHK(hipMalloc(&s->rowptr32, sizeof(int32_t) * (n + 1)));
@@ -358,6 +497,22 @@ static int mc_build(igap_gpu_ctx *c, Solver *s)
         for (int32_t j = prp[r]; j < prp[r + 1]; j++) {
             const int32_t cj = pcol[j];
             if ((cj < q0 || cj > q0 + 2) && colour(perm[cj] / 3) == colour(perm[r] / 3)) { s->mc_sweep_ok = 0; break; }
+        }
+    }
+    // point-block structure: the three rows of a point have equal length and
+    // list each neighbour point as the columns 3Q, 3Q+1, 3Q+2, the point's own
+    // block after its lower neighbours
+    s->mc_block_ok = s->mc_sweep_ok;
+    for (int64_t r = 0; r < n && s->mc_block_ok; r += 3) {
+        const int32_t b0 = prp[r], len = prp[r + 1] - b0;
+        if (len % 3 || (pdpos[r] - b0) % 3) { s->mc_block_ok = 0; break; }
+        for (int d = 0; d < 3 && s->mc_block_ok; d++) {
+            const int32_t bd = prp[r + d];
+            if (prp[r + d + 1] - bd != len || pdpos[r + d] - bd != pdpos[r] - b0 + d) { s->mc_block_ok = 0; break; }
+            for (int32_t k = 0; k < len; k += 3) {
+                const int32_t q = pcol[b0 + k];
+                if (q % 3 || pcol[bd + k] != q || pcol[bd + k + 1] != q + 1 || pcol[bd + k + 2] != q + 2) { s->mc_block_ok = 0; break; }
+            }
         }
     }
     HIPCHK(hipMalloc(&s->mc_dpos, sizeof(int32_t) * n));
@@ -524,6 +679,31 @@ static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int order, const 
             HIPCHK(hipGetLastError());
             return IGAP_OK;
         }
+        if (s->mc_block_ok) {
+            for (int cc = 0; cc < 27; cc++) {
+                const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
+                if (np == 0) continue;
+                if (p == IGAP_F64)
+                    hipLaunchKernelGGL(k_mcb_lower<double>, dim3((unsigned)np), dim3(64), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col,
+                                       s->mc_dpos, (const double *)s->mc_ilu[p], s->t, s->t2);
+                else
+                    hipLaunchKernelGGL(k_mcb_lower<float>, dim3((unsigned)np), dim3(64), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col,
+                                       s->mc_dpos, (const float *)s->mc_ilu[p], s->t, s->t2);
+            }
+            for (int cc = 26; cc >= 0; cc--) {
+                const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
+                if (np == 0) continue;
+                if (p == IGAP_F64)
+                    hipLaunchKernelGGL(k_mcb_upper<double>, dim3((unsigned)np), dim3(64), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col,
+                                       s->mc_dpos, (const double *)s->mc_ilu[p], s->t2, s->t);
+                else
+                    hipLaunchKernelGGL(k_mcb_upper<float>, dim3((unsigned)np), dim3(64), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col,
+                                       s->mc_dpos, (const float *)s->mc_ilu[p], s->t2, s->t);
+            }
+            hipLaunchKernelGGL(k_scatter, grid1(n), dim3(256), 0, c->stream, n, s->mc_perm, s->t, y);
+            HIPCHK(hipGetLastError());
+            return IGAP_OK;
+        }
         for (int cc = 0; cc < 27; cc++) {
             const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
             if (np == 0) continue;
@@ -595,6 +775,17 @@ static int nrm2(igap_gpu_ctx *c, Solver *s, const double *x, double *out)
     return IGAP_OK;
 }
 
+// h[k] = V[:, k] . w for k < nk (columns of V at stride ld)
+static int dots(igap_gpu_ctx *c, Solver *s, const double *V, int64_t ld, int nk, const double *w, double *h)
+{
+    const int64_t n = c->ndof;
+    const int nch = (int)((n + DOT_CHUNK - 1) / DOT_CHUNK);
+    hipLaunchKernelGGL(k_dot_part, dim3((unsigned)nch, (unsigned)nk), dim3(256), 0, c->stream, n, ld, V, w, s->part);
+    hipLaunchKernelGGL(k_dot_final, dim3((unsigned)nk), dim3(64), 0, c->stream, nch, (const double *)s->part, h);
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
 // GMRES(m), right preconditioned: solve A d = b (d starts at 0)
 static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres)
 {
@@ -627,20 +818,24 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
             rc = spmv(c, p, s->z, vn);
             if (rc) return rc;
             // classical Gram-Schmidt, twice
-            const double one = 1.0, zero = 0.0, mone = -1.0;
-            for (int k = 0; k <= j; k++) hv[k] = 0.0;
+            const double one = 1.0, mone = -1.0;
+            // (both projections, the norm and the normalisation stay on the
+            // device; one copy back per step)
+            const int hs = m + 2;
             for (int pass = 0; pass < 2; pass++) {
-                BLCHK(rocblas_dgemv(s->bl, rocblas_operation_transpose, N, j + 1, &one, s->V, N, vn, 1, &zero, s->h, 1));
-                BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j + 1, &mone, s->V, N, s->h, 1, &one, vn, 1));
-                std::vector<double> hp(j + 1);
-                HIPCHK(hipMemcpyAsync(hp.data(), s->h, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(hipStreamSynchronize(c->stream));
-                for (int k = 0; k <= j; k++) hv[k] += hp[k];
+                rc = dots(c, s, s->V, n, j + 1, vn, s->h + pass * hs);
+                if (rc) return rc;
+                BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j + 1, &mone, s->V, N, s->h + pass * hs, 1, &one, vn, 1));
             }
-            double hn = 0;
-            rc = nrm2(c, s, vn, &hn);
+            rc = dots(c, s, vn, n, 1, vn, s->h + 2 * hs);
             if (rc) return rc;
-            if (hn > 0) hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0 / hn, vn, 0.0, vn);
+            hipLaunchKernelGGL(k_scale_rsqrt, grid1(n), dim3(256), 0, c->stream, n, s->h + 2 * hs, vn);
+            HIPCHK(hipGetLastError());
+            std::vector<double> hp(3 * hs);
+            HIPCHK(hipMemcpyAsync(hp.data(), s->h, sizeof(double) * 3 * hs, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int k = 0; k <= j; k++) hv[k] = hp[k] + hp[hs + k];
+            const double hn = hp[2 * hs] > 0.0 ? std::sqrt(hp[2 * hs]) : 0.0;
             // Givens
             for (int k = 0; k < j; k++) {
                 double t = cs[k] * hv[k] + sn[k] * hv[k + 1];
