@@ -102,6 +102,7 @@ _SIGS = {
     "igap_gpu_timing_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_i64_p]),
     "igap_gpu_timing_reset": (C.c_int, [C.c_void_p]),
     "igap_gpu_selftest_mfma": (C.c_int, [c_double_p, c_float_p]),
+    "igap_gpu_selftest_ilu": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_double_p]),
     "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),
     "igap_gpu_snes_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     # include/igap_ws.h

@@ -202,6 +202,14 @@ class Assm:
         check(lib().igap_gpu_intdensity(self.ctx, _dp(HH), C.byref(n)))
         return HH[:n.value]
 
+    def selftest_ilu(self, prec="f64"):
+        """Own point-block ILU(0) against rocSPARSE's on the tangent last
+        assembled in ``prec``: (applicable, max|a-b|/max|b|)."""
+        ok = C.c_int()
+        d = C.c_double()
+        check(lib().igap_gpu_selftest_ilu(self.ctx, 0 if prec == "f64" else 1, C.byref(ok), C.byref(d)))
+        return bool(ok.value), d.value
+
     def save_solution(self, fname):
         """IGAPSolvSaveSolution: the reference's solution file (universal dof order)."""
         check(lib().igap_gpu_save_solution(self.ctx, str(fname).encode()))

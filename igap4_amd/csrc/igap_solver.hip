@@ -59,6 +59,8 @@ struct Solver {
     int mc_sweep_ok = 0;            // no two points of one colour are coupled (false for a periodic
                                     // direction whose size is not a multiple of 3): colour sweeps valid
     int mc_block_ok = 0;            // point-block pattern (k_mcb_* applicable)
+    int32_t *mc_pcoord = nullptr;   // per permuted point: owned-grid (i, j, k) packed 10 bits each
+    int *zpiv = nullptr;            // first row with a zero pivot of the last own factorisation
     double *part = nullptr;         // partial dot products, (m+2) x ceil(n / DOT_CHUNK)
     rocsparse_mat_info mc_info[2] = {nullptr, nullptr};
     void *mc_ilu[2] = {nullptr, nullptr};
@@ -82,7 +84,7 @@ void solver_destroy(Solver *s)
     if (s->bl) rocblas_destroy_handle(s->bl);
     void *bufs[] = {s->ilu[0], s->ilu[1], s->buf[0], s->buf[1], s->rowptr32, s->V, s->w, s->z, s->t, s->d, s->x,
                     s->xt, s->F, s->Ft, s->h, s->fa, s->fb, s->diag, s->t2, s->mc_perm, s->mc_rowptr,
-                    s->mc_col, s->mc_src, s->mc_dpos, s->part, s->mc_ilu[0], s->mc_ilu[1], s->mc_buf[0], s->mc_buf[1]};
+                    s->mc_col, s->mc_src, s->mc_dpos, s->part, s->mc_pcoord, s->zpiv, s->mc_ilu[0], s->mc_ilu[1], s->mc_buf[0], s->mc_buf[1]};
     for (void *b : bufs)
         if (b) hipFree(b);
     delete s;
@@ -274,6 +276,126 @@ __global__ __launch_bounds__(64) void k_mcb_upper(int r0, int npts, const int32_
         z[r + 2] = z2;
     }
 }
+
+// ILU(0) of the coloured-order matrix on its point-block pattern: the
+// scalar ILU(0) (unit lower L, upper U with diagonal, no fill outside the
+// pattern) computed a point at a time.  Colours are factored in order (one
+// launch each); a point's three rows depend only on rows of earlier colours,
+// already final.  One wave per point, its rows in LDS: for each lower
+// neighbour Q in increasing order, the 3x3 block L_PQ = A_PQ U_QQ^-1 (the
+// scalar eliminations of Q's three columns), then A_PR -= L_PQ U_QR for every
+// upper neighbour R of Q that is also a neighbour of P (found through the
+// grid offset of R from P, periodic directions taken modulo their size);
+// finally the scalar elimination inside the point's own diagonal block, which
+// also updates its upper neighbours.  The summation order differs from a
+// row-by-row scalar factorisation; the factor is the same up to roundoff.
+constexpr int MCB_MAXNB = 125;   // 5^3 stencil points
+__device__ __forceinline__ int mcb_offcode(int a, int b, int nb0, int nb1, int nb2, int per0, int per1, int per2)
+{
+    int di = (b & 1023) - (a & 1023), dj = ((b >> 10) & 1023) - ((a >> 10) & 1023), dk = (b >> 20) - (a >> 20);
+    if (per0) { if (di > nb0 / 2) di -= nb0; else if (di < -(nb0 / 2)) di += nb0; }
+    if (per1) { if (dj > nb1 / 2) dj -= nb1; else if (dj < -(nb1 / 2)) dj += nb1; }
+    if (per2) { if (dk > nb2 / 2) dk -= nb2; else if (dk < -(nb2 / 2)) dk += nb2; }
+    if (di < -2 || di > 2 || dj < -2 || dj > 2 || dk < -2 || dk > 2) return -1;
+    return (di + 2) * 25 + (dj + 2) * 5 + (dk + 2);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_mcb_ilu(int r0, int npts, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                const int32_t *__restrict__ dpos, const int32_t *__restrict__ pcoord, int nb0,
+                                                int nb1, int nb2, int per0, int per1, int per2, T *__restrict__ f, int *__restrict__ zpiv)
+{
+    __shared__ T A[3][3 * MCB_MAXNB];
+    __shared__ int nbr[MCB_MAXNB];
+    __shared__ int map[MCB_MAXNB];
+    __shared__ T L[3][3];
+    const int qi = blockIdx.x, lane = threadIdx.x;
+    if (qi >= npts) return;
+    const int r = r0 + 3 * qi;
+    const int b0 = rp[r], b1 = rp[r + 1], b2 = rp[r + 2];
+    const int len = b1 - b0, nnb = len / 3, nl = (dpos[r] - b0) / 3;
+    for (int i = lane; i < len; i += 64) {
+        A[0][i] = f[b0 + i];
+        A[1][i] = f[b1 + i];
+        A[2][i] = f[b2 + i];
+    }
+    for (int i = lane; i < MCB_MAXNB; i += 64) map[i] = -1;
+    __syncthreads();
+    const int cp = pcoord[r / 3];
+    for (int t = lane; t < nnb; t += 64) {
+        const int q = col[b0 + 3 * t] / 3;
+        nbr[t] = q;
+        const int code = mcb_offcode(cp, pcoord[q], nb0, nb1, nb2, per0, per1, per2);
+        if (code >= 0) map[code] = t;
+    }
+    __syncthreads();
+    for (int s = 0; s < nl; s++) {
+        const int Q = nbr[s];
+        const int qb0 = rp[3 * Q], qb1 = rp[3 * Q + 1], qb2 = rp[3 * Q + 2];
+        const int qd = dpos[3 * Q] - qb0, qnnb = (qb1 - qb0) / 3;
+        if (lane < 3) {
+            const T u00 = f[qb0 + qd], u01 = f[qb0 + qd + 1], u02 = f[qb0 + qd + 2];
+            const T u11 = f[qb1 + qd + 1], u12 = f[qb1 + qd + 2], u22 = f[qb2 + qd + 2];
+            const T a0 = A[lane][3 * s] / u00;
+            const T a1 = (A[lane][3 * s + 1] - a0 * u01) / u11;
+            const T a2 = (A[lane][3 * s + 2] - a0 * u02 - a1 * u12) / u22;
+            A[lane][3 * s] = a0;
+            A[lane][3 * s + 1] = a1;
+            A[lane][3 * s + 2] = a2;
+            L[lane][0] = a0;
+            L[lane][1] = a1;
+            L[lane][2] = a2;
+        }
+        __syncthreads();
+        for (int t = qd / 3 + 1 + lane; t < qnnb; t += 64) {
+            const int code = mcb_offcode(cp, pcoord[col[qb0 + 3 * t] / 3], nb0, nb1, nb2, per0, per1, per2);
+            const int tp = code >= 0 ? map[code] : -1;
+            if (tp < 0) continue;
+            T u[3][3];
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                u[0][e] = f[qb0 + 3 * t + e];
+                u[1][e] = f[qb1 + 3 * t + e];
+                u[2][e] = f[qb2 + 3 * t + e];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int e = 0; e < 3; e++) A[d][3 * tp + e] -= L[d][0] * u[0][e] + L[d][1] * u[1][e] + L[d][2] * u[2][e];
+        }
+        __syncthreads();
+    }
+    const int o = 3 * nl;
+    const T p0 = A[0][o];
+    const T l10 = A[1][o] / p0, l20 = A[2][o] / p0;
+    __syncthreads();
+    for (int i = o + 1 + lane; i < len; i += 64) {
+        A[1][i] -= l10 * A[0][i];
+        A[2][i] -= l20 * A[0][i];
+    }
+    if (lane == 0) {
+        A[1][o] = l10;
+        A[2][o] = l20;
+    }
+    __syncthreads();
+    const T p1 = A[1][o + 1];
+    const T l21 = A[2][o + 1] / p1;
+    __syncthreads();
+    for (int i = o + 2 + lane; i < len; i += 64) A[2][i] -= l21 * A[1][i];
+    if (lane == 0) A[2][o + 1] = l21;
+    __syncthreads();
+    if (lane == 0) {
+        const int zr = p0 == (T)0 ? r : p1 == (T)0 ? r + 1 : A[2][o + 2] == (T)0 ? r + 2 : -1;
+        if (zr >= 0) atomicMin(zpiv, zr);
+    }
+    for (int i = lane; i < len; i += 64) {
+        f[b0 + i] = A[0][i];
+        f[b1 + i] = A[1][i];
+        f[b2 + i] = A[2][i];
+    }
+}
+
+__global__ void k_set_i32(int *p, int v) { *p = v; }
 
 // Partial dot products of the Krylov vectors with w: part[k][ch] =
 // sum over rows of chunk ch of V[k ld + i] w[i]; one workgroup per (chunk,
@@ -515,6 +637,22 @@ static int mc_build(igap_gpu_ctx *c, Solver *s)
             }
         }
     }
+    for (int64_t r = 0; r < n && s->mc_block_ok; r += 3)
+        if (prp[r + 1] - prp[r] > 3 * MCB_MAXNB) s->mc_block_ok = 0;
+    if (s->mc_block_ok) {
+        if (c->nb[0] > 1023 || c->nb[1] > 1023 || c->nb[2] > 1023) s->mc_block_ok = 0;
+    }
+    {
+        std::vector<int32_t> pc(ncp);
+        for (int64_t q = 0; q < ncp; q++) {
+            const int64_t a = cps[q];
+            const int64_t i = a / ((int64_t)nb1 * nb2), j = (a / nb2) % nb1, k = a % nb2;
+            pc[q] = (int32_t)(i | (j << 10) | (k << 20));
+        }
+        HIPCHK(hipMalloc(&s->mc_pcoord, sizeof(int32_t) * ncp));
+        HIPCHK(hipMemcpyAsync(s->mc_pcoord, pc.data(), sizeof(int32_t) * ncp, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     HIPCHK(hipMalloc(&s->mc_dpos, sizeof(int32_t) * n));
     HIPCHK(hipMemcpyAsync(s->mc_dpos, pdpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMalloc(&s->mc_perm, sizeof(int32_t) * n));
@@ -530,7 +668,34 @@ static int mc_build(igap_gpu_ctx *c, Solver *s)
 }
 
 // ILU(0) of the tangent in the coloured order, precision p
-static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p)
+// own factorisation (k_mcb_ilu), colour by colour, of the values in mc_ilu[p]
+static int mc_factor_own(igap_gpu_ctx *c, Solver *s, int p)
+{
+    if (!s->zpiv) HIPCHK(hipMalloc(&s->zpiv, sizeof(int)));
+    hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, c->stream, s->zpiv, 0x7fffffff);
+    for (int cc = 0; cc < 27; cc++) {
+        const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
+        if (np == 0) continue;
+        if (p == IGAP_F64)
+            hipLaunchKernelGGL(k_mcb_ilu<double>, dim3((unsigned)np), dim3(64), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col, s->mc_dpos,
+                               s->mc_pcoord, c->nb[0], c->nb[1], c->nb[2], c->periodic[0], c->periodic[1], c->periodic[2],
+                               (double *)s->mc_ilu[p], s->zpiv);
+        else
+            hipLaunchKernelGGL(k_mcb_ilu<float>, dim3((unsigned)np), dim3(64), 0, c->stream, r0, np, s->mc_rowptr, s->mc_col, s->mc_dpos,
+                               s->mc_pcoord, c->nb[0], c->nb[1], c->nb[2], c->periodic[0], c->periodic[1], c->periodic[2],
+                               (float *)s->mc_ilu[p], s->zpiv);
+    }
+    HIPCHK(hipGetLastError());
+    int zp = 0;
+    HIPCHK(hipMemcpyAsync(&zp, s->zpiv, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (zp != 0x7fffffff) return fail(IGAP_ERR_STATE, "ILU(0), coloured order: zero pivot in row %d", zp);
+    return IGAP_OK;
+}
+
+// ILU(0) of the tangent in the coloured order, precision p; impl 0: own
+// point-block factorisation where the pattern allows it, 1: rocSPARSE
+static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p, int impl = 0)
 {
     int rc = mc_build(c, s);
     if (rc) return rc;
@@ -543,6 +708,7 @@ static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p)
         hipLaunchKernelGGL((k_gather<float, float>), grid1(nnz), dim3(256), 0, c->stream, (int64_t)nnz, s->mc_src, c->d_val32,
                            (float *)s->mc_ilu[p]);
     HIPCHK(hipGetLastError());
+    if (impl == 0 && s->mc_block_ok) return mc_factor_own(c, s, p);
     if (!s->mc_analysed[p]) {
         SPCHK(rocsparse_create_mat_info(&s->mc_info[p]));
         size_t b0 = 0, b1 = 0, b2 = 0;
@@ -1008,5 +1174,49 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     HIPCHK(hipMemcpyAsync(c->d_R, s->F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     st->t_total_ms = ms(T0, clk::now());
+    return IGAP_OK;
+}
+
+// Self test of the point-block ILU(0): factor the context's current tangent
+// values (precision prec) in the coloured order with k_mcb_ilu and with
+// rocSPARSE csrilu0; *diff = max |a - b| / max |b| over the factor entries.
+// *applicable = 0 (and *diff = 0) where the pattern has no point-block
+// colour structure (then the solver uses rocSPARSE anyway).
+extern "C" int igap_gpu_selftest_ilu(igap_gpu_ctx *c, int prec, int *applicable, double *diff)
+{
+    if (!c || !applicable || !diff) return fail(IGAP_ERR_ARG, "null argument");
+    if (prec != IGAP_F64 && prec != IGAP_F32) return fail(IGAP_ERR_ARG, "prec must be IGAP_F64 or IGAP_F32");
+    HIPCHK(hipSetDevice(c->device));
+    if (prec == IGAP_F64 ? !c->d_val64 : !c->d_val32) return fail(IGAP_ERR_STATE, "no tangent of this precision assembled");
+    Solver *s = nullptr;
+    int rc = solver_get(c, c->solver ? c->solver->m : 200, &s);
+    if (rc) return rc;
+    rc = mc_build(c, s);
+    if (rc) return rc;
+    *applicable = s->mc_block_ok;
+    *diff = 0.0;
+    if (!s->mc_block_ok) return IGAP_OK;
+    const int64_t nnz = c->nnz;
+    std::vector<double> a(nnz), b(nnz);
+    for (int impl = 0; impl < 2; impl++) {
+        rc = pc_ilu_setup_mc(c, s, prec, impl);
+        if (rc) return rc;
+        std::vector<double> &dst = impl == 0 ? a : b;
+        if (prec == IGAP_F64) {
+            HIPCHK(hipMemcpyAsync(dst.data(), s->mc_ilu[prec], sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
+            std::vector<float> t(nnz);
+            HIPCHK(hipMemcpyAsync(t.data(), s->mc_ilu[prec], sizeof(float) * nnz, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int64_t j = 0; j < nnz; j++) dst[j] = t[j];
+        }
+    }
+    double md = 0.0, mb = 0.0;
+    for (int64_t j = 0; j < nnz; j++) {
+        md = std::max(md, std::fabs(a[j] - b[j]));
+        mb = std::max(mb, std::fabs(b[j]));
+    }
+    *diff = mb > 0 ? md / mb : md;
     return IGAP_OK;
 }

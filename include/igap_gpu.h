@@ -212,6 +212,16 @@ int igap_gpu_timing_reset(igap_gpu_ctx *ctx);
  * 4 x 64 raw result registers (out[r*64 + l]).  Host pointers. */
 int igap_gpu_selftest_mfma(double *out64, float *out32);
 
+/* Self test of the solver's own ILU(0) factorisation (no reference
+ * counterpart; the reference's factor is PETSc's PCILU): factors the tangent
+ * last assembled in precision prec in the 27-colour order once with the
+ * library's point-block kernels and once with rocSPARSE csrilu0, and returns
+ * *diff = max|a - b| / max|b| over the factor entries.  *applicable = 0 when
+ * the pattern lacks the point-block colour structure (a periodic direction
+ * whose size is not a multiple of 3), in which case the solver always uses
+ * rocSPARSE. */
+int igap_gpu_selftest_ilu(igap_gpu_ctx *ctx, int prec, int *applicable, double *diff);
+
 #ifdef __cplusplus
 }
 #endif
