@@ -21,13 +21,21 @@ def tip_bc(tip_total, s, tip_dof=2):
 
 
 def run_load_steps(A: Assm, nsteps, tip_total=0.05, guess="previous", opts=None, tip_dof=2,
-                   history=None, callback=None, max_steps=None, start=1, prev=None):
+                   history=None, callback=None, max_steps=None, start=1, prev=None, start_rtol=None,
+                   ksp_atol_factor=None):
     """Ramp the tip displacement over ``nsteps`` load steps, solving each to the
     tolerances of ``opts``.  ``guess`` = "previous" (the reference's behaviour),
     "extrapolate" (2 U_k - U_{k-1}) or a callable ``guess(k, prev)`` returning
     the initial iterate from the list of previous solutions.  ``start`` and
     ``prev`` (solutions up to step start-1, last one current) resume a ramp;
-    ``max_steps`` stops after that step.  Returns a list of per-step dicts."""
+    ``max_steps`` stops after that step.
+
+    ``start_rtol``: solve every step to ||R|| <= start_rtol ||R0|| with R0 the
+    residual of the previous solution under the step's load (the first
+    residual of the reference's iteration), whatever the initial iterate, and
+    with ``ksp_atol_factor`` stop each Krylov solve at that fraction of the
+    threshold.  ``wall_ms`` covers the guess (and R0) and the solve.  Returns
+    a list of per-step dicts."""
     out = []
     n = A.ndof
     prev = [A.get_uhist()[:n].copy()] if prev is None else [np.asarray(u)[:n].copy() for u in prev]
@@ -35,6 +43,15 @@ def run_load_steps(A: Assm, nsteps, tip_total=0.05, guess="previous", opts=None,
     for k in range(start, last + 1):
         s = k / nsteps
         A.set_bc(tip_bc(tip_total, s, tip_dof))
+        t0 = time.perf_counter()
+        o = opts
+        if start_rtol is not None:
+            o = type(opts).from_buffer_copy(opts)
+            r0 = float(np.linalg.norm(A.residual(prev[-1])))
+            o.snes_atol = start_rtol * r0
+            o.snes_rtol = 0.0
+            if ksp_atol_factor is not None:
+                o.ksp_atol = ksp_atol_factor * o.snes_atol
         if guess == "extrapolate" and len(prev) >= 2:
             U0 = 2.0 * prev[-1] - prev[-2]
         elif callable(guess):
@@ -42,11 +59,12 @@ def run_load_steps(A: Assm, nsteps, tip_total=0.05, guess="previous", opts=None,
         else:
             U0 = prev[-1]
         A.set_uhist(U0)
-        t0 = time.perf_counter()
-        st = A.snes_solve(opts)
+        st = A.snes_solve(o)
         st["wall_ms"] = (time.perf_counter() - t0) * 1e3
         st["step"] = k
         st["load"] = s
+        if start_rtol is not None:
+            st["r0_start"] = r0
         U = A.get_uhist()[:n].copy()
         prev.append(U)
         if history is not None:

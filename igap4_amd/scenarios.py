@@ -66,11 +66,14 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=
 
 
 def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_order=0, rtol=1e-10, guess="previous",
-         max_steps=None, device=0, verbose=True):
+         max_steps=None, device=0, verbose=True, atol=1e-30, pc_lag=1):
+    """Clamped beam with the tip displacement ramped over ``nsteps`` load
+    steps; each step solved to ||R|| <= max(rtol ||R0||, atol)."""
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
-    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_order=pc_order)
+    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=atol, tangent_prec=prec, pc=pc, pc_order=pc_order,
+                                  pc_lag=pc_lag)
 
     def cb(k, U, st):
         if verbose:
@@ -80,7 +83,8 @@ def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_orde
     steps = run_load_steps(A, nsteps, tip_total=tip, guess=guess, opts=opts, callback=cb, max_steps=max_steps)
     tt = time.perf_counter() - t0
     out = {"scenario": "ramp", "nelem": list(nelem), "ndof": A.ndof, "nnz": A.nnz, "tip": tip, "load_steps": nsteps,
-           "tangent": prec, "pc": pc, "pc_order": pc_order, "rtol": rtol, "guess": guess, "total_s": tt,
+           "tangent": prec, "pc": pc, "pc_order": pc_order, "pc_lag": pc_lag, "rtol": rtol, "atol": atol, "guess": guess,
+           "total_s": tt,
            **_agg(steps)}
     A.close()
     return out
@@ -98,6 +102,8 @@ def main():
     ap.add_argument("--pc-order", type=int, default=0)
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--atol", type=float, default=1e-30)
+    ap.add_argument("--pc-lag", type=int, default=1)
     ap.add_argument("--guess", default="previous")
     a = ap.parse_args()
     if a.which == "transient":
@@ -105,7 +111,8 @@ def main():
         r = transient(nel, a.steps, a.prec or "f32", a.pc or "jacobi", a.pc_order, a.dt, a.rtol, a.guess)
     else:
         nel = tuple(int(x) for x in (a.nelem or "64,16,16").split(","))
-        r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_order, a.rtol, a.guess, a.max_steps)
+        r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_order, a.rtol, a.guess, a.max_steps,
+                 atol=a.atol, pc_lag=a.pc_lag)
     print(json.dumps(r), flush=True)
 
 

@@ -2,17 +2,25 @@
 
 A 3-D convolutional network (include/igap_ws.h: 4 layers of 3x3x3
 convolutions, 16 channels, BF16 matrix-core products with FP32 accumulation,
-trained on the GPU with Adam) predicts a correction to the linear
-extrapolation of the next load-step solution from
+trained on the GPU with Adam) corrects the quadratic extrapolation of the
+next load-step solution.  With the increments dU_j = U_j - U_{j-1} and the
+second difference d2 = dU_{k-1} - dU_{k-2}, its inputs are
 
-  * the last two solution increments  dU_{k-1} = U_{k-1} - U_{k-2},
-    dU_{k-2} = U_{k-2} - U_{k-3}                            (6 channels)
-  * the load increment  dG_k = g_k - g_{k-1}  (Dirichlet data)  (3 channels)
+  * dU_{k-1} / s1,  s1 = rms(dU_{k-1})                      (3 channels)
+  * d2 / s2,        s2 = rms(d2)                            (3 channels)
+  * dG_k / s1, the load increment g_k - g_{k-1} (Dirichlet data) (3 channels)
 
-all divided by s_k = rms(dU_{k-1}).  The target is the second difference
-(U_k - 2 U_{k-1} + U_{k-2}) / s_k, zero on Dirichlet dofs.  The guess
-U_ext + s_k net(x) is used only if its residual norm is below the residual norm
-of the extrapolation U_ext = 2 U_{k-1} - U_{k-2}, which otherwise stays the
+so that the small second difference reaches the BF16 products at full
+relative precision instead of as the difference of two O(1) inputs.  The
+network predicts the third difference (U_k - 3 U_{k-1} + 3 U_{k-2} - U_{k-3})
+/ s2, zero on Dirichlet dofs, and the candidate is
+
+  U_ext + d2 + s2 net(x),   U_ext = 2 U_{k-1} - U_{k-2},
+
+i.e. quadratic extrapolation plus the learned term (a skip connection: the
+library creates the last layer at zero, so an untrained network gives the
+quadratic extrapolation).  The candidate is used only if its residual norm is below the
+residual norm of the linear extrapolation U_ext, which otherwise stays the
 initial iterate; Newton then solves to the same tolerance either way.
 
 IGAP4 itself starts every step from the previous solution
@@ -111,23 +119,28 @@ def selftest_mfma_bf16():
 
 
 # ------------------------------------------------------------------ features
+def _rms(a):
+    r = float(np.sqrt(np.mean(a * a)))
+    return r if r > 0 else 1.0
+
+
 def features(dU1, dU2, dG):
-    """x[v][9] (float32) and the scale s = rms(dU1)."""
-    s = float(np.sqrt(np.mean(dU1 * dU1)))
-    s = s if s > 0 else 1.0
-    x = np.concatenate([dU1.reshape(-1, 3), dU2.reshape(-1, 3), dG.reshape(-1, 3)], axis=1) / s
-    return x.astype(np.float32), s
+    """x[v][9] (float32) and the scales (s1, s2) = (rms(dU1), rms(dU1 - dU2))."""
+    d2 = dU1 - dU2
+    s1, s2 = _rms(dU1), _rms(d2)
+    x = np.concatenate([dU1.reshape(-1, 3) / s1, d2.reshape(-1, 3) / s2, dG.reshape(-1, 3) / s1], axis=1)
+    return x.astype(np.float32), s1, s2
 
 
 def samples_from_trajectory(U, G, mask):
     """Training pairs from solutions U[0..K] and Dirichlet data G[0..K]:
-    (x_k, target_k) for k = 3..K."""
+    (x_k, target_k) for k = 3..K, target = third difference / s2."""
     out = []
     keep = (mask.reshape(-1) == 0)
     for k in range(3, len(U)):
         dU1, dU2 = U[k - 1] - U[k - 2], U[k - 2] - U[k - 3]
-        x, s = features(dU1, dU2, G[k] - G[k - 1])
-        t = ((U[k] - 2 * U[k - 1] + U[k - 2]) * keep) / s
+        x, s1, s2 = features(dU1, dU2, G[k] - G[k - 1])
+        t = ((U[k] - 3 * U[k - 1] + 3 * U[k - 2] - U[k - 3]) * keep) / s2
         out.append((x, t.reshape(-1, 3).astype(np.float32)))
     return out
 
@@ -148,12 +161,54 @@ def train(net, samples, epochs=60, lr=2e-3, seed=0, log=None):
     return hist
 
 
-class LearnedGuess:
-    """``guess(k, prev)`` for run_load_steps: extrapolation, replaced by the
-    network's guess when that has the smaller residual norm."""
+def mean_loss(net, samples):
+    """Mean over samples of the training loss (mean square error)."""
+    return float(np.mean([net.gradients(x, t)[0] for x, t in samples]))
 
-    def __init__(self, A, net, use_net=True):
-        self.A, self.net, self.use_net = A, net, use_net
+
+def train_selected(net, samples, epochs=60, lr=2e-3, seed=0, holdout=0.1, log=None):
+    """Train on the earlier steps, validate on the last ``holdout`` fraction
+    (the steps nearest to the ones the network will be used on), keep the
+    epoch with the lowest validation loss, and switch the correction off
+    (last layer zero: the skip connection alone) when even that epoch does
+    not beat a zero correction on the held-out steps.  Returns a dict."""
+    nv = max(1, int(round(holdout * len(samples)))) if len(samples) > 1 else 0
+    tr, va = samples[:len(samples) - nv], samples[len(samples) - nv:]
+    rng = np.random.default_rng(seed)
+    p0 = net.get_params()
+    zero_val = float(np.mean([np.mean(t.astype(np.float64) ** 2) for _, t in va])) if va else 0.0
+    best, best_p, hist, vhist = np.inf, p0.copy(), [], []
+    for ep in range(epochs):
+        order = rng.permutation(len(tr))
+        hist.append(float(np.mean([net.train_step(*tr[i], lr) for i in order])))
+        v = mean_loss(net, va) if va else hist[-1]
+        vhist.append(v)
+        if v < best:
+            best, best_p = v, net.get_params()
+        if log:
+            log(ep, hist[-1], v)
+    enabled = bool(best < zero_val) if va else True
+    if enabled:
+        net.set_params(best_p)
+    else:
+        p = p0.copy()
+        net.set_params(p)
+    return {"train_loss": hist, "val_loss": vhist, "val_loss_best": best, "val_loss_zero_correction": zero_val,
+            "train_samples": len(tr), "val_samples": len(va), "correction_enabled": enabled}
+
+
+class LearnedGuess:
+    """``guess(k, prev)`` for run_load_steps.  mode "learned": quadratic
+    extrapolation + network, used when its residual norm is below the linear
+    extrapolation's; "quadratic": the same candidate without the network,
+    same acceptance; "linear": the linear extrapolation always."""
+
+    def __init__(self, A, net=None, mode="learned"):
+        if mode not in ("learned", "quadratic", "linear"):
+            raise ValueError(f"unknown mode {mode!r}")
+        if mode == "learned" and net is None:
+            raise ValueError("mode 'learned' needs a network")
+        self.A, self.net, self.mode = A, net, mode
         self.mask = A.dirichlet()[0].reshape(-1)
         self.keep = (self.mask == 0)
         self.g_prev = None
@@ -165,19 +220,18 @@ class LearnedGuess:
     def __call__(self, k, prev):
         g = self.A.dirichlet()[1]
         rec = {"step": k}
-        if len(prev) >= 2:
-            ext = 2.0 * prev[-1] - prev[-2]
-        else:
-            ext = prev[-1].copy()
+        ext = 2.0 * prev[-1] - prev[-2] if len(prev) >= 2 else prev[-1].copy()
         guess = ext
-        if self.use_net and len(prev) >= 3 and self.g_prev is not None:
-            x, s = features(prev[-1] - prev[-2], prev[-2] - prev[-3], g - self.g_prev)
-            c = self.net.forward(x).astype(np.float64).reshape(-1) * s * self.keep
-            cand = ext + c
+        if self.mode != "linear" and len(prev) >= 3 and self.g_prev is not None:
+            dU1, dU2 = prev[-1] - prev[-2], prev[-2] - prev[-3]
+            cand = ext + (dU1 - dU2) * self.keep
+            if self.mode == "learned":
+                x, s1, s2 = features(dU1, dU2, g - self.g_prev)
+                cand = cand + self.net.forward(x).astype(np.float64).reshape(-1) * s2 * self.keep
             r_ext = float(np.linalg.norm(self.A.residual(ext)))
-            r_net = float(np.linalg.norm(self.A.residual(cand)))
-            rec.update(r_ext=r_ext, r_net=r_net, accepted=bool(r_net < r_ext))
-            if r_net < r_ext:
+            r_c = float(np.linalg.norm(self.A.residual(cand)))
+            rec.update(r_ext=r_ext, r_cand=r_c, accepted=bool(r_c < r_ext))
+            if r_c < r_ext:
                 guess = cand
         self.g_prev = g.copy()
         self.log.append(rec)
@@ -185,12 +239,28 @@ class LearnedGuess:
 
 
 # ------------------------------------------------------------------ driver
-def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f64", pc_order=0,
-        seed=0, device=0, verbose=True, atol=1e-10):
+def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
+        seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
+        arms=("plain", "linear", "quadratic", "learned")):
     """Config-4 scenario: steps 1..ntrain generate the training trajectory
-    (extrapolation guesses), the network is trained on it, then steps
-    ntrain+1..nsteps are solved twice from the same state: with the learned
-    guess (accepted against extrapolation) and with extrapolation only."""
+    (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
+    differences the network learns are not dominated by solver tolerance),
+    the network is trained on it, then steps ntrain+1..nsteps are solved once
+    per arm from the same state:
+
+      plain      the previous solution as the initial iterate (the reference's
+                 behaviour), FP64 tangent and factor, the reference's KSP
+                 tolerances: the plain FP64 GPU path
+      linear     linear extrapolation                      } tangent and ILU
+      quadratic  quadratic extrapolation, accepted against  } in ``prec``;
+                 linear extrapolation                       } Krylov stops at
+      learned    quadratic extrapolation + network, same    } ksp_factor x the
+                 acceptance                                 } Newton threshold
+
+    Every step of every arm is solved to ||R|| <= rtol ||R0||, R0 the residual
+    of the previous solution under the step's load (the first residual of the
+    reference's iteration), so a better initial iterate is not asked for more;
+    the per-step wall time includes building the guess."""
     from . import assm as am
     from .loadstep import run_load_steps, tip_bc
 
@@ -200,10 +270,8 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
 
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     grid = mesh.nb
-    # the reference's criterion ||R|| < atol (src/interface.c:220): a relative
-    # test against the first residual of a step would demand more of a better
-    # initial guess
-    opts = am.default_solver_opts(snes_atol=atol, snes_rtol=0.0, tangent_prec=prec, pc_order=pc_order)
+    opts64 = am.default_solver_opts(tangent_prec="f64", pc_order=pc_order)
+    optsx = am.default_solver_opts(tangent_prec=prec, pc_order=pc_order)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
     U, G = [A.get_uhist()[:A.ndof].copy()], [A.dirichlet()[1]]
@@ -214,53 +282,78 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         say(f"  gen step {k}: newton {st['newton_its']} ksp {st['ksp_its']} {st['wall_ms']:.0f} ms")
 
     t0 = time.perf_counter()
-    gen = run_load_steps(A, nsteps, tip_total=tip_total, guess="extrapolate", opts=opts, callback=keep,
-                         max_steps=ntrain)
+    gen = run_load_steps(A, nsteps, tip_total=tip_total, guess="extrapolate", opts=opts64, callback=keep,
+                         max_steps=ntrain, start_rtol=gen_rtol)
     t_gen = time.perf_counter() - t0
+    if len(gen) < ntrain or gen[-1]["converged"] <= 0:
+        A.close()
+        raise RuntimeError(f"training trajectory stopped at load step {len(gen)} of {ntrain} (not converged)")
     mask = A.dirichlet()[0]
     samples = samples_from_trajectory(U, G, mask)
     net = WarmStartNet(grid, seed=seed, device=device)
     t0 = time.perf_counter()
-    hist = train(net, samples, epochs=epochs, lr=lr, seed=seed,
-                 log=lambda ep, l: say(f"  epoch {ep}: loss {l:.3e}") if ep % 10 == 0 else None)
+    tr = train_selected(net, samples, epochs=epochs, lr=lr, seed=seed,
+                        log=lambda ep, l, v: say(f"  epoch {ep}: loss {l:.3e} held-out {v:.3e}")
+                        if ep % 10 == 0 else None)
+    hist = tr["train_loss"]
+    say(f"  held-out loss: best {tr['val_loss_best']:.3e}, zero correction {tr['val_loss_zero_correction']:.3e}; "
+        f"correction {'on' if tr['correction_enabled'] else 'off'}")
     t_train = time.perf_counter() - t0
 
-    results = {}
-    finals = {}
-    for name, use in (("learned", True), ("extrapolation", False)):
+    logs, finals = {}, {}
+    for name in arms:
         A.set_bc(tip_bc(tip_total, ntrain / nsteps))
-        gs = LearnedGuess(A, net, use_net=use)
-        gs.set_previous_load(G[-1])
-        st = run_load_steps(A, nsteps, tip_total=tip_total, guess=gs, opts=opts, start=ntrain + 1, prev=U[-3:])
-        for s, r in zip(st, gs.log):
-            r.update(newton_its=s["newton_its"], ksp_its=s["ksp_its"], wall_ms=s["wall_ms"], converged=s["converged"])
-        results[name] = gs.log
+        if name == "plain":
+            gs, opts, log, kf = "previous", opts64, [], None
+        else:
+            gs = LearnedGuess(A, net, mode=name)
+            gs.set_previous_load(G[-1])
+            opts, log, kf = optsx, gs.log, ksp_factor
+        st = run_load_steps(A, nsteps, tip_total=tip_total, guess=gs, opts=opts, start=ntrain + 1, prev=U[-3:],
+                            start_rtol=rtol, ksp_atol_factor=kf,
+                            callback=lambda k, u, x, nm=name: say(f"  {nm} step {k}: newton {x['newton_its']} "
+                                                                  f"ksp {x['ksp_its']} {x['wall_ms']:.0f} ms"))
+        if name == "plain":
+            log.extend({"step": x["step"]} for x in st)
+        for x, r in zip(st, log):
+            r.update(newton_its=x["newton_its"], ksp_its=x["ksp_its"], wall_ms=x["wall_ms"], converged=x["converged"])
+        logs[name] = log
         finals[name] = A.get_uhist()[:A.ndof].copy()
-        say(f"  {name}: newton {sum(r['newton_its'] for r in gs.log)} wall {sum(r['wall_ms'] for r in gs.log):.0f} ms")
+        say(f"  {name}: newton {sum(r['newton_its'] for r in log)} ksp {sum(r['ksp_its'] for r in log)} "
+            f"wall {sum(r['wall_ms'] for r in log):.0f} ms")
     net.close()
     A.close()
 
     def agg(log):
-        return {"steps": len(log), "newton_its": int(sum(r["newton_its"] for r in log)),
-                "ksp_its": int(sum(r["ksp_its"] for r in log)),
-                "wall_ms_per_step": float(np.mean([r["wall_ms"] for r in log])),
-                "all_converged": bool(all(r["converged"] > 0 for r in log))}
+        out = {"steps": len(log), "newton_its": int(sum(r["newton_its"] for r in log)),
+               "ksp_its": int(sum(r["ksp_its"] for r in log)),
+               "wall_ms_per_step": float(np.mean([r["wall_ms"] for r in log])),
+               "all_converged": bool(all(r["converged"] > 0 for r in log) and len(log) == nsteps - ntrain)}
+        ratios = [r["r_cand"] / r["r_ext"] for r in log if "r_cand" in r and r["r_ext"] > 0]
+        if ratios:
+            out["accepted_fraction"] = float(np.mean([r.get("accepted", False) for r in log]))
+            out["median_residual_ratio_candidate_over_linear"] = float(np.median(ratios))
+        return out
 
-    lr_log = results["learned"]
-    ratios = [r["r_net"] / r["r_ext"] for r in lr_log if "r_net" in r and r["r_ext"] > 0]
+    ref = finals.get("plain", finals[arms[0]])
+    scale = max(1e-300, float(np.max(np.abs(ref))))
     summary = {
         "workload": f"static SVK + gradient beam {nelem[0]}x{nelem[1]}x{nelem[2]}, tip {tip_total} in {nsteps} load "
                     f"steps, train on 1..{ntrain}, evaluate {ntrain + 1}..{nsteps}",
-        "grid": list(grid), "train_samples": len(samples), "epochs": epochs,
-        "loss_first": hist[0], "loss_last": hist[-1],
+        "criterion": f"||R|| <= {rtol:g} ||R(U_prev; load_k)|| per step; arms other than plain: {prec} tangent "
+                     f"and factor, Krylov to {ksp_factor:g} x the Newton threshold",
+        "grid": list(grid), "train_samples": tr["train_samples"], "val_samples": tr["val_samples"], "epochs": epochs,
+        "loss_first": hist[0], "loss_last": hist[-1], "val_loss_best": tr["val_loss_best"],
+        "val_loss_zero_correction": tr["val_loss_zero_correction"], "correction_enabled": tr["correction_enabled"],
         "t_generate_s": t_gen, "t_train_s": t_train,
-        "generate": {"newton_its": int(sum(s["newton_its"] for s in gen))},
-        "learned": agg(lr_log), "extrapolation": agg(results["extrapolation"]),
-        "accepted_fraction": float(np.mean([r.get("accepted", False) for r in lr_log])),
-        "median_residual_ratio_net_over_extrapolation": float(np.median(ratios)) if ratios else None,
-        "final_rel_diff": float(np.max(np.abs(finals["learned"] - finals["extrapolation"]))
-                                / max(1e-300, np.max(np.abs(finals["extrapolation"])))),
+        "generate": {"newton_its": int(sum(x["newton_its"] for x in gen))},
     }
+    for name in arms:
+        summary[name] = agg(logs[name])
+        summary[name]["final_rel_diff_vs_plain"] = float(np.max(np.abs(finals[name] - ref)) / scale)
+    if "plain" in arms:
+        for name in arms:
+            summary[name]["speedup_vs_plain"] = summary["plain"]["wall_ms_per_step"] / summary[name]["wall_ms_per_step"]
     return summary
 
 
@@ -272,11 +365,16 @@ def main():
     ap.add_argument("--tip", type=float, default=0.5)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--prec", default="f64")
-    ap.add_argument("--pc-order", type=int, default=0)
+    ap.add_argument("--prec", default="f32")
+    ap.add_argument("--pc-order", type=int, default=1)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--gen-rtol", type=float, default=1e-12)
+    ap.add_argument("--ksp-factor", type=float, default=0.1)
+    ap.add_argument("--arms", default="plain,linear,quadratic,learned")
     a = ap.parse_args()
     nel = tuple(int(x) for x in a.nelem.split(","))
-    s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order)
+    s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
+            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")))
     print(json.dumps(s), flush=True)
 
 

@@ -117,15 +117,17 @@ def test_training_lowers_the_loss(ws):
 
 def test_warm_start_ablation_small_beam(ws, igap):
     """Config-4 scenario at the config-1 size (16x4x4, a ramp of 60 steps to a
-    large tip displacement): trained on steps 1..30, evaluated on 31..60 with
-    and without the network.  The accepted guesses beat extrapolation by
-    construction; the network must be accepted on most steps and both runs
-    must reach the same converged answer."""
+    large tip displacement): trained on steps 1..30, evaluated on 31..60 from
+    the same state with the previous solution (plain FP64), linear and
+    quadratic extrapolation and the learned guess (FP32 tangent).  Every arm
+    reaches the plain FP64 answer to 1e-8; the learned candidate is accepted
+    against linear extrapolation on most steps, with a smaller residual."""
     s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=80, lr=2e-3, verbose=False)
-    print({k: s[k] for k in ("learned", "extrapolation", "accepted_fraction",
-                             "median_residual_ratio_net_over_extrapolation", "loss_first", "loss_last")})
-    assert s["learned"]["all_converged"] and s["extrapolation"]["all_converged"]
+    for arm in ("plain", "linear", "quadratic", "learned"):
+        print(arm, s[arm])
+        assert s[arm]["all_converged"]
+        assert s[arm]["final_rel_diff_vs_plain"] < 1e-8
+    print({k: s[k] for k in ("loss_first", "loss_last")})
     assert s["loss_last"] < s["loss_first"]
-    assert s["accepted_fraction"] > 0.5
-    assert s["median_residual_ratio_net_over_extrapolation"] < 1.0
-    assert s["final_rel_diff"] < 1e-8
+    assert s["learned"]["accepted_fraction"] > 0.5
+    assert s["learned"]["median_residual_ratio_candidate_over_linear"] < 1.0
