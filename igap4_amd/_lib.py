@@ -48,7 +48,7 @@ class SolverOpts(C.Structure):
                 ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double), ("ksp_maxit", C.c_int),
                 ("ksp_restart", C.c_int), ("pc", C.c_int), ("tangent_prec", C.c_int),
                 ("linesearch", C.c_int), ("pc_order", C.c_int),
-                ("pc_lag", C.c_int)]
+                ("pc_lag", C.c_int), ("monitor", C.c_int), ("mg_levels", C.c_int)]
 
 
 class SolverStats(C.Structure):
@@ -65,6 +65,7 @@ class SolverStats(C.Structure):
 IGAP_PC_ILU0 = 0
 IGAP_PC_JACOBI = 1
 IGAP_PC_NONE = 2
+IGAP_PC_MG = 3
 
 # every symbol include/igap_gpu.h declares, with its ctypes signature
 _SIGS = {

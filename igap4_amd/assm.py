@@ -293,7 +293,8 @@ def default_solver_opts(**kw):
         if k in ("tangent_prec",) and isinstance(v, str):
             v = _lib.IGAP_F64 if v == "f64" else _lib.IGAP_F32
         if k == "pc" and isinstance(v, str):
-            v = {"ilu0": _lib.IGAP_PC_ILU0, "jacobi": _lib.IGAP_PC_JACOBI, "none": _lib.IGAP_PC_NONE}[v]
+            v = {"ilu0": _lib.IGAP_PC_ILU0, "jacobi": _lib.IGAP_PC_JACOBI, "none": _lib.IGAP_PC_NONE,
+                 "mg": _lib.IGAP_PC_MG}[v]
         setattr(o, k, v)
     return o
 

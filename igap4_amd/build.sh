@@ -8,6 +8,6 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
     ${IGAP_EXTRA_FLAGS} \
     "$HERE/csrc/igap_kernels.hip" "$HERE/csrc/igap_solver.hip" "$HERE/csrc/igap_ws.hip" \
     "$HERE/csrc/igap_capi.cpp" "$HERE/csrc/igap_ws_capi.cpp" \
-    -lrocsparse -lrocblas \
+    -lrocsparse -lrocblas -lrocsolver \
     -o "$HERE/libigap_gpu.so.tmp"
 mv "$HERE/libigap_gpu.so.tmp" "$HERE/libigap_gpu.so"

@@ -251,6 +251,7 @@ static void dirichlet_face(const igap_gpu_ctx *c, const double *par, int f, int 
 
 static int setup_bc(igap_gpu_ctx *c, const igap_bc_desc *bc)
 {
+    c->bc_h = *bc;
     const int p = 2;
     std::vector<int32_t> mask(c->ndof, 0);
     std::vector<double> g(c->ndof, 0.0), F(c->ndof, 0.0), vals;
@@ -432,6 +433,9 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     TRY(dalloc(&c->d_X, 3 * c->ncp_active));
     HIPCHK(hipMemcpy(c->d_w, mesh->w, sizeof(double) * c->ncp_active, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_X, mesh->X, sizeof(double) * 3 * c->ncp_active, hipMemcpyHostToDevice));
+    c->w_h.assign(mesh->w, mesh->w + c->ncp_active);
+    c->X_h.assign(mesh->X, mesh->X + 3 * c->ncp_active);
+    c->phys_h = *phys;
     MeshK &M = c->M;
     std::memset(&M, 0, sizeof M);
     for (int d = 0; d < 3; d++) {
@@ -491,7 +495,7 @@ extern "C" int igap_gpu_destroy(igap_gpu_ctx *c)
     if (c->solver) solver_destroy(c->solver);
     for (int i = 0; i < 4; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return IGAP_OK;
 }

@@ -24,10 +24,14 @@ void solver_destroy(Solver *s);
 struct igap_gpu_ctx {
     int device;
     hipStream_t stream;
+    int own_stream = 1;             // 0: stream borrowed from another context (multigrid levels)
     int nelem[3], na[3], nb[3], periodic[3];
     int64_t ncp_active, ncp, ndof, nnz, nel;
     int torder;
     std::vector<double> knot_h[3];
+    std::vector<double> w_h, X_h;   // host copies of the patch (active numbering)
+    igap_phys_desc phys_h;          // descriptors the context was built from
+    igap_bc_desc bc_h;
     igap::PhysK P;
     igap::MeshK M;
     // device

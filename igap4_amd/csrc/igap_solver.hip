@@ -22,10 +22,12 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsparse/rocsparse.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -35,7 +37,11 @@
 
 namespace igap {
 
+struct MGHier;
+void mg_destroy(MGHier *g);
+
 struct Solver {
+    MGHier *mg = nullptr;               // multigrid hierarchy (IGAP_PC_MG), built on first use
     int m = 0;                      // GMRES restart
     int64_t n = 0;
     rocsparse_handle sp = nullptr;
@@ -71,6 +77,7 @@ struct Solver {
 void solver_destroy(Solver *s)
 {
     if (!s) return;
+    if (s->mg) mg_destroy(s->mg);
     if (s->sp) {
         for (int p = 0; p < 2; p++) {
             if (s->info[p]) rocsparse_destroy_mat_info(s->info[p]);
@@ -396,6 +403,133 @@ __global__ __launch_bounds__(64) void k_mcb_ilu(int r0, int npts, const int32_t 
 }
 
 __global__ void k_set_i32(int *p, int v) { *p = v; }
+
+// One direction of a tensor-product transfer between control-point grids:
+// out(i0,i1,i2)[c] = sum_e w[e] in(i0,i1,i2 with the coordinate along dir
+// replaced by ci[e])[c] over the entries e of row (coordinate along dir) of a
+// 1-D operator in CSR form; three components per point, z fastest.
+__global__ void k_apply1d(int dir, int o0, int o1, int o2, int min, const int *__restrict__ rp, const int *__restrict__ ci,
+                          const double *__restrict__ w, const double *__restrict__ in, double *__restrict__ out)
+{
+    const int64_t n = 3LL * o0 * o1 * o2;
+    const int d1 = dir == 1 ? min : o1, d2 = dir == 2 ? min : o2;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += st) {
+        const int c = (int)(t % 3);
+        const int64_t pt = t / 3;
+        const int i2 = (int)(pt % o2), i1 = (int)((pt / o2) % o1), i0 = (int)(pt / ((int64_t)o1 * o2));
+        const int row = dir == 0 ? i0 : dir == 1 ? i1 : i2;
+        double acc = 0.0;
+        for (int e = rp[row]; e < rp[row + 1]; e++) {
+            const int j = ci[e];
+            const int64_t idx = dir == 0 ? ((int64_t)j * d1 + i1) * d2 + i2
+                              : dir == 1 ? ((int64_t)i0 * d1 + j) * d2 + i2
+                                         : ((int64_t)i0 * d1 + i1) * d2 + j;
+            acc += w[e] * in[3 * idx + c];
+        }
+        out[t] = acc;
+    }
+}
+
+// x[i] = 0 where mask[i] != 0 (Dirichlet rows)
+__global__ void k_zero_masked(int64_t n, const int32_t *__restrict__ mask, double *__restrict__ x)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st)
+        if (mask[i]) x[i] = 0.0;
+}
+
+// Inverses of the 3x3 diagonal blocks (one per control point) of a CSR
+// matrix whose rows 3p..3p+2 hold columns 3p..3p+2 (a singular block gets
+// the identity)
+template <typename T>
+__global__ void k_block_diag_inv(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                 const T *__restrict__ val, double *__restrict__ Dinv)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ncp; p += st) {
+        double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int d = 0; d < 3; d++) {
+            const int64_t r = 3 * p + d;
+            int64_t lo = rp[r], hi = rp[r + 1] - 1;   // first column >= 3p (columns sorted)
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (col[mid] < 3 * p) lo = mid + 1;
+                else hi = mid;
+            }
+            for (int64_t j = lo; j < rp[r + 1] && col[j] <= 3 * p + 2; j++)
+                if (col[j] >= 3 * p) a[3 * d + (col[j] - 3 * p)] = (double)val[j];
+        }
+        const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+        const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+        double *o = Dinv + 9 * p;
+        if (det == 0.0 || !(det == det)) {
+            for (int q = 0; q < 9; q++) o[q] = (q % 4 == 0) ? 1.0 : 0.0;
+            continue;
+        }
+        const double id = 1.0 / det;
+        o[0] = c00 * id;
+        o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+        o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+        o[3] = c01 * id;
+        o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+        o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+        o[6] = c02 * id;
+        o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+        o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    }
+}
+
+// d = alpha d + beta Dinv r; x += d   (one Chebyshev step, point-block Jacobi)
+__global__ void k_cheb_step(int64_t ncp, const double *__restrict__ Dinv, const double *__restrict__ r, double *__restrict__ d,
+                            double *__restrict__ x, double alpha, double beta, int add)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ncp; p += st) {
+        const double *m = Dinv + 9 * p;
+        const double r0 = r[3 * p], r1 = r[3 * p + 1], r2 = r[3 * p + 2];
+        const double z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+        const double z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+        const double z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+        double d0 = beta * z0, d1 = beta * z1, d2 = beta * z2;
+        if (alpha != 0.0) {   // (d is not read on the first step)
+            d0 += alpha * d[3 * p];
+            d1 += alpha * d[3 * p + 1];
+            d2 += alpha * d[3 * p + 2];
+        }
+        d[3 * p] = d0;
+        d[3 * p + 1] = d1;
+        d[3 * p + 2] = d2;
+        if (add) {
+            x[3 * p] += d0;
+            x[3 * p + 1] += d1;
+            x[3 * p + 2] += d2;
+        }
+    }
+}
+
+// v[i] in [-1, 1): a fixed integer hash of i (start vector of the power iteration)
+__global__ void k_fill_hash(int64_t n, double *__restrict__ v)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) {
+        uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        v[i] = (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+}
+
+// dense column-major copy of a CSR matrix (n <= a few thousand)
+template <typename T>
+__global__ void k_csr_to_dense(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const T *__restrict__ val,
+                               double *__restrict__ D)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += st)
+        for (int64_t j = rp[r]; j < rp[r + 1]; j++) D[(int64_t)col[j] * n + r] = (double)val[j];
+}
 
 // Partial dot products of the Krylov vectors with w: part[k][ch] =
 // sum over rows of chunk ch of V[k ld + i] w[i]; one workgroup per (chunk,
@@ -815,9 +949,12 @@ static int pc_ilu_setup(igap_gpu_ctx *c, Solver *s, int p)
 }
 
 // y = M^{-1} x (x and y must differ from s->t and s->t2)
+static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const double *b, double *x);
+
 static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int order, const double *x, double *y)
 {
     int64_t n = c->ndof;
+    if (pc == IGAP_PC_MG) return mg_cycle(c, s, p, order, 0, x, y);
     if (pc == IGAP_PC_ILU0 && order == 1) {
         hipLaunchKernelGGL((k_gather<double, double>), grid1(n), dim3(256), 0, c->stream, n, s->mc_perm, x, s->t);
         if (!s->mc_sweep_ok) {   // generic level-scheduled solves of the same factor
@@ -952,6 +1089,527 @@ static int dots(igap_gpu_ctx *c, Solver *s, const double *V, int64_t ld, int nk,
     return IGAP_OK;
 }
 
+// ---------------------------------------------------------------- multigrid
+// Geometric multigrid preconditioner on the B-spline hierarchy of the patch
+// (non-periodic patches with unit weights).  Level l+1 halves every direction
+// of level l whose element count is even and >= 4, by removing every other
+// interior knot, so its quadratic spline space is a subspace of level l's:
+// the prolongation P (coarse control points -> fine) is exact, computed per
+// direction by collocation at the fine Greville abscissae and checked at
+// sample points.  A coarse level is a context of its own (same physics and
+// boundary types) whose geometry and state are the fine ones interpolated at
+// the coarse Greville abscissae (T, exact for fields in the coarse space), so
+// its tangent is re-assembled by the same kernels at every refactorisation.
+// One V-cycle: Chebyshev smoothing of degree MG_CHEB_DEG with point-block
+// (3x3) Jacobi on [0.1, 1.1] lambda_max(D^-1 A) (lambda_max from power
+// iterations at set-up), residual restricted by P^T (zero on coarse
+// Dirichlet rows), coarse correction prolongated by P (zero on fine
+// Dirichlet rows), Chebyshev post-smoothing; the coarsest level is solved by
+// a dense LU (rocSOLVER) when it has at most MG_DENSE_MAX dofs.  A fixed
+// linear operator, so it serves as GMRES's right preconditioner.
+namespace igap {
+constexpr int MG_MAXLEV = 8;
+constexpr int64_t MG_COARSE_NDOF = 4000;
+constexpr int64_t MG_DENSE_MAX = 8000;
+constexpr int MG_CHEB_DEG = 3;
+constexpr int MG_COARSE_DEG = 12;
+constexpr int MG_POWER_ITS = 20;
+
+struct Op1 {   // 1-D operator in CSR form on the device
+    int rows = 0, cols = 0;
+    int *rp = nullptr, *ci = nullptr;
+    double *w = nullptr;
+};
+
+struct MGLevel {
+    igap_gpu_ctx *c = nullptr;   // level 0: the caller's context (not owned)
+    Solver *s = nullptr;         // level 0: the caller's solver (not owned)
+    int halve[3] = {0, 0, 0};    // directions halved towards level l+1
+    Op1 P[3], R[3], T[3];        // to level l+1: prolongation, P^T, interpolation
+    double *r = nullptr, *t = nullptr, *b = nullptr, *x = nullptr;
+    double *Dinv = nullptr;      // 3x3 diagonal-block inverses (smoother)
+    double lmax = 1.0;           // lambda_max(D^-1 A) estimate
+    double *dense = nullptr;     // coarsest level: LU factors (column-major)
+    rocblas_int *ipiv = nullptr, *info = nullptr;
+};
+
+struct MGHier {
+    int nlev = 0;
+    MGLevel lev[MG_MAXLEV];
+    double *tmp1 = nullptr, *tmp2 = nullptr;
+    int bc_type[36];
+};
+
+static void op1_free(Op1 &o)
+{
+    if (o.rp) (void)hipFree(o.rp);
+    if (o.ci) (void)hipFree(o.ci);
+    if (o.w) (void)hipFree(o.w);
+    o = Op1();
+}
+
+void mg_destroy(MGHier *g)
+{
+    if (!g) return;
+    for (int l = 0; l < g->nlev; l++) {
+        MGLevel &L = g->lev[l];
+        for (int d = 0; d < 3; d++) {
+            op1_free(L.P[d]);
+            op1_free(L.R[d]);
+            op1_free(L.T[d]);
+        }
+        double *v[] = {L.r, L.t, L.b, L.x, L.Dinv, L.dense};
+        for (double *q : v)
+            if (q) (void)hipFree(q);
+        if (L.ipiv) (void)hipFree(L.ipiv);
+        if (L.info) (void)hipFree(L.info);
+        if (l > 0 && L.c) igap_gpu_destroy(L.c);
+    }
+    if (g->tmp1) (void)hipFree(g->tmp1);
+    if (g->tmp2) (void)hipFree(g->tmp2);
+    delete g;
+}
+}  // namespace igap
+
+// quadratic B-splines on an open knot vector kv (nfun + 3 knots): the three
+// nonzero values at x (span clamped to the last non-degenerate one, so the
+// right end belongs to it) and the index of the first
+static void bs2_eval(const std::vector<double> &kv, int nfun, double x, int &first, double N[3])
+{
+    int m = 2;
+    while (m < nfun - 1 && !(x < kv[m + 1])) m++;
+    double left[3] = {0, 0, 0}, right[3] = {0, 0, 0};
+    N[0] = 1.0;
+    for (int j = 1; j <= 2; j++) {
+        left[j] = x - kv[m + 1 - j];
+        right[j] = kv[m + j] - x;
+        double saved = 0.0;
+        for (int r = 0; r < j; r++) {
+            const double t = N[r] / (right[r + 1] + left[j - r]);
+            N[r] = saved + right[r + 1] * t;
+            saved = left[j - r] * t;
+        }
+        N[j] = saved;
+    }
+    first = m - 2;
+}
+
+// A X = B for dense row-major A (n x n) and B (n x k), partial pivoting; B is
+// overwritten by X.  Returns false for a singular A.
+static bool dense_solve(int n, int k, std::vector<double> A, std::vector<double> &B)
+{
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        for (int r = col + 1; r < n; r++)
+            if (std::fabs(A[(size_t)r * n + col]) > std::fabs(A[(size_t)piv * n + col])) piv = r;
+        if (A[(size_t)piv * n + col] == 0.0) return false;
+        if (piv != col) {
+            for (int j = 0; j < n; j++) std::swap(A[(size_t)col * n + j], A[(size_t)piv * n + j]);
+            for (int j = 0; j < k; j++) std::swap(B[(size_t)col * k + j], B[(size_t)piv * k + j]);
+        }
+        const double d = A[(size_t)col * n + col];
+        for (int r = col + 1; r < n; r++) {
+            const double f = A[(size_t)r * n + col] / d;
+            if (f == 0.0) continue;
+            for (int j = col; j < n; j++) A[(size_t)r * n + j] -= f * A[(size_t)col * n + j];
+            for (int j = 0; j < k; j++) B[(size_t)r * k + j] -= f * B[(size_t)col * k + j];
+        }
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        for (int j = 0; j < k; j++) {
+            double v = B[(size_t)r * k + j];
+            for (int q = r + 1; q < n; q++) v -= A[(size_t)r * n + q] * B[(size_t)q * k + j];
+            B[(size_t)r * k + j] = v / A[(size_t)r * n + r];
+        }
+    }
+    return true;
+}
+
+// row-major dense (rows x cols) -> device CSR, entries below 1e-15 dropped
+static int op1_upload(const std::vector<double> &M, int rows, int cols, Op1 &o)
+{
+    std::vector<int> rp(rows + 1, 0), ci;
+    std::vector<double> w;
+    for (int r = 0; r < rows; r++) {
+        for (int j = 0; j < cols; j++)
+            if (std::fabs(M[(size_t)r * cols + j]) > 1e-15) {
+                ci.push_back(j);
+                w.push_back(M[(size_t)r * cols + j]);
+            }
+        rp[r + 1] = (int)ci.size();
+    }
+    o.rows = rows;
+    o.cols = cols;
+    HIPCHK(hipMalloc(&o.rp, sizeof(int) * (rows + 1)));
+    HIPCHK(hipMalloc(&o.ci, sizeof(int) * std::max<size_t>(1, ci.size())));
+    HIPCHK(hipMalloc(&o.w, sizeof(double) * std::max<size_t>(1, w.size())));
+    HIPCHK(hipMemcpy(o.rp, rp.data(), sizeof(int) * (rows + 1), hipMemcpyHostToDevice));
+    if (!ci.empty()) {
+        HIPCHK(hipMemcpy(o.ci, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(o.w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    }
+    return IGAP_OK;
+}
+
+// 1-D operators between the fine open knot vector kf (nel elements, even)
+// and its coarsening kc: P (nf x nc), T (nc x nf), nf = nel + 2, nc = nel/2 + 2
+static int mg_1d(const std::vector<double> &kf, int nel, std::vector<double> &kc, std::vector<double> &P, std::vector<double> &T)
+{
+    const int ncel = nel / 2, nf = nel + 2, nc = ncel + 2;
+    kc.assign(ncel + 5, 0.0);
+    for (int i = 0; i <= 2; i++) kc[i] = kf[2];
+    for (int i = 0; i <= ncel; i++) kc[2 + i] = kf[2 + 2 * i];
+    kc[ncel + 3] = kc[ncel + 4] = kf[nel + 2];
+    auto greville = [](const std::vector<double> &kv, int i) { return 0.5 * (kv[i + 1] + kv[i + 2]); };
+    auto row = [](const std::vector<double> &kv, int nfun, double x, std::vector<double> &out, int off, int stride) {
+        int f;
+        double N[3];
+        bs2_eval(kv, nfun, x, f, N);
+        for (int a = 0; a < 3; a++) out[(size_t)off + (size_t)(f + a) * stride] = N[a];
+    };
+    // P = Af^-1 Rc: Af[s][i] = Nf_i(gf_s), Rc[s][j] = Nc_j(gf_s)
+    std::vector<double> Af((size_t)nf * nf, 0.0), Rc((size_t)nf * nc, 0.0);
+    for (int s_ = 0; s_ < nf; s_++) {
+        const double x = greville(kf, s_);
+        row(kf, nf, x, Af, s_ * nf, 1);
+        row(kc, nc, x, Rc, s_ * nc, 1);
+    }
+    if (!dense_solve(nf, nc, Af, Rc)) return fail(IGAP_ERR_STATE, "multigrid: singular fine collocation matrix");
+    P = Rc;
+    // T = Ac^-1 Bf: Ac[s][j] = Nc_j(gc_s), Bf[s][i] = Nf_i(gc_s)
+    std::vector<double> Ac((size_t)nc * nc, 0.0), Bf((size_t)nc * nf, 0.0);
+    for (int s_ = 0; s_ < nc; s_++) {
+        const double x = greville(kc, s_);
+        row(kc, nc, x, Ac, s_ * nc, 1);
+        row(kf, nf, x, Bf, s_ * nf, 1);
+    }
+    if (!dense_solve(nc, nf, Ac, Bf)) return fail(IGAP_ERR_STATE, "multigrid: singular coarse collocation matrix");
+    T = Bf;
+    // check the refinement relation Nc_j = sum_i P[i][j] Nf_i at 5 points per fine span
+    double err = 0.0;
+    for (int e = 0; e < nel; e++)
+        for (int q = 0; q < 5; q++) {
+            const double x = kf[2 + e] + (q + 0.5) / 5.0 * (kf[3 + e] - kf[2 + e]);
+            std::vector<double> nfv(nf, 0.0), ncv(nc, 0.0);
+            row(kf, nf, x, nfv, 0, 1);
+            row(kc, nc, x, ncv, 0, 1);
+            for (int j = 0; j < nc; j++) {
+                double v = 0.0;
+                for (int i = 0; i < nf; i++) v += P[(size_t)i * nc + j] * nfv[i];
+                err = std::max(err, std::fabs(v - ncv[j]));
+            }
+        }
+    if (err > 1e-10) return fail(IGAP_ERR_STATE, "multigrid: refinement relation violated (%.2e)", err);
+    return IGAP_OK;
+}
+
+// host: out = (T0 x T1 x T2) in over a 3-component field on (n0,n1,n2)
+// points, ops given as row-major dense matrices (nullptr = identity)
+static std::vector<double> apply3_host(const std::vector<double> &in, const int nin[3], const std::vector<double> *op[3],
+                                       const int nout[3])
+{
+    std::vector<double> cur = in;
+    int dims[3] = {nin[0], nin[1], nin[2]};
+    for (int d = 0; d < 3; d++) {
+        if (!op[d]) continue;
+        int od[3] = {dims[0], dims[1], dims[2]};
+        od[d] = nout[d];
+        std::vector<double> nxt((size_t)3 * od[0] * od[1] * od[2], 0.0);
+        const std::vector<double> &M = *op[d];
+        for (int i0 = 0; i0 < od[0]; i0++)
+            for (int i1 = 0; i1 < od[1]; i1++)
+                for (int i2 = 0; i2 < od[2]; i2++) {
+                    const int rowi = d == 0 ? i0 : d == 1 ? i1 : i2;
+                    double acc[3] = {0, 0, 0};
+                    for (int j = 0; j < dims[d]; j++) {
+                        const double wv = M[(size_t)rowi * dims[d] + j];
+                        if (wv == 0.0) continue;
+                        const int a0 = d == 0 ? j : i0, a1 = d == 1 ? j : i1, a2 = d == 2 ? j : i2;
+                        const size_t idx = ((size_t)a0 * dims[1] + a1) * dims[2] + a2;
+                        for (int q = 0; q < 3; q++) acc[q] += wv * cur[3 * idx + q];
+                    }
+                    const size_t o = ((size_t)i0 * od[1] + i1) * od[2] + i2;
+                    for (int q = 0; q < 3; q++) nxt[3 * o + q] = acc[q];
+                }
+        cur.swap(nxt);
+        dims[d] = nout[d];
+    }
+    return cur;
+}
+
+// device: out = (op0 x op1 x op2) in on the control-point grid; which:
+// 0 = P (coarse -> fine of level l), 1 = P^T, 2 = T (fine -> coarse)
+static int mg_transfer(igap_gpu_ctx *c, MGHier *g, int l, int which, const double *in, double *out)
+{
+    MGLevel &F = g->lev[l];
+    const igap_gpu_ctx *cf = F.c, *cc = g->lev[l + 1].c;
+    int dims[3];
+    for (int d = 0; d < 3; d++) dims[d] = which == 0 ? cc->nb[d] : cf->nb[d];
+    int dl[3], nd = 0;
+    for (int d = 0; d < 3; d++)
+        if (F.halve[d]) dl[nd++] = d;
+    if (nd == 0) return fail(IGAP_ERR_STATE, "multigrid: level without coarsening");
+    const double *src = in;
+    for (int q = 0; q < nd; q++) {
+        const int d = dl[q];
+        const Op1 &o = which == 0 ? F.P[d] : which == 1 ? F.R[d] : F.T[d];
+        int od[3] = {dims[0], dims[1], dims[2]};
+        od[d] = o.rows;
+        double *dst = q == nd - 1 ? out : (q % 2 == 0 ? g->tmp1 : g->tmp2);
+        const int64_t n = 3LL * od[0] * od[1] * od[2];
+        int64_t blocks = (n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(k_apply1d, dim3((unsigned)blocks), dim3(256), 0, c->stream, d, od[0], od[1], od[2], dims[d], o.rp, o.ci,
+                           o.w, src, dst);
+        src = dst;
+        dims[d] = o.rows;
+    }
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
+static int mg_build(igap_gpu_ctx *c, Solver *s, int maxlev)
+{
+    if (s->mg) return IGAP_OK;
+    for (int d = 0; d < 3; d++)
+        if (c->periodic[d]) return fail(IGAP_ERR_ARG, "IGAP_PC_MG needs a non-periodic patch");
+    for (double wv : c->w_h)
+        if (wv != 1.0) return fail(IGAP_ERR_ARG, "IGAP_PC_MG needs unit weights (a B-spline patch)");
+    if (maxlev <= 0 || maxlev > MG_MAXLEV) maxlev = MG_MAXLEV;
+    MGHier *g = new MGHier();
+    s->mg = g;
+    std::memcpy(g->bc_type, c->bc_h.bc_type, sizeof g->bc_type);
+    g->lev[0].c = c;
+    g->lev[0].s = s;
+    g->nlev = 1;
+    const size_t bytes0 = sizeof(double) * (size_t)c->ndof;
+    HIPCHK(hipMalloc(&g->tmp1, bytes0));
+    HIPCHK(hipMalloc(&g->tmp2, bytes0));
+    while (g->nlev < maxlev) {
+        MGLevel &F = g->lev[g->nlev - 1];
+        igap_gpu_ctx *cf = F.c;
+        if (cf->ndof <= MG_COARSE_NDOF) break;
+        int any = 0;
+        for (int d = 0; d < 3; d++) {
+            F.halve[d] = (cf->nelem[d] % 2 == 0 && cf->nelem[d] >= 4) ? 1 : 0;
+            any |= F.halve[d];
+        }
+        if (!any) break;
+        // coarse knots, 1-D operators, coarse geometry
+        std::vector<double> kc[3], Pd[3], Td[3];
+        int nelc[3], nfd[3], ncd[3];
+        const std::vector<double> *Top[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < 3; d++) {
+            nfd[d] = cf->nb[d];
+            if (F.halve[d]) {
+                int rc = mg_1d(cf->knot_h[d], cf->nelem[d], kc[d], Pd[d], Td[d]);
+                if (rc) return rc;
+                nelc[d] = cf->nelem[d] / 2;
+                Top[d] = &Td[d];
+            } else {
+                kc[d] = cf->knot_h[d];
+                nelc[d] = cf->nelem[d];
+            }
+            ncd[d] = nelc[d] + 2;
+        }
+        std::vector<double> Xc = apply3_host(cf->X_h, nfd, Top, ncd);
+        std::vector<double> wc((size_t)ncd[0] * ncd[1] * ncd[2], 1.0);
+        igap_mesh_desc md;
+        std::memset(&md, 0, sizeof md);
+        md.porder = 2;
+        for (int d = 0; d < 3; d++) {
+            md.nelem[d] = nelc[d];
+            md.periodic[d] = 0;
+            md.knot[d] = kc[d].data();
+        }
+        md.w = wc.data();
+        md.X = Xc.data();
+        MGLevel &C = g->lev[g->nlev];
+        int rc = igap_gpu_create(&md, &c->phys_h, &c->bc_h, c->device, 0, &C.c);
+        if (rc) return rc;
+        // every level works on the fine context's stream (its rocSPARSE and
+        // rocBLAS handles included), so the cycle needs no cross-stream waits
+        HIPCHK(hipStreamSynchronize(C.c->stream));
+        HIPCHK(hipStreamDestroy(C.c->stream));
+        C.c->stream = c->stream;
+        C.c->own_stream = 0;
+        rc = solver_get(C.c, 1, &C.s);
+        if (rc) return rc;
+        for (int d = 0; d < 3; d++) {
+            if (!F.halve[d]) continue;
+            std::vector<double> Rt((size_t)ncd[d] * nfd[d]);
+            for (int i = 0; i < nfd[d]; i++)
+                for (int j = 0; j < ncd[d]; j++) Rt[(size_t)j * nfd[d] + i] = Pd[d][(size_t)i * ncd[d] + j];
+            if ((rc = op1_upload(Pd[d], nfd[d], ncd[d], F.P[d]))) return rc;
+            if ((rc = op1_upload(Rt, ncd[d], nfd[d], F.R[d]))) return rc;
+            if ((rc = op1_upload(Td[d], ncd[d], nfd[d], F.T[d]))) return rc;
+        }
+        g->nlev++;
+    }
+    for (int l = 0; l < g->nlev; l++) {
+        MGLevel &L = g->lev[l];
+        const size_t b = sizeof(double) * (size_t)L.c->ndof;
+        double **v[] = {&L.r, &L.t, &L.b, &L.x};
+        for (double **q : v) HIPCHK(hipMalloc(q, b));
+        HIPCHK(hipMalloc(&L.Dinv, sizeof(double) * 3 * (size_t)L.c->ndof));
+    }
+    MGLevel &Lc = g->lev[g->nlev - 1];
+    if (g->nlev > 1 && Lc.c->ndof <= MG_DENSE_MAX) {
+        const size_t nn = (size_t)Lc.c->ndof;
+        HIPCHK(hipMalloc(&Lc.dense, sizeof(double) * nn * nn));
+        HIPCHK(hipMalloc(&Lc.ipiv, sizeof(rocblas_int) * nn));
+        HIPCHK(hipMalloc(&Lc.info, sizeof(rocblas_int)));
+    }
+    return IGAP_OK;
+}
+
+// x = Chebyshev(b; x) with point-block Jacobi on level l: deg steps from
+// x = 0 (zero_init) or from the given x
+static int mg_cheb(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *b, double *x, int zero_init, int deg)
+{
+    MGLevel &L = g->lev[l];
+    igap_gpu_ctx *cl = L.c;
+    const int64_t n = cl->ndof, ncp = n / 3;
+    const dim3 gn = grid1(n), gp = grid1(ncp), bs(256);
+    const double lo = 0.1 * L.lmax, hi = 1.1 * L.lmax;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    int rc;
+    if (zero_init) {
+        HIPCHK(launch_fill_f64(x, 0.0, n, c->stream));
+        HIPCHK(hipMemcpyAsync(L.r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        if ((rc = spmv(cl, p, x, L.r))) return rc;
+        hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, 1.0, b, -1.0, L.r);
+    }
+    double rho = 1.0 / sigma;
+    hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, 0.0, 1.0 / theta, 1);
+    for (int k = 1; k < deg; k++) {
+        if ((rc = spmv(cl, p, L.t, g->tmp1))) return rc;
+        hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, -1.0, (const double *)g->tmp1, 1.0, L.r);
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, rn * rho,
+                           2.0 * rn / delta, 1);
+        rho = rn;
+    }
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
+// smoother data of level l: block-diagonal inverses and lambda_max(D^-1 A)
+static int mg_smoother_setup(igap_gpu_ctx *c, MGHier *g, int l, int p)
+{
+    MGLevel &L = g->lev[l];
+    igap_gpu_ctx *cl = L.c;
+    const int64_t n = cl->ndof, ncp = n / 3;
+    const dim3 gn = grid1(n), gp = grid1(ncp), bs(256);
+    if (p == IGAP_F64)
+        hipLaunchKernelGGL(k_block_diag_inv<double>, gp, bs, 0, c->stream, ncp, (const int64_t *)cl->d_rowptr, (const int32_t *)cl->d_col,
+                           (const double *)cl->d_val64, L.Dinv);
+    else
+        hipLaunchKernelGGL(k_block_diag_inv<float>, gp, bs, 0, c->stream, ncp, (const int64_t *)cl->d_rowptr, (const int32_t *)cl->d_col,
+                           (const float *)cl->d_val32, L.Dinv);
+    // power iteration on D^-1 A: v <- D^-1 A v / |.|
+    hipLaunchKernelGGL(k_fill_hash, gn, bs, 0, c->stream, n, L.x);
+    double lam = 1.0;
+    for (int it = 0; it < MG_POWER_ITS; it++) {
+        int rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scale_rsqrt, gn, bs, 0, c->stream, n, (const double *)L.s->h, L.x);
+        if ((rc = spmv(cl, p, L.x, L.r))) return rc;
+        hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.x, L.t, 0.0, 1.0, 0);
+        if ((rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h))) return rc;
+        double q = 0;
+        HIPCHK(hipMemcpyAsync(&q, L.s->h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        lam = std::sqrt(q);   // |D^-1 A v| with |v| = 1
+    }
+    HIPCHK(hipGetLastError());
+    L.lmax = lam > 0 ? lam : 1.0;
+    return IGAP_OK;
+}
+
+
+// coarse states and tangents for the current fine state (U_hist of the fine
+// context, level 0 already holding the Newton iterate), smoothers, and the
+// coarsest level's LU
+static int mg_setup(igap_gpu_ctx *c, Solver *s, int p)
+{
+    MGHier *g = s->mg;
+    const bool bc_changed = std::memcmp(g->bc_type, c->bc_h.bc_type, sizeof g->bc_type) != 0;
+    if (bc_changed) std::memcpy(g->bc_type, c->bc_h.bc_type, sizeof g->bc_type);
+    for (int l = 1; l < g->nlev; l++) {
+        igap_gpu_ctx *cf = g->lev[l - 1].c, *cc = g->lev[l].c;
+        if (bc_changed) {
+            int rc = igap_gpu_set_bc(cc, &c->bc_h);
+            if (rc) return rc;
+        }
+        for (int h = 0; h <= cf->torder; h++) {
+            int rc = mg_transfer(c, g, l - 1, 2, cf->d_Uh + (size_t)h * cf->ndof, cc->d_Uh + (size_t)h * cc->ndof);
+            if (rc) return rc;
+        }
+        int rc = igap_gpu_tangent(cc, cc->d_Uh, nullptr, p);
+        if (rc) return rc;
+    }
+    for (int l = 0; l < g->nlev; l++) {
+        MGLevel &L = g->lev[l];
+        if (L.dense) {
+            const int64_t n = L.c->ndof;
+            HIPCHK(launch_fill_f64(L.dense, 0.0, n * n, c->stream));
+            if (p == IGAP_F64)
+                hipLaunchKernelGGL(k_csr_to_dense<double>, grid1(n), dim3(256), 0, c->stream, n, (const int64_t *)L.c->d_rowptr,
+                                   (const int32_t *)L.c->d_col, (const double *)L.c->d_val64, L.dense);
+            else
+                hipLaunchKernelGGL(k_csr_to_dense<float>, grid1(n), dim3(256), 0, c->stream, n, (const int64_t *)L.c->d_rowptr,
+                                   (const int32_t *)L.c->d_col, (const float *)L.c->d_val32, L.dense);
+            BLCHK(rocsolver_dgetrf(L.s->bl, (rocblas_int)n, (rocblas_int)n, L.dense, (rocblas_int)n, L.ipiv, L.info));
+            rocblas_int info = 0;
+            HIPCHK(hipMemcpyAsync(&info, L.info, sizeof info, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (info != 0) return fail(IGAP_ERR_STATE, "multigrid: coarsest matrix singular (getrf info %d)", (int)info);
+        } else {
+            int rc = mg_smoother_setup(c, g, l, p);
+            if (rc) return rc;
+        }
+    }
+    return IGAP_OK;
+}
+
+// x = V-cycle(b) from level l (x and b distinct from the level's work vectors)
+static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const double *b, double *x)
+{
+    (void)order;
+    MGHier *g = s->mg;
+    MGLevel &L = g->lev[l];
+    igap_gpu_ctx *cl = L.c;
+    const int64_t n = cl->ndof;
+    const dim3 gn = grid1(n), bs(256);
+    int rc;
+    if (l == g->nlev - 1) {
+        if (L.dense) {
+            HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+            BLCHK(rocsolver_dgetrs(L.s->bl, rocblas_operation_none, (rocblas_int)n, 1, L.dense, (rocblas_int)n, L.ipiv, x,
+                                   (rocblas_int)n));
+            return IGAP_OK;
+        }
+        return mg_cheb(c, g, l, p, b, x, 1, MG_COARSE_DEG);
+    }
+    MGLevel &C = g->lev[l + 1];
+    if ((rc = mg_cheb(c, g, l, p, b, x, 1, MG_CHEB_DEG))) return rc;
+    // r = b - A x, restricted (zero on coarse Dirichlet rows)
+    if ((rc = spmv(cl, p, x, L.r))) return rc;
+    hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, 1.0, b, -1.0, L.r);
+    if ((rc = mg_transfer(c, g, l, 1, L.r, C.b))) return rc;
+    hipLaunchKernelGGL(k_zero_masked, grid1(C.c->ndof), bs, 0, c->stream, C.c->ndof, C.c->d_mask, C.b);
+    if ((rc = mg_cycle(c, s, p, order, l + 1, C.b, C.x))) return rc;
+    // x += P x_c (not on Dirichlet rows), then post-smoothing
+    if ((rc = mg_transfer(c, g, l, 0, C.x, L.t))) return rc;
+    hipLaunchKernelGGL(k_zero_masked, gn, bs, 0, c->stream, n, cl->d_mask, L.t);
+    hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, 1.0, L.t, 1.0, x);
+    if ((rc = mg_cheb(c, g, l, p, b, x, 0, MG_CHEB_DEG))) return rc;
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
 // GMRES(m), right preconditioned: solve A d = b (d starts at 0)
 static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres)
 {
@@ -1073,6 +1731,7 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     if (o->tangent_prec != IGAP_F64 && o->tangent_prec != IGAP_F32) return fail(IGAP_ERR_ARG, "tangent_prec");
     if (o->pc_order != 0 && o->pc_order != 1) return fail(IGAP_ERR_ARG, "pc_order must be 0 (natural) or 1 (coloured)");
     if (o->pc_lag < 0) return fail(IGAP_ERR_ARG, "pc_lag must be >= 0");
+    if (o->pc < IGAP_PC_ILU0 || o->pc > IGAP_PC_MG) return fail(IGAP_ERR_ARG, "pc %d unknown", o->pc);
     HIPCHK(hipSetDevice(c->device));
     std::memset(st, 0, sizeof *st);
     auto T0 = clk::now();
@@ -1113,6 +1772,9 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         if (o->pc == IGAP_PC_ILU0 && refactor) {
             rc = o->pc_order == 1 ? pc_ilu_setup_mc(c, s, p) : pc_ilu_setup(c, s, p);
             if (rc) return rc;
+        } else if (o->pc == IGAP_PC_MG && refactor) {
+            if ((rc = mg_build(c, s, o->mg_levels))) return rc;
+            if ((rc = mg_setup(c, s, p))) return rc;
         } else if (o->pc == IGAP_PC_JACOBI && refactor) {
             if (p == IGAP_F64) hipLaunchKernelGGL(k_diag_inv<double>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, s->diag);
             else hipLaunchKernelGGL(k_diag_inv<float>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val32, s->diag);
@@ -1169,6 +1831,11 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         fnorm = fn_trial;
         st->t_residual_ms += ms(t, clk::now());
         st->newton_its++;
+        if (o->monitor) {
+            std::fprintf(stderr, "  newton %d: |F| %.3e  ksp %d (relres %.1e)  lambda %.3g  %.0f ms\n", st->newton_its, fnorm,
+                         kits, relres, lambda, ms(T0, clk::now()));
+            std::fflush(stderr);
+        }
     }
     HIPCHK(hipMemcpyAsync(c->d_Uh, s->x, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_R, s->F, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));

@@ -270,8 +270,9 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
 
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     grid = mesh.nb
-    opts64 = am.default_solver_opts(tangent_prec="f64", pc_order=pc_order)
-    optsx = am.default_solver_opts(tangent_prec=prec, pc_order=pc_order)
+    mon = 1 if verbose else 0
+    opts64 = am.default_solver_opts(tangent_prec="f64", pc_order=pc_order, monitor=mon)
+    optsx = am.default_solver_opts(tangent_prec=prec, pc_order=pc_order, monitor=mon)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
     U, G = [A.get_uhist()[:A.ndof].copy()], [A.dirichlet()[1]]

@@ -154,7 +154,12 @@ int igap_gpu_tangent(igap_gpu_ctx *ctx, const double *U, void *vals, int prec);
 /* ---- Newton-Krylov solve (IGAPSolvInit / IGAPSolvSolve, src/interface.c:
  * 207-250, 302-328: SNES NEWTONLS with BT line search, KSP GMRES right
  * preconditioned, PC ILU(0)) with every vector in HBM. ---------------------- */
-enum { IGAP_PC_ILU0 = 0, IGAP_PC_JACOBI = 1, IGAP_PC_NONE = 2 };
+/* IGAP_PC_MG (no counterpart in the reference, which uses ASM/ILU(0)):
+ * geometric multigrid V-cycle on the patch's B-spline hierarchy (every even
+ * direction halved per level; exact knot-insertion prolongation; coarse
+ * tangents re-assembled at the interpolated state; ILU(0) smoothing in the
+ * pc_order ordering).  Non-periodic patches with unit weights only. */
+enum { IGAP_PC_ILU0 = 0, IGAP_PC_JACOBI = 1, IGAP_PC_NONE = 2, IGAP_PC_MG = 3 };
 
 typedef struct {
     double snes_atol;     /* stop when ||R|| <= snes_atol (reference: 1e-10, interface.c:220) */
@@ -174,6 +179,8 @@ typedef struct {
     int    pc_lag;        /* refactor the preconditioner every pc_lag-th Newton iteration  */
                           /* of a solve (1, the reference: every iteration; k > 1 keeps    */
                           /* the factor of an earlier tangent, SNESSetLagPreconditioner)   */
+    int    monitor;       /* 1: one line per Newton iteration on stderr (-snes_monitor)    */
+    int    mg_levels;     /* IGAP_PC_MG: at most this many levels (0: up to 8)            */
 } igap_solver_opts;
 
 typedef struct {

@@ -104,3 +104,33 @@ def test_own_ilu_matches_rocsparse(igap, oracle, case):
         return
     assert ok
     assert diff < (1e-4 if prec == "f32" else 1e-12)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_multigrid_pc_same_answer_fewer_krylov(igap, prec):
+    """Geometric multigrid preconditioner (32x8x8 -> 16x4x4 levels, ILU(0)
+    smoothing): Newton reaches the ILU(0)-preconditioned answer with fewer
+    Krylov iterations, to 1e-9 of the displacement scale in FP64 and 1e-7 with
+    the FP32 tangent (both solves stop at ||R|| ~ 1e-12, whose displacement
+    error the soft bending modes amplify)."""
+    m = igap.MeshIGABeam(L=(4, 1, 1), nelem=(32, 8, 8))
+    out = {}
+    for pc in ("ilu0", "mg"):
+        with igap.Assm(m, igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCClampedTip(tip=0.02)) as A:
+            A.set_uhist(np.zeros(A.ndof))
+            st = A.snes_solve(snes_rtol=1e-11, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_order=1)
+            out[pc] = (A.get_uhist()[:A.ndof].copy(), st)
+        print(f"  {pc}: converged {out[pc][1]['converged']} newton {out[pc][1]['newton_its']} "
+              f"ksp {out[pc][1]['ksp_its']} |F| {out[pc][1]['fnorm']:.2e}")
+    assert out["ilu0"][1]["converged"] > 0 and out["mg"][1]["converged"] > 0
+    u0, u1 = out["ilu0"][0], out["mg"][0]
+    assert np.max(np.abs(u1 - u0)) <= (1e-9 if prec == "f64" else 1e-7) * np.max(np.abs(u0))
+    assert out["mg"][1]["ksp_its"] < out["ilu0"][1]["ksp_its"]
+
+
+def test_multigrid_pc_rejects_periodic(igap):
+    m = igap.MeshIGABeam(L=(1, 1, 1), nelem=(6, 6, 6), periodic=(1, 0, 0))
+    with igap.Assm(m, igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCDefault()) as A:
+        A.set_uhist(1e-3 * np.random.default_rng(0).standard_normal(A.ndof))
+        with pytest.raises(Exception, match="non-periodic"):
+            A.snes_solve(pc="mg", pc_order=1)
