@@ -48,7 +48,8 @@ class SolverOpts(C.Structure):
                 ("ksp_rtol", C.c_double), ("ksp_atol", C.c_double), ("ksp_maxit", C.c_int),
                 ("ksp_restart", C.c_int), ("pc", C.c_int), ("tangent_prec", C.c_int),
                 ("linesearch", C.c_int), ("pc_order", C.c_int),
-                ("pc_lag", C.c_int), ("monitor", C.c_int), ("mg_levels", C.c_int)]
+                ("pc_lag", C.c_int), ("monitor", C.c_int), ("mg_levels", C.c_int),
+                ("pc_fp32", C.c_int)]
 
 
 class SolverStats(C.Structure):

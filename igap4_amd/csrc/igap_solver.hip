@@ -1508,22 +1508,21 @@ static int mg_smoother_setup(igap_gpu_ctx *c, MGHier *g, int l, int p)
     else
         hipLaunchKernelGGL(k_block_diag_inv<float>, gp, bs, 0, c->stream, ncp, (const int64_t *)cl->d_rowptr, (const int32_t *)cl->d_col,
                            (const float *)cl->d_val32, L.Dinv);
-    // power iteration on D^-1 A: v <- D^-1 A v / |.|
+    // power iteration on D^-1 A: v <- D^-1 A v / |v|, normalised on the
+    // device; lambda = |D^-1 A v| for the last unit v (one copy back)
     hipLaunchKernelGGL(k_fill_hash, gn, bs, 0, c->stream, n, L.x);
-    double lam = 1.0;
+    int rc;
     for (int it = 0; it < MG_POWER_ITS; it++) {
-        int rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h);
-        if (rc) return rc;
+        if ((rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h))) return rc;
         hipLaunchKernelGGL(k_scale_rsqrt, gn, bs, 0, c->stream, n, (const double *)L.s->h, L.x);
         if ((rc = spmv(cl, p, L.x, L.r))) return rc;
         hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.x, L.t, 0.0, 1.0, 0);
-        if ((rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h))) return rc;
-        double q = 0;
-        HIPCHK(hipMemcpyAsync(&q, L.s->h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        lam = std::sqrt(q);   // |D^-1 A v| with |v| = 1
     }
-    HIPCHK(hipGetLastError());
+    if ((rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h))) return rc;
+    double q = 0;
+    HIPCHK(hipMemcpyAsync(&q, L.s->h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double lam = std::sqrt(q);
     L.lmax = lam > 0 ? lam : 1.0;
     return IGAP_OK;
 }
@@ -1613,6 +1612,7 @@ static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const d
 // GMRES(m), right preconditioned: solve A d = b (d starts at 0)
 static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres)
 {
+    const int pp = o->pc_fp32 ? IGAP_F32 : p;   // preconditioner precision
     const int64_t n = c->ndof;
     const int m = s->m;
     const rocblas_int N = (rocblas_int)n;
@@ -1637,7 +1637,7 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
         int j = 0;
         for (; j < m && total < o->ksp_maxit; j++) {
             double *vj = s->V + (size_t)j * n, *vn = s->V + (size_t)(j + 1) * n;
-            rc = pc_apply(c, s, o->pc, p, o->pc_order, vj, s->z);
+            rc = pc_apply(c, s, o->pc, pp, o->pc_order, vj, s->z);
             if (rc) return rc;
             rc = spmv(c, p, s->z, vn);
             if (rc) return rc;
@@ -1686,7 +1686,7 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
         HIPCHK(hipMemcpyAsync(s->h, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
         const double one = 1.0, zero = 0.0;
         BLCHK(rocblas_dgemv(s->bl, rocblas_operation_none, N, j, &one, s->V, N, s->h, 1, &zero, s->w, 1));
-        rc = pc_apply(c, s, o->pc, p, o->pc_order, s->w, s->z);
+        rc = pc_apply(c, s, o->pc, pp, o->pc_order, s->w, s->z);
         if (rc) return rc;
         hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, 1.0, s->z, 1.0, d);
         if (rnorm <= o->ksp_rtol * bnorm || rnorm <= o->ksp_atol) break;
@@ -1769,12 +1769,20 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         st->t_tangent_ms += ms(t, clk::now());
         t = clk::now();
         const bool refactor = (it % (o->pc_lag > 1 ? o->pc_lag : 1)) == 0;
+        const int pp = o->pc_fp32 ? IGAP_F32 : p;   // preconditioner precision
+        if (refactor && pp != p && (o->pc == IGAP_PC_ILU0 || o->pc == IGAP_PC_MG)) {
+            // FP32 copy of the FP64 tangent for the preconditioner
+            float *v32 = igap_gpu_csr_val32(c);
+            if (!v32) return IGAP_ERR_NOMEM;
+            hipLaunchKernelGGL(k_d2f, grid1(c->nnz), dim3(256), 0, c->stream, c->nnz, (const double *)c->d_val64, v32);
+            HIPCHK(hipGetLastError());
+        }
         if (o->pc == IGAP_PC_ILU0 && refactor) {
-            rc = o->pc_order == 1 ? pc_ilu_setup_mc(c, s, p) : pc_ilu_setup(c, s, p);
+            rc = o->pc_order == 1 ? pc_ilu_setup_mc(c, s, pp) : pc_ilu_setup(c, s, pp);
             if (rc) return rc;
         } else if (o->pc == IGAP_PC_MG && refactor) {
             if ((rc = mg_build(c, s, o->mg_levels))) return rc;
-            if ((rc = mg_setup(c, s, p))) return rc;
+            if ((rc = mg_setup(c, s, pp))) return rc;
         } else if (o->pc == IGAP_PC_JACOBI && refactor) {
             if (p == IGAP_F64) hipLaunchKernelGGL(k_diag_inv<double>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, s->diag);
             else hipLaunchKernelGGL(k_diag_inv<float>, grid1(n), dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val32, s->diag);

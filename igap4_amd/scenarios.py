@@ -66,7 +66,7 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=
 
 
 def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_order=0, rtol=1e-10, guess="previous",
-         max_steps=None, device=0, verbose=True, atol=1e-30, pc_lag=1, snes_maxit=50, mg_levels=0):
+         max_steps=None, device=0, verbose=True, atol=1e-30, pc_lag=1, snes_maxit=50, mg_levels=0, pc_fp32=0):
     """Clamped beam with the tip displacement ramped over ``nsteps`` load
     steps; each step solved to ||R|| <= max(rtol ||R0||, atol)."""
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
@@ -74,7 +74,7 @@ def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_orde
     A.set_uhist(np.zeros(A.ndof))
     opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=atol, tangent_prec=prec, pc=pc, pc_order=pc_order,
                                   pc_lag=pc_lag, monitor=1 if verbose else 0, snes_maxit=snes_maxit,
-                                  mg_levels=mg_levels)
+                                  mg_levels=mg_levels, pc_fp32=pc_fp32)
 
     def cb(k, U, st):
         if verbose:
@@ -84,7 +84,8 @@ def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_orde
     steps = run_load_steps(A, nsteps, tip_total=tip, guess=guess, opts=opts, callback=cb, max_steps=max_steps)
     tt = time.perf_counter() - t0
     out = {"scenario": "ramp", "nelem": list(nelem), "ndof": A.ndof, "nnz": A.nnz, "tip": tip, "load_steps": nsteps,
-           "tangent": prec, "pc": pc, "pc_order": pc_order, "pc_lag": pc_lag, "rtol": rtol, "atol": atol, "guess": guess,
+           "tangent": prec, "pc": pc, "pc_order": pc_order, "pc_lag": pc_lag, "pc_fp32": pc_fp32, "rtol": rtol,
+           "atol": atol, "guess": guess,
            "total_s": tt,
            **_agg(steps)}
     A.close()
@@ -107,6 +108,7 @@ def main():
     ap.add_argument("--pc-lag", type=int, default=1)
     ap.add_argument("--snes-maxit", type=int, default=50)
     ap.add_argument("--mg-levels", type=int, default=0)
+    ap.add_argument("--pc-fp32", type=int, default=0)
     ap.add_argument("--guess", default="previous")
     a = ap.parse_args()
     if a.which == "transient":
@@ -115,7 +117,7 @@ def main():
     else:
         nel = tuple(int(x) for x in (a.nelem or "64,16,16").split(","))
         r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_order, a.rtol, a.guess, a.max_steps,
-                 atol=a.atol, pc_lag=a.pc_lag, snes_maxit=a.snes_maxit, mg_levels=a.mg_levels)
+                 atol=a.atol, pc_lag=a.pc_lag, snes_maxit=a.snes_maxit, mg_levels=a.mg_levels, pc_fp32=a.pc_fp32)
     print(json.dumps(r), flush=True)
 
 

@@ -19,9 +19,12 @@ network predicts the third difference (U_k - 3 U_{k-1} + 3 U_{k-2} - U_{k-3})
 
 i.e. quadratic extrapolation plus the learned term (a skip connection: the
 library creates the last layer at zero, so an untrained network gives the
-quadratic extrapolation).  The candidate is used only if its residual norm is below the
-residual norm of the linear extrapolation U_ext, which otherwise stays the
-initial iterate; Newton then solves to the same tolerance either way.
+quadratic extrapolation).  Training keeps the epoch with the lowest loss on
+the last training steps (held out), and the learned term is used at all
+only if it lowers the residual norm of the quadratic extrapolation on those
+held-out steps.  The candidate is used only if its residual norm is below
+the residual norm of the linear extrapolation U_ext, which otherwise stays
+the initial iterate; Newton then solves to the same tolerance either way.
 
 IGAP4 itself starts every step from the previous solution
 (src/interface.c:252-259,319); extrapolation and the learned correction are
@@ -203,12 +206,13 @@ class LearnedGuess:
     extrapolation's; "quadratic": the same candidate without the network,
     same acceptance; "linear": the linear extrapolation always."""
 
-    def __init__(self, A, net=None, mode="learned"):
+    def __init__(self, A, net=None, mode="learned", use_correction=True):
         if mode not in ("learned", "quadratic", "linear"):
             raise ValueError(f"unknown mode {mode!r}")
         if mode == "learned" and net is None:
             raise ValueError("mode 'learned' needs a network")
         self.A, self.net, self.mode = A, net, mode
+        self.use_correction = use_correction
         self.mask = A.dirichlet()[0].reshape(-1)
         self.keep = (self.mask == 0)
         self.g_prev = None
@@ -225,7 +229,7 @@ class LearnedGuess:
         if self.mode != "linear" and len(prev) >= 3 and self.g_prev is not None:
             dU1, dU2 = prev[-1] - prev[-2], prev[-2] - prev[-3]
             cand = ext + (dU1 - dU2) * self.keep
-            if self.mode == "learned":
+            if self.mode == "learned" and self.use_correction:
                 x, s1, s2 = features(dU1, dU2, g - self.g_prev)
                 cand = cand + self.net.forward(x).astype(np.float64).reshape(-1) * s2 * self.keep
             r_ext = float(np.linalg.norm(self.A.residual(ext)))
@@ -241,7 +245,7 @@ class LearnedGuess:
 # ------------------------------------------------------------------ driver
 def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
         seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
-        arms=("plain", "linear", "quadratic", "learned")):
+        arms=("plain", "linear", "quadratic", "learned"), pc="ilu0"):
     """Config-4 scenario: steps 1..ntrain generate the training trajectory
     (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
     differences the network learns are not dominated by solver tolerance),
@@ -271,8 +275,11 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     grid = mesh.nb
     mon = 1 if verbose else 0
-    opts64 = am.default_solver_opts(tangent_prec="f64", pc_order=pc_order, monitor=mon)
-    optsx = am.default_solver_opts(tangent_prec=prec, pc_order=pc_order, monitor=mon)
+    opts64 = am.default_solver_opts(tangent_prec="f64", pc=pc, pc_order=pc_order, monitor=mon)
+    # prec "f32": FP32 tangent and preconditioner; "mixed": FP64 tangent (Newton
+    # and Krylov operator), FP32 preconditioner
+    optsx = am.default_solver_opts(tangent_prec="f32" if prec == "f32" else "f64", pc=pc, pc_order=pc_order,
+                                   monitor=mon, pc_fp32=1 if prec in ("f32", "mixed") else 0)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
     U, G = [A.get_uhist()[:A.ndof].copy()], [A.dirichlet()[1]]
@@ -299,6 +306,23 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     hist = tr["train_loss"]
     say(f"  held-out loss: best {tr['val_loss_best']:.3e}, zero correction {tr['val_loss_zero_correction']:.3e}; "
         f"correction {'on' if tr['correction_enabled'] else 'off'}")
+    # the acceptance metric on the held-out steps: residual norm of the
+    # quadratic extrapolation with and without the learned correction
+    keepm = (mask.reshape(-1) == 0)
+    ratios_val = []
+    if tr["correction_enabled"]:
+        for i in range(len(samples) - tr["val_samples"], len(samples)):
+            k = i + 3
+            A.set_bc(tip_bc(tip_total, k / nsteps))
+            dU1, dU2 = U[k - 1] - U[k - 2], U[k - 2] - U[k - 3]
+            cq = 2.0 * U[k - 1] - U[k - 2] + (dU1 - dU2) * keepm
+            x, s1, s2 = features(dU1, dU2, G[k] - G[k - 1])
+            cl = cq + net.forward(x).astype(np.float64).reshape(-1) * s2 * keepm
+            rq = float(np.linalg.norm(A.residual(cq)))
+            ratios_val.append(float(np.linalg.norm(A.residual(cl))) / rq if rq > 0 else 1.0)
+    use_corr = bool(tr["correction_enabled"] and ratios_val and np.median(ratios_val) < 1.0)
+    say(f"  held-out residual ratio learned/quadratic: "
+        f"{np.median(ratios_val) if ratios_val else float('nan'):.3g}; correction {'used' if use_corr else 'off'}")
     t_train = time.perf_counter() - t0
 
     logs, finals = {}, {}
@@ -307,7 +331,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         if name == "plain":
             gs, opts, log, kf = "previous", opts64, [], None
         else:
-            gs = LearnedGuess(A, net, mode=name)
+            gs = LearnedGuess(A, net, mode=name, use_correction=use_corr)
             gs.set_previous_load(G[-1])
             opts, log, kf = optsx, gs.log, ksp_factor
         st = run_load_steps(A, nsteps, tip_total=tip_total, guess=gs, opts=opts, start=ntrain + 1, prev=U[-3:],
@@ -341,11 +365,13 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     summary = {
         "workload": f"static SVK + gradient beam {nelem[0]}x{nelem[1]}x{nelem[2]}, tip {tip_total} in {nsteps} load "
                     f"steps, train on 1..{ntrain}, evaluate {ntrain + 1}..{nsteps}",
-        "criterion": f"||R|| <= {rtol:g} ||R(U_prev; load_k)|| per step; arms other than plain: {prec} tangent "
-                     f"and factor, Krylov to {ksp_factor:g} x the Newton threshold",
+        "criterion": f"||R|| <= {rtol:g} ||R(U_prev; load_k)|| per step; preconditioner {pc}; arms other than "
+                     f"plain: {prec} tangent and preconditioner, Krylov to {ksp_factor:g} x the Newton threshold",
         "grid": list(grid), "train_samples": tr["train_samples"], "val_samples": tr["val_samples"], "epochs": epochs,
         "loss_first": hist[0], "loss_last": hist[-1], "val_loss_best": tr["val_loss_best"],
         "val_loss_zero_correction": tr["val_loss_zero_correction"], "correction_enabled": tr["correction_enabled"],
+        "val_residual_ratio_learned_over_quadratic": float(np.median(ratios_val)) if ratios_val else None,
+        "correction_used": use_corr,
         "t_generate_s": t_gen, "t_train_s": t_train,
         "generate": {"newton_its": int(sum(x["newton_its"] for x in gen))},
     }
@@ -366,16 +392,17 @@ def main():
     ap.add_argument("--tip", type=float, default=0.5)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--prec", default="f32")
+    ap.add_argument("--prec", default="f32", choices=["f64", "f32", "mixed"])
     ap.add_argument("--pc-order", type=int, default=1)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--gen-rtol", type=float, default=1e-12)
     ap.add_argument("--ksp-factor", type=float, default=0.1)
     ap.add_argument("--arms", default="plain,linear,quadratic,learned")
+    ap.add_argument("--pc", default="ilu0", choices=["ilu0", "mg"])
     a = ap.parse_args()
     nel = tuple(int(x) for x in a.nelem.split(","))
     s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
-            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")))
+            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc)
     print(json.dumps(s), flush=True)
 
 

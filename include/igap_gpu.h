@@ -181,6 +181,8 @@ typedef struct {
                           /* the factor of an earlier tangent, SNESSetLagPreconditioner)   */
     int    monitor;       /* 1: one line per Newton iteration on stderr (-snes_monitor)    */
     int    mg_levels;     /* IGAP_PC_MG: at most this many levels (0: up to 8)            */
+    int    pc_fp32;       /* 1: preconditioner (ILU(0) factor or multigrid levels) in FP32 */
+                          /* with an FP64 tangent: Newton and GMRES keep the FP64 operator */
 } igap_solver_opts;
 
 typedef struct {
