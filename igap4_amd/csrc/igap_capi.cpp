@@ -488,7 +488,7 @@ extern "C" int igap_gpu_destroy(igap_gpu_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (int d = 0; d < 3; d++) hipFree(c->d_knot[d]);
-    void *bufs[] = {c->d_w, c->d_X, c->d_Uh, c->d_R, c->d_Re, c->d_Ke[0], c->d_Ke[1], c->d_rowptr, c->d_col, c->d_val64,
+    void *bufs[] = {c->d_w, c->d_X, c->d_Uh, c->d_R, c->d_Re, c->d_Ke[0], c->d_Ke[1], c->d_rowptr, c->d_col, c->d_pcol, c->d_val64,
                     c->d_val32, c->d_mask, c->d_g, c->d_F, c->d_Hd, c->d_H};
     for (void *b : bufs)
         if (b) hipFree(b);

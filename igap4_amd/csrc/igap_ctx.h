@@ -41,6 +41,7 @@ struct igap_gpu_ctx {
     void *d_Ke[2] = {nullptr, nullptr};
     int64_t *d_rowptr = nullptr;
     int32_t *d_col = nullptr;
+    int32_t *d_pcol = nullptr;      // neighbour control point of each 3x3 block (nnz/9), built on first use
     double *d_val64 = nullptr;
     float *d_val32 = nullptr;
     int32_t *d_mask = nullptr;

@@ -112,6 +112,53 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
     if (lane == 0) y[row] = s;
 }
 
+// The assembled pattern is point-blocked: the three rows of control point p
+// list the same neighbour points q, each as the columns 3q, 3q+1, 3q+2, and
+// rowptr[3p] = 9 (number of blocks before p).  pcol[rowptr[3p]/9 + k] = q_k.
+__global__ void k_point_cols(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t *__restrict__ pcol)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ncp; p += st) {
+        const int64_t b0 = rp[3 * p], nn = (rp[3 * p + 1] - b0) / 3;
+        for (int64_t k = 0; k < nn; k++) pcol[b0 / 9 + k] = col[b0 + 3 * k] / 3;
+    }
+}
+
+// y = A x on the point-block pattern: one wave per control point, lane k
+// takes neighbour points k and k+64 (one index load, three x loads and the
+// nine values of the block for the three rows)
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_pb(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ pcol,
+                                                 const T *__restrict__ val, const double *__restrict__ x, double *__restrict__ y)
+{
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= ncp) return;
+    const int64_t b0 = rp[3 * p], b1 = rp[3 * p + 1], b2 = rp[3 * p + 2];
+    const int nn = (int)((b1 - b0) / 3);
+    const int32_t *pc = pcol + b0 / 9;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = lane; k < nn; k += 64) {
+        const int64_t q = pc[k];
+        const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
+        const int64_t o = 3 * k;
+        s0 += (double)val[b0 + o] * x0 + (double)val[b0 + o + 1] * x1 + (double)val[b0 + o + 2] * x2;
+        s1 += (double)val[b1 + o] * x0 + (double)val[b1 + o + 1] * x1 + (double)val[b1 + o + 2] * x2;
+        s2 += (double)val[b2 + o] * x0 + (double)val[b2 + o + 1] * x1 + (double)val[b2 + o + 2] * x2;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+        y[3 * p] = s0;
+        y[3 * p + 1] = s1;
+        y[3 * p + 2] = s2;
+    }
+}
+
 // y[i] = x[idx[i]]  (values or vectors into the coloured order)
 // (grid-stride: launched with grid1)
 template <typename T, typename U>
@@ -1064,10 +1111,20 @@ static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int order, const 
 
 static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y)
 {
-    int64_t n = c->ndof;
-    dim3 g((unsigned)((n + 3) / 4));
-    if (p == IGAP_F64) hipLaunchKernelGGL(k_spmv<double>, g, dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val64, x, y);
-    else hipLaunchKernelGGL(k_spmv<float>, g, dim3(256), 0, c->stream, n, c->d_rowptr, c->d_col, c->d_val32, x, y);
+    const int64_t n = c->ndof, ncp = n / 3;
+    if (!c->d_pcol) {
+        if (c->nnz % 9) return fail(IGAP_ERR_STATE, "pattern is not point-blocked (nnz %lld)", (long long)c->nnz);
+        HIPCHK(hipMalloc(&c->d_pcol, sizeof(int32_t) * (size_t)(c->nnz / 9)));
+        hipLaunchKernelGGL(k_point_cols, grid1(ncp), dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr,
+                           (const int32_t *)c->d_col, c->d_pcol);
+    }
+    const dim3 g((unsigned)((ncp + 3) / 4));
+    if (p == IGAP_F64)
+        hipLaunchKernelGGL(k_spmv_pb<double>, g, dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr, (const int32_t *)c->d_pcol,
+                           (const double *)c->d_val64, x, y);
+    else
+        hipLaunchKernelGGL(k_spmv_pb<float>, g, dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr, (const int32_t *)c->d_pcol,
+                           (const float *)c->d_val32, x, y);
     HIPCHK(hipGetLastError());
     return IGAP_OK;
 }
@@ -1104,9 +1161,10 @@ static int dots(igap_gpu_ctx *c, Solver *s, const double *V, int64_t ld, int nk,
 // (3x3) Jacobi on [0.1, 1.1] lambda_max(D^-1 A) (lambda_max from power
 // iterations at set-up), residual restricted by P^T (zero on coarse
 // Dirichlet rows), coarse correction prolongated by P (zero on fine
-// Dirichlet rows), Chebyshev post-smoothing; the coarsest level is solved by
-// a dense LU (rocSOLVER) when it has at most MG_DENSE_MAX dofs.  A fixed
-// linear operator, so it serves as GMRES's right preconditioner.
+// Dirichlet rows), Chebyshev post-smoothing; the coarsest level is solved
+// with its explicit inverse (rocSOLVER LU and inversion at set-up, one GEMV
+// per cycle) when it has at most MG_DENSE_MAX dofs.  A fixed linear
+// operator, so it serves as GMRES's right preconditioner.
 namespace igap {
 constexpr int MG_MAXLEV = 8;
 constexpr int64_t MG_COARSE_NDOF = 4000;
@@ -1565,6 +1623,11 @@ static int mg_setup(igap_gpu_ctx *c, Solver *s, int p)
             HIPCHK(hipMemcpyAsync(&info, L.info, sizeof info, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             if (info != 0) return fail(IGAP_ERR_STATE, "multigrid: coarsest matrix singular (getrf info %d)", (int)info);
+            // explicit inverse: one dense matrix-vector product per cycle
+            BLCHK(rocsolver_dgetri(L.s->bl, (rocblas_int)n, L.dense, (rocblas_int)n, L.ipiv, L.info));
+            HIPCHK(hipMemcpyAsync(&info, L.info, sizeof info, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (info != 0) return fail(IGAP_ERR_STATE, "multigrid: coarsest inverse failed (getri info %d)", (int)info);
         } else {
             int rc = mg_smoother_setup(c, g, l, p);
             if (rc) return rc;
@@ -1585,9 +1648,9 @@ static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const d
     int rc;
     if (l == g->nlev - 1) {
         if (L.dense) {
-            HIPCHK(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-            BLCHK(rocsolver_dgetrs(L.s->bl, rocblas_operation_none, (rocblas_int)n, 1, L.dense, (rocblas_int)n, L.ipiv, x,
-                                   (rocblas_int)n));
+            const double one = 1.0, zero = 0.0;
+            BLCHK(rocblas_dgemv(L.s->bl, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, L.dense, (rocblas_int)n, b, 1,
+                                &zero, x, 1));
             return IGAP_OK;
         }
         return mg_cheb(c, g, l, p, b, x, 1, MG_COARSE_DEG);
