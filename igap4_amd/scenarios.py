@@ -39,7 +39,7 @@ def _agg(steps):
 
 
 def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=0, dt=1e-3, rtol=1e-10,
-              guess="previous", device=0, verbose=True):
+              guess="previous", device=0, verbose=True, pc_fp32=0, pc_lag=1):
     """gradelasttime with the reference parameters except dt; relative
     Newton tolerance rtol per step."""
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
@@ -49,7 +49,8 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=
     A = am.Assm(mesh, ph, am.BCDefault(), device=device)
     u1, u0 = initial_velocity_modes(_owned_X(mesh), (4, 1, 1), dt)
     A.set_uhist(np.concatenate([u1, u1, u0]))
-    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_order=pc_order)
+    opts = am.default_solver_opts(snes_rtol=rtol, snes_atol=1e-30, tangent_prec=prec, pc=pc, pc_order=pc_order,
+                                  pc_fp32=pc_fp32, pc_lag=pc_lag)
 
     def cb(k, H, st):
         if verbose:
@@ -60,7 +61,8 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=
     steps = run_time_steps(A, nsteps, opts=opts, guess=guess, callback=cb)
     tt = time.perf_counter() - t0
     out = {"scenario": "transient", "nelem": list(nelem), "ndof": A.ndof, "nnz": A.nnz, "dt": dt, "tangent": prec,
-           "pc": pc, "pc_order": pc_order, "rtol": rtol, "total_s": tt, **_agg(steps)}
+           "pc": pc, "pc_order": pc_order, "pc_fp32": pc_fp32, "pc_lag": pc_lag, "rtol": rtol, "total_s": tt,
+           **_agg(steps)}
     A.close()
     return out
 
@@ -113,7 +115,8 @@ def main():
     a = ap.parse_args()
     if a.which == "transient":
         nel = tuple(int(x) for x in (a.nelem or "128,32,32").split(","))
-        r = transient(nel, a.steps, a.prec or "f32", a.pc or "jacobi", a.pc_order, a.dt, a.rtol, a.guess)
+        r = transient(nel, a.steps, a.prec or "f32", a.pc or "jacobi", a.pc_order, a.dt, a.rtol, a.guess,
+                      pc_fp32=a.pc_fp32, pc_lag=a.pc_lag)
     else:
         nel = tuple(int(x) for x in (a.nelem or "64,16,16").split(","))
         r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_order, a.rtol, a.guess, a.max_steps,
