@@ -905,6 +905,93 @@ __global__ __launch_bounds__(128) void k_tangent_gather(MeshK M, const TK *__res
     }
 }
 
+// The same sums read row-wise: one workgroup per owned control point a walks
+// the (at most 27) elements containing a in ascending element order; for each
+// it reads a's three rows of K_e (243 contiguous values, one per thread, all
+// issued before the sums) and adds them into an LDS accumulator indexed by the
+// stencil column point, with a barrier between elements.  Every entry gets
+// its contributions in ascending element order, as in k_tangent_gather, so the
+// values are bitwise the same; the loads are whole-row, coalesced segments.
+template <typename TK, typename TV>
+__global__ __launch_bounds__(256) void k_tangent_gather_rows(MeshK M, const TK *__restrict__ Ke, const int64_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ mask, TV *__restrict__ vals)
+{
+    __shared__ double acc[125 * 9];
+    const int64_t a = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nb1 = M.nb[1], nb2 = M.nb[2];
+    const int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+    int Sx[5], Sy[5], Sz[5];
+    const int nsx = stencil1(ax, M.nb[0], M.periodic[0], Sx);
+    const int nsy = stencil1(ay, M.nb[1], M.periodic[1], Sy);
+    const int nsz = stencil1(az, M.nb[2], M.periodic[2], Sz);
+    int ex[3], lax[3], ey[3], lay[3], ez[3], laz[3], dummy[3];
+    const int nx = elem_pairs2(ax, ax, M.nelem[0], M.nb[0], M.periodic[0], ex, lax, dummy);
+    const int ny = elem_pairs2(ay, ay, M.nelem[1], M.nb[1], M.periodic[1], ey, lay, dummy);
+    const int nz = elem_pairs2(az, az, M.nelem[2], M.nb[2], M.periodic[2], ez, laz, dummy);
+    const int nst = nsx * nsy * nsz;
+    for (int i = tid; i < nst * 9; i += 256) acc[i] = 0.0;
+    // this thread's value of a's rows: row 3 la + i, column c = 3 lb + j
+    const int i = tid / 81, c = tid - 81 * (tid / 81), lb = c / 3, j = c - 3 * (c / 3);
+    const int lbx = lb / 9, lby = (lb / 3) % 3, lbz = lb % 3;
+    // stencil slot of the thread's column point in element slot (x, y, z)
+    int sxv[3] = {0, 0, 0}, syv[3] = {0, 0, 0}, szv[3] = {0, 0, 0};
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        if (x < nx) {
+            const int bx = own1(ex[x] + lbx, M.nb[0], M.periodic[0]);
+            for (int k = 0; k < nsx; k++) if (Sx[k] == bx) sxv[x] = k;
+        }
+        if (x < ny) {
+            const int by = own1(ey[x] + lby, M.nb[1], M.periodic[1]);
+            for (int k = 0; k < nsy; k++) if (Sy[k] == by) syv[x] = k;
+        }
+        if (x < nz) {
+            const int bz = own1(ez[x] + lbz, M.nb[2], M.periodic[2]);
+            for (int k = 0; k < nsz; k++) if (Sz[k] == bz) szv[x] = k;
+        }
+    }
+    double v[27];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++)
+#pragma unroll
+            for (int z = 0; z < 3; z++) {
+                double t = 0.0;
+                if (tid < 243 && x < nx && y < ny && z < nz) {
+                    const int64_t e = ((int64_t)ex[x] * M.nelem[1] + ey[y]) * M.nelem[2] + ez[z];
+                    const int la = 9 * lax[x] + 3 * lay[y] + laz[z];
+                    t = (double)Ke[e * 6561 + (3 * la + i) * 81 + c];
+                }
+                v[9 * x + 3 * y + z] = t;
+            }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++)
+#pragma unroll
+            for (int z = 0; z < 3; z++) {
+                if (x < nx && y < ny && z < nz) {
+                    if (tid < 243) {
+                        const int t = (sxv[x] * nsy + syv[y]) * nsz + szv[z];
+                        acc[(3 * t + i) * 3 + j] += v[9 * x + 3 * y + z];
+                    }
+                    __syncthreads();
+                }
+            }
+    // CSR rows 3a+i: positions rowptr + 3t + j, consecutive threads on one row
+    for (int idx = tid; idx < nst * 9; idx += 256) {
+        const int ii = idx / (3 * nst), r = idx - ii * 3 * nst, t = r / 3, jj = r - 3 * t;
+        const int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
+        const int64_t bcp = ((int64_t)M.nb[1] * M.nb[2]) * Sx[sx] + (int64_t)M.nb[2] * Sy[sy] + Sz[sz];
+        const int64_t row = 3 * a + ii, col = 3 * bcp + jj;
+        const double val = (mask[row] || mask[col]) ? (row == col ? 1.0 : 0.0) : acc[(3 * t + ii) * 3 + jj];
+        vals[rowptr[row] + r] = (TV)val;
+    }
+}
+
 __global__ __launch_bounds__(128) void k_csr_cols(MeshK M, const int64_t *__restrict__ rowptr, int32_t *__restrict__ col)
 {
     const int64_t a = blockIdx.x;
@@ -1008,9 +1095,11 @@ hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh,
 hipError_t launch_tangent_gather(const MeshK &M, const void *Ke, int prec, const int64_t *rowptr, const int32_t *mask, void *vals,
                                  int64_t ncp, hipStream_t s)
 {
-    dim3 grid((unsigned)ncp), block(128);
-    if (prec == 0) hipLaunchKernelGGL((k_tangent_gather<double, double>), grid, block, 0, s, M, (const double *)Ke, rowptr, mask, (double *)vals);
-    else hipLaunchKernelGGL((k_tangent_gather<float, float>), grid, block, 0, s, M, (const float *)Ke, rowptr, mask, (float *)vals);
+    dim3 grid((unsigned)ncp), block(256);
+    if (prec == 0)
+        hipLaunchKernelGGL((k_tangent_gather_rows<double, double>), grid, block, 0, s, M, (const double *)Ke, rowptr, mask, (double *)vals);
+    else
+        hipLaunchKernelGGL((k_tangent_gather_rows<float, float>), grid, block, 0, s, M, (const float *)Ke, rowptr, mask, (float *)vals);
     return hipGetLastError();
 }
 
