@@ -6,8 +6,8 @@
 //                     Dirichlet rows R_d = U_d - g_d (:258, :273-497)
 //   k_tangent_elem    per-element tangents K_e[81x81] on the matrix cores
 //                     (volume loop of Tangent_iga, :607-764)
-//   k_tangent_gather  deterministic per-nonzero sum of K_e into the CSR values
-//                     with MatZeroRowsColumns(diag=1) fused (:763, :844)
+//   k_tangent_gather_rows deterministic per-nonzero sum of K_e into the CSR
+//                     values with MatZeroRowsColumns(diag=1) fused (:763, :844)
 //   k_csr_cols        column indices of the CSR pattern
 //
 // Element numbering e = (ex*ny + ey)*nz + ez (the reference's x->y->z loop
@@ -855,63 +855,15 @@ __device__ __forceinline__ int elem_pairs2(int a, int b, int nelem, int nb, int 
     return cnt;
 }
 
-// one workgroup per owned control point a (its three rows); thread t <-> the
-// t-th control point of the row's sorted column stencil
-template <typename TK, typename TV>
-__global__ __launch_bounds__(128) void k_tangent_gather(MeshK M, const TK *__restrict__ Ke, const int64_t *__restrict__ rowptr,
-                                                        const int32_t *__restrict__ mask, TV *__restrict__ vals)
-{
-    const int64_t a = blockIdx.x;
-    int nb1 = M.nb[1], nb2 = M.nb[2];
-    int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
-    int Sx[5], Sy[5], Sz[5];
-    int nsx = stencil1(ax, M.nb[0], M.periodic[0], Sx);
-    int nsy = stencil1(ay, M.nb[1], M.periodic[1], Sy);
-    int nsz = stencil1(az, M.nb[2], M.periodic[2], Sz);
-    int t = threadIdx.x;
-    if (t >= nsx * nsy * nsz) return;
-    int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
-    int bx = Sx[sx], by = Sy[sy], bz = Sz[sz];
-    int ex[3], lax[3], lbx[3], ey[3], lay[3], lby[3], ez[3], laz[3], lbz[3];
-    int nx = elem_pairs2(ax, bx, M.nelem[0], M.nb[0], M.periodic[0], ex, lax, lbx);
-    int ny = elem_pairs2(ay, by, M.nelem[1], M.nb[1], M.periodic[1], ey, lay, lby);
-    int nz = elem_pairs2(az, bz, M.nelem[2], M.nb[2], M.periodic[2], ez, laz, lbz);
-    double s[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) s[i] = 0.0;
-    for (int x = 0; x < nx; x++)
-        for (int y = 0; y < ny; y++)
-            for (int z = 0; z < nz; z++) {
-                int64_t e = ((int64_t)ex[x] * M.nelem[1] + ey[y]) * M.nelem[2] + ez[z];
-                int la = 9 * lax[x] + 3 * lay[y] + laz[z], lb = 9 * lbx[x] + 3 * lby[y] + lbz[z];
-                const TK *k = Ke + e * 6561 + (3 * la) * 81 + 3 * lb;
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++) s[3 * i + j] += (double)k[81 * i + j];
-            }
-    int64_t bcp = ((int64_t)M.nb[1] * M.nb[2]) * bx + (int64_t)M.nb[2] * by + bz;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        int64_t row = 3 * a + i;
-        int64_t pos = rowptr[row] + 3 * t;
-        bool mr = mask[row] != 0;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            int64_t col = 3 * bcp + j;
-            double v = (mr || mask[col]) ? (row == col ? 1.0 : 0.0) : s[3 * i + j];
-            vals[pos + j] = (TV)v;
-        }
-    }
-}
-
-// The same sums read row-wise: one workgroup per owned control point a walks
-// the (at most 27) elements containing a in ascending element order; for each
-// it reads a's three rows of K_e (243 contiguous values, one per thread, all
-// issued before the sums) and adds them into an LDS accumulator indexed by the
-// stencil column point, with a barrier between elements.  Every entry gets
-// its contributions in ascending element order, as in k_tangent_gather, so the
-// values are bitwise the same; the loads are whole-row, coalesced segments.
+// Deterministic per-nonzero sum of the element tangents into the CSR values,
+// with MatZeroRowsColumns(diag = 1) applied (src/assmcore_iga.c:763,844):
+// one workgroup per owned control point a walks the (at most 27) elements
+// containing a in ascending element order; for each it reads a's three rows
+// of K_e (243 contiguous values, one per thread, all issued before the sums)
+// and adds them into an LDS accumulator indexed by the stencil column point,
+// with a barrier between elements.  Every entry gets its contributions in
+// ascending element order (bitwise repeatable); the loads are whole-row,
+// coalesced segments.
 template <typename TK, typename TV>
 __global__ __launch_bounds__(256) void k_tangent_gather_rows(MeshK M, const TK *__restrict__ Ke, const int64_t *__restrict__ rowptr,
                                                              const int32_t *__restrict__ mask, TV *__restrict__ vals)
