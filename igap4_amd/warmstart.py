@@ -19,7 +19,10 @@ network predicts the third difference (U_k - 3 U_{k-1} + 3 U_{k-2} - U_{k-3})
 
 i.e. quadratic extrapolation plus the learned term (a skip connection: the
 library creates the last layer at zero, so an untrained network gives the
-quadratic extrapolation).  Training keeps the epoch with the lowest loss on
+quadratic extrapolation).  The learned term is projected onto the
+next-coarser spline space (CoarseFilter: exact knot-insertion prolongation
+after interpolation at the coarse Greville abscissae), which keeps its smooth
+part and drops the rough error the stiffness would amplify.  Training keeps the epoch with the lowest loss on
 the last training steps (held out), and the learned term is used at all
 only if it lowers the residual norm of the quadratic extrapolation on those
 held-out steps.  The candidate is used only if its residual norm is below
@@ -200,19 +203,87 @@ def train_selected(net, samples, epochs=60, lr=2e-3, seed=0, holdout=0.1, log=No
             "train_samples": len(tr), "val_samples": len(va), "correction_enabled": enabled}
 
 
+# ------------------------------------------------------- coarse-space filter
+def _bs2_rows(kv, nfun, x):
+    """Values of the nfun quadratic B-splines of the open knot vector kv at the
+    points x (rows), the right end belonging to the last span."""
+    x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    out = np.zeros((x.size, nfun))
+    for r, xv in enumerate(x):
+        m = 2
+        while m < nfun - 1 and not (xv < kv[m + 1]):
+            m += 1
+        left, right, N = np.zeros(3), np.zeros(3), np.zeros(3)
+        N[0] = 1.0
+        for j in (1, 2):
+            left[j] = xv - kv[m + 1 - j]
+            right[j] = kv[m + j] - xv
+            saved = 0.0
+            for k in range(j):
+                t = N[k] / (right[k + 1] + left[j - k])
+                N[k] = saved + right[k + 1] * t
+                saved = left[j - k] * t
+            N[j] = saved
+        out[r, m - 2:m + 1] = N
+    return out
+
+
+def coarse_ops_1d(kf, nel):
+    """Prolongation P (nel+2 x nel/2+2, exact knot insertion) and the
+    interpolation T at the coarse Greville abscissae (nel/2+2 x nel+2) for an
+    open quadratic knot vector with an even number nel >= 4 of elements."""
+    kf = np.asarray(kf, dtype=np.float64)
+    nc_el = nel // 2
+    kc = np.concatenate([[kf[2]] * 3, kf[2 + 2 * np.arange(1, nc_el)], [kf[nel + 2]] * 3])
+    nf, nc = nel + 2, nc_el + 2
+    gf = 0.5 * (kf[1:nf + 1] + kf[2:nf + 2])
+    gc = 0.5 * (kc[1:nc + 1] + kc[2:nc + 2])
+    P = np.linalg.solve(_bs2_rows(kf, nf, gf), _bs2_rows(kc, nc, gf))
+    T = np.linalg.solve(_bs2_rows(kc, nc, gc), _bs2_rows(kf, nf, gc))
+    return P, T
+
+
+class CoarseFilter:
+    """c -> P (T c): the projection of a control-point field onto the
+    next-coarser spline space (every even direction halved), which keeps the
+    smooth part of a correction and drops what the stiffness would amplify.
+    Non-periodic patches only (identity otherwise)."""
+
+    def __init__(self, mesh):
+        self.shape = tuple(mesh.nb)
+        self.ops = []
+        for d in range(3):
+            n = mesh.nelem[d]
+            if mesh.periodic[d] or n % 2 or n < 4:
+                self.ops.append(None)
+            else:
+                self.ops.append(coarse_ops_1d(mesh.knots[d], n))
+
+    def __call__(self, c):
+        x = np.asarray(c, dtype=np.float64).reshape(self.shape + (3,))
+        for d, op in enumerate(self.ops):
+            if op is not None:
+                x = np.moveaxis(np.tensordot(op[1], x, axes=(1, d)), 0, d)
+        for d, op in enumerate(self.ops):
+            if op is not None:
+                x = np.moveaxis(np.tensordot(op[0], x, axes=(1, d)), 0, d)
+        return x.reshape(-1)
+
+
 class LearnedGuess:
     """``guess(k, prev)`` for run_load_steps.  mode "learned": quadratic
     extrapolation + network, used when its residual norm is below the linear
     extrapolation's; "quadratic": the same candidate without the network,
     same acceptance; "linear": the linear extrapolation always."""
 
-    def __init__(self, A, net=None, mode="learned", use_correction=True):
+    def __init__(self, A, net=None, mode="learned", use_correction=True, filt=None):
         if mode not in ("learned", "quadratic", "linear"):
             raise ValueError(f"unknown mode {mode!r}")
         if mode == "learned" and net is None:
             raise ValueError("mode 'learned' needs a network")
         self.A, self.net, self.mode = A, net, mode
         self.use_correction = use_correction
+        self.filt = filt
         self.mask = A.dirichlet()[0].reshape(-1)
         self.keep = (self.mask == 0)
         self.g_prev = None
@@ -231,7 +302,10 @@ class LearnedGuess:
             cand = ext + (dU1 - dU2) * self.keep
             if self.mode == "learned" and self.use_correction:
                 x, s1, s2 = features(dU1, dU2, g - self.g_prev)
-                cand = cand + self.net.forward(x).astype(np.float64).reshape(-1) * s2 * self.keep
+                corr = self.net.forward(x).astype(np.float64).reshape(-1)
+                if self.filt is not None:
+                    corr = self.filt(corr)
+                cand = cand + corr * s2 * self.keep
             r_ext = float(np.linalg.norm(self.A.residual(ext)))
             r_c = float(np.linalg.norm(self.A.residual(cand)))
             rec.update(r_ext=r_ext, r_cand=r_c, accepted=bool(r_c < r_ext))
@@ -245,7 +319,7 @@ class LearnedGuess:
 # ------------------------------------------------------------------ driver
 def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
         seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
-        arms=("plain", "linear", "quadratic", "learned"), pc="ilu0"):
+        arms=("plain", "linear", "quadratic", "learned"), pc="ilu0", coarse_filter=True):
     """Config-4 scenario: steps 1..ntrain generate the training trajectory
     (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
     differences the network learns are not dominated by solver tolerance),
@@ -309,6 +383,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     # the acceptance metric on the held-out steps: residual norm of the
     # quadratic extrapolation with and without the learned correction
     keepm = (mask.reshape(-1) == 0)
+    filt = CoarseFilter(mesh) if coarse_filter else None
     ratios_val = []
     if tr["correction_enabled"]:
         for i in range(len(samples) - tr["val_samples"], len(samples)):
@@ -317,7 +392,10 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
             dU1, dU2 = U[k - 1] - U[k - 2], U[k - 2] - U[k - 3]
             cq = 2.0 * U[k - 1] - U[k - 2] + (dU1 - dU2) * keepm
             x, s1, s2 = features(dU1, dU2, G[k] - G[k - 1])
-            cl = cq + net.forward(x).astype(np.float64).reshape(-1) * s2 * keepm
+            corr = net.forward(x).astype(np.float64).reshape(-1)
+            if filt is not None:
+                corr = filt(corr)
+            cl = cq + corr * s2 * keepm
             rq = float(np.linalg.norm(A.residual(cq)))
             ratios_val.append(float(np.linalg.norm(A.residual(cl))) / rq if rq > 0 else 1.0)
     use_corr = bool(tr["correction_enabled"] and ratios_val and np.median(ratios_val) < 1.0)
@@ -331,7 +409,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         if name == "plain":
             gs, opts, log, kf = "previous", opts64, [], None
         else:
-            gs = LearnedGuess(A, net, mode=name, use_correction=use_corr)
+            gs = LearnedGuess(A, net, mode=name, use_correction=use_corr, filt=filt)
             gs.set_previous_load(G[-1])
             opts, log, kf = optsx, gs.log, ksp_factor
         st = run_load_steps(A, nsteps, tip_total=tip_total, guess=gs, opts=opts, start=ntrain + 1, prev=U[-3:],
@@ -371,7 +449,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         "loss_first": hist[0], "loss_last": hist[-1], "val_loss_best": tr["val_loss_best"],
         "val_loss_zero_correction": tr["val_loss_zero_correction"], "correction_enabled": tr["correction_enabled"],
         "val_residual_ratio_learned_over_quadratic": float(np.median(ratios_val)) if ratios_val else None,
-        "correction_used": use_corr,
+        "correction_used": use_corr, "coarse_filter": bool(coarse_filter),
         "t_generate_s": t_gen, "t_train_s": t_train,
         "generate": {"newton_its": int(sum(x["newton_its"] for x in gen))},
     }
@@ -399,10 +477,11 @@ def main():
     ap.add_argument("--ksp-factor", type=float, default=0.1)
     ap.add_argument("--arms", default="plain,linear,quadratic,learned")
     ap.add_argument("--pc", default="ilu0", choices=["ilu0", "mg"])
+    ap.add_argument("--coarse-filter", type=int, default=1)
     a = ap.parse_args()
     nel = tuple(int(x) for x in a.nelem.split(","))
     s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
-            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc)
+            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc, coarse_filter=bool(a.coarse_filter))
     print(json.dumps(s), flush=True)
 
 
