@@ -6,8 +6,9 @@
 //                     Dirichlet rows R_d = U_d - g_d (:258, :273-497)
 //   k_tangent_elem    per-element tangents K_e[81x81] on the matrix cores
 //                     (volume loop of Tangent_iga, :607-764)
-//   k_tangent_gather_rows deterministic per-nonzero sum of K_e into the CSR
-//                     values with MatZeroRowsColumns(diag=1) fused (:763, :844)
+//   k_tangent_gather_rows / k_tangent_gather  deterministic per-nonzero sum of
+//                     K_e into the CSR values with MatZeroRowsColumns(diag=1)
+//                     fused (:763, :844); row-wise for FP64, column-wise FP32
 //   k_csr_cols        column indices of the CSR pattern
 //
 // Element numbering e = (ex*ny + ey)*nz + ez (the reference's x->y->z loop
@@ -855,6 +856,59 @@ __device__ __forceinline__ int elem_pairs2(int a, int b, int nelem, int nb, int 
     return cnt;
 }
 
+// Column-wise variant of the same sums (used for FP32 element matrices, where
+// it is faster): one workgroup per owned control point a, thread t <-> the
+// t-th control point b of the row's column stencil, summing the (at most 27)
+// 3x3 contributions of the elements containing a and b in ascending element
+// order -- the order k_tangent_gather_rows uses, so the values are the same.
+template <typename TK, typename TV>
+__global__ __launch_bounds__(128) void k_tangent_gather(MeshK M, const TK *__restrict__ Ke, const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ mask, TV *__restrict__ vals)
+{
+    const int64_t a = blockIdx.x;
+    int nb1 = M.nb[1], nb2 = M.nb[2];
+    int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+    int Sx[5], Sy[5], Sz[5];
+    int nsx = stencil1(ax, M.nb[0], M.periodic[0], Sx);
+    int nsy = stencil1(ay, M.nb[1], M.periodic[1], Sy);
+    int nsz = stencil1(az, M.nb[2], M.periodic[2], Sz);
+    int t = threadIdx.x;
+    if (t >= nsx * nsy * nsz) return;
+    int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
+    int bx = Sx[sx], by = Sy[sy], bz = Sz[sz];
+    int ex[3], lax[3], lbx[3], ey[3], lay[3], lby[3], ez[3], laz[3], lbz[3];
+    int nx = elem_pairs2(ax, bx, M.nelem[0], M.nb[0], M.periodic[0], ex, lax, lbx);
+    int ny = elem_pairs2(ay, by, M.nelem[1], M.nb[1], M.periodic[1], ey, lay, lby);
+    int nz = elem_pairs2(az, bz, M.nelem[2], M.nb[2], M.periodic[2], ez, laz, lbz);
+    double s[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) s[i] = 0.0;
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                int64_t e = ((int64_t)ex[x] * M.nelem[1] + ey[y]) * M.nelem[2] + ez[z];
+                int la = 9 * lax[x] + 3 * lay[y] + laz[z], lb = 9 * lbx[x] + 3 * lby[y] + lbz[z];
+                const TK *k = Ke + e * 6561 + (3 * la) * 81 + 3 * lb;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) s[3 * i + j] += (double)k[81 * i + j];
+            }
+    int64_t bcp = ((int64_t)M.nb[1] * M.nb[2]) * bx + (int64_t)M.nb[2] * by + bz;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        int64_t row = 3 * a + i;
+        int64_t pos = rowptr[row] + 3 * t;
+        bool mr = mask[row] != 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            int64_t col = 3 * bcp + j;
+            double v = (mr || mask[col]) ? (row == col ? 1.0 : 0.0) : s[3 * i + j];
+            vals[pos + j] = (TV)v;
+        }
+    }
+}
+
 // Deterministic per-nonzero sum of the element tangents into the CSR values,
 // with MatZeroRowsColumns(diag = 1) applied (src/assmcore_iga.c:763,844):
 // one workgroup per owned control point a walks the (at most 27) elements
@@ -1051,7 +1105,7 @@ hipError_t launch_tangent_gather(const MeshK &M, const void *Ke, int prec, const
     if (prec == 0)
         hipLaunchKernelGGL((k_tangent_gather_rows<double, double>), grid, block, 0, s, M, (const double *)Ke, rowptr, mask, (double *)vals);
     else
-        hipLaunchKernelGGL((k_tangent_gather_rows<float, float>), grid, block, 0, s, M, (const float *)Ke, rowptr, mask, (float *)vals);
+        hipLaunchKernelGGL((k_tangent_gather<float, float>), grid, dim3(128), 0, s, M, (const float *)Ke, rowptr, mask, (float *)vals);
     return hipGetLastError();
 }
 
