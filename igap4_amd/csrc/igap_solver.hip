@@ -1167,7 +1167,7 @@ static int dots(igap_gpu_ctx *c, Solver *s, const double *V, int64_t ld, int nk,
 // operator, so it serves as GMRES's right preconditioner.
 namespace igap {
 constexpr int MG_MAXLEV = 8;
-constexpr int64_t MG_COARSE_NDOF = 4000;
+constexpr int64_t MG_COARSE_NDOF = 1500;
 constexpr int64_t MG_DENSE_MAX = 8000;
 constexpr int MG_CHEB_DEG = 3;
 constexpr int MG_COARSE_DEG = 12;
