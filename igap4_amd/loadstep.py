@@ -77,29 +77,49 @@ def run_load_steps(A: Assm, nsteps, tip_total=0.05, guess="previous", opts=None,
     return out
 
 
-def run_time_steps(A: Assm, nsteps, opts=None, guess="previous", callback=None):
+def run_time_steps(A: Assm, nsteps, opts=None, guess="previous", callback=None, start_rtol=None,
+                   ksp_atol_factor=None):
     """Transient steps of gradelasttime: before each step the history levels
     shift (two back <- previous <- last converged) and level 0 gets the initial
     iterate ("previous": the last converged state, the reference's behaviour;
-    "extrapolate": 2 U_n - U_{n-1}).  U_hist must hold [U_n, U_n, U_{n-1}] or
-    any state whose levels 1 and 2 are the last two time levels.  Returns a
-    list of per-step dicts."""
+    "extrapolate": 2 U_n - U_{n-1}; or a callable ``guess(k, prev)`` on the
+    list of converged states, last one current).  U_hist must hold
+    [U_n, U_n, U_{n-1}] or any state whose levels 1 and 2 are the last two time
+    levels.  ``start_rtol`` / ``ksp_atol_factor`` as in run_load_steps (R0 =
+    the residual of the previous state, the reference's first residual);
+    ``wall_ms`` covers the guess and the solve.  Returns a list of per-step
+    dicts."""
     n = A.ndof
     H = A.get_uhist().reshape(-1, n)
     if H.shape[0] < 3:
         raise ValueError("run_time_steps needs a transient context (three history levels)")
-    u1, u0 = H[1].copy(), H[2].copy()
+    prev = [H[2].copy(), H[1].copy()]
     out = []
     for k in range(1, nsteps + 1):
-        if k > 1:
-            u0, u1 = u1, H[0].copy()
-        U = u1.copy() if guess == "previous" else 2.0 * u1 - u0
-        A.set_uhist(np.concatenate([U, u1, u0]))
+        u1, u0 = prev[-1], prev[-2]
         t0 = time.perf_counter()
-        st = A.snes_solve(opts)
+        o = opts
+        if start_rtol is not None:
+            o = type(opts).from_buffer_copy(opts)
+            r0 = float(np.linalg.norm(A.residual(np.concatenate([u1, u1, u0]))))
+            o.snes_atol = start_rtol * r0
+            o.snes_rtol = 0.0
+            if ksp_atol_factor is not None:
+                o.ksp_atol = ksp_atol_factor * o.snes_atol
+        if callable(guess):
+            U = guess(k, prev)
+        elif guess == "previous":
+            U = u1.copy()
+        else:
+            U = 2.0 * u1 - u0
+        A.set_uhist(np.concatenate([U, u1, u0]))
+        st = A.snes_solve(o)
         st["wall_ms"] = (time.perf_counter() - t0) * 1e3
         st["step"] = k
+        if start_rtol is not None:
+            st["r0_start"] = r0
         H = A.get_uhist().reshape(-1, n)
+        prev.append(H[0].copy())
         if callback:
             callback(k, H, st)
         out.append(st)

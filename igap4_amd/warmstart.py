@@ -276,7 +276,7 @@ class LearnedGuess:
     extrapolation's; "quadratic": the same candidate without the network,
     same acceptance; "linear": the linear extrapolation always."""
 
-    def __init__(self, A, net=None, mode="learned", use_correction=True, filt=None):
+    def __init__(self, A, net=None, mode="learned", use_correction=True, filt=None, levels=None):
         if mode not in ("learned", "quadratic", "linear"):
             raise ValueError(f"unknown mode {mode!r}")
         if mode == "learned" and net is None:
@@ -284,6 +284,8 @@ class LearnedGuess:
         self.A, self.net, self.mode = A, net, mode
         self.use_correction = use_correction
         self.filt = filt
+        # U_hist of a candidate state (transient: [c, U_n, U_{n-1}])
+        self.levels = levels or (lambda c, prev: c)
         self.mask = A.dirichlet()[0].reshape(-1)
         self.keep = (self.mask == 0)
         self.g_prev = None
@@ -306,8 +308,8 @@ class LearnedGuess:
                 if self.filt is not None:
                     corr = self.filt(corr)
                 cand = cand + corr * s2 * self.keep
-            r_ext = float(np.linalg.norm(self.A.residual(ext)))
-            r_c = float(np.linalg.norm(self.A.residual(cand)))
+            r_ext = float(np.linalg.norm(self.A.residual(self.levels(ext, prev))))
+            r_c = float(np.linalg.norm(self.A.residual(self.levels(cand, prev))))
             rec.update(r_ext=r_ext, r_cand=r_c, accepted=bool(r_c < r_ext))
             if r_c < r_ext:
                 guess = cand
@@ -462,6 +464,139 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     return summary
 
 
+def run_transient(nelem=(128, 32, 32), nsteps=200, ntrain=100, dt=1e-3, epochs=60, lr=2e-3, prec="f32", pc="jacobi",
+                  seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
+                  arms=("plain", "linear", "quadratic", "learned"), coarse_filter=True):
+    """The same comparison on time steps of gradelasttime (the config-3
+    workload: L = 4:1:1 beam, zero displacement and the 8-mode seeded initial
+    velocity, the reference's parameters with time step dt): steps
+    1..ntrain generate the training trajectory, steps ntrain+1..nsteps are
+    solved once per arm from the same two time levels.  plain = the previous
+    state as initial iterate (the reference's), the reference's KSP
+    tolerances; the other arms stop the Krylov solve at ksp_factor x the
+    Newton threshold.  Tangent and preconditioner as ``prec`` / ``pc`` in every
+    arm.  Features: the last increment, the last second difference, and a zero
+    load channel (no load varies in time)."""
+    from . import assm as am
+    from .loadstep import initial_velocity_modes, run_time_steps
+
+    def say(*a):
+        if verbose:
+            print(*a, flush=True)
+
+    mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
+    par = list(am.PhysGradElastTime().par)
+    par[7] = dt
+    A = am.Assm(mesh, am.PhysGradElastTime(par=par), am.BCDefault(), device=device)
+    n = A.ndof
+    na, nb = mesh.na, mesh.nb
+    Xo = mesh.X.reshape(na[0], na[1], na[2], 3)[:nb[0], :nb[1], :nb[2]].reshape(-1, 3)
+    u1, u0 = initial_velocity_modes(Xo, (4, 1, 1), dt)
+    mon = 1 if verbose else 0
+    opts = am.default_solver_opts(tangent_prec=prec, pc=pc, pc_order=1, monitor=0)
+    A.set_uhist(np.concatenate([u1, u1, u0]))
+    U = [u0.copy(), u1.copy()]
+
+    def keep(k, H, st):
+        U.append(H[0].copy())
+        say(f"  gen step {k}: newton {st['newton_its']} ksp {st['ksp_its']} {st['wall_ms']:.0f} ms")
+
+    t0 = time.perf_counter()
+    gen = run_time_steps(A, ntrain, opts=opts, guess="extrapolate", callback=keep, start_rtol=gen_rtol)
+    t_gen = time.perf_counter() - t0
+    if len(gen) < ntrain or gen[-1]["converged"] <= 0:
+        A.close()
+        raise RuntimeError(f"training trajectory stopped at time step {len(gen)} of {ntrain} (not converged)")
+    mask = A.dirichlet()[0]
+    G = [np.zeros(n) for _ in U]
+    samples = samples_from_trajectory(U, G, mask)
+    net = WarmStartNet(nb, seed=seed, device=device)
+    t0 = time.perf_counter()
+    tr = train_selected(net, samples, epochs=epochs, lr=lr, seed=seed,
+                        log=lambda ep, l, v: say(f"  epoch {ep}: loss {l:.3e} held-out {v:.3e}")
+                        if ep % 10 == 0 else None)
+    keepm = (mask.reshape(-1) == 0)
+    filt = CoarseFilter(mesh) if coarse_filter else None
+    ratios_val = []
+    if tr["correction_enabled"]:
+        for i in range(len(samples) - tr["val_samples"], len(samples)):
+            k = i + 3
+            dU1, dU2 = U[k - 1] - U[k - 2], U[k - 2] - U[k - 3]
+            cq = 2.0 * U[k - 1] - U[k - 2] + (dU1 - dU2) * keepm
+            x, s1, s2 = features(dU1, dU2, np.zeros(n))
+            corr = net.forward(x).astype(np.float64).reshape(-1)
+            if filt is not None:
+                corr = filt(corr)
+            cl = cq + corr * s2 * keepm
+            hist = np.concatenate([U[k - 1], U[k - 2]])
+            rq = float(np.linalg.norm(A.residual(np.concatenate([cq, hist]))))
+            ratios_val.append(float(np.linalg.norm(A.residual(np.concatenate([cl, hist])))) / rq if rq > 0 else 1.0)
+    use_corr = bool(tr["correction_enabled"] and ratios_val and np.median(ratios_val) < 1.0)
+    say(f"  held-out residual ratio learned/quadratic: "
+        f"{np.median(ratios_val) if ratios_val else float('nan'):.3g}; correction {'used' if use_corr else 'off'}")
+    t_train = time.perf_counter() - t0
+
+    def levels(c, prev):
+        return np.concatenate([c, prev[-1], prev[-2]])
+
+    logs, finals = {}, {}
+    for name in arms:
+        A.set_uhist(np.concatenate([U[-1], U[-1], U[-2]]))
+        o = am.default_solver_opts(tangent_prec=prec, pc=pc, pc_order=1, monitor=mon)
+        if name == "plain":
+            gs, log, kf = "previous", [], None
+        else:
+            gs = LearnedGuess(A, net, mode=name, use_correction=use_corr, filt=filt, levels=levels)
+            gs.set_previous_load(np.zeros(n))
+            log, kf = gs.log, ksp_factor
+        st = run_time_steps(A, nsteps - ntrain, opts=o, guess=gs, start_rtol=rtol, ksp_atol_factor=kf,
+                            callback=lambda k, H, x, nm=name: say(f"  {nm} step {ntrain + k}: newton "
+                                                                  f"{x['newton_its']} ksp {x['ksp_its']} "
+                                                                  f"{x['wall_ms']:.0f} ms"))
+        if name == "plain":
+            log.extend({"step": x["step"]} for x in st)
+        for x, r in zip(st, log):
+            r.update(newton_its=x["newton_its"], ksp_its=x["ksp_its"], wall_ms=x["wall_ms"], converged=x["converged"])
+        logs[name] = log
+        finals[name] = A.get_uhist()[:n].copy()
+        say(f"  {name}: newton {sum(r['newton_its'] for r in log)} ksp {sum(r['ksp_its'] for r in log)} "
+            f"wall {sum(r['wall_ms'] for r in log):.0f} ms")
+    net.close()
+    A.close()
+
+    def agg(log):
+        out = {"steps": len(log), "newton_its": int(sum(r["newton_its"] for r in log)),
+               "ksp_its": int(sum(r["ksp_its"] for r in log)),
+               "wall_ms_per_step": float(np.mean([r["wall_ms"] for r in log])),
+               "all_converged": bool(all(r["converged"] > 0 for r in log) and len(log) == nsteps - ntrain)}
+        ratios = [r["r_cand"] / r["r_ext"] for r in log if "r_cand" in r and r["r_ext"] > 0]
+        if ratios:
+            out["accepted_fraction"] = float(np.mean([r.get("accepted", False) for r in log]))
+            out["median_residual_ratio_candidate_over_linear"] = float(np.median(ratios))
+        return out
+
+    ref = finals.get("plain", finals[arms[0]])
+    scale = max(1e-300, float(np.max(np.abs(ref))))
+    summary = {
+        "workload": f"gradelasttime beam {nelem[0]}x{nelem[1]}x{nelem[2]}, dt {dt:g}, 8-mode initial velocity, "
+                    f"{nsteps} time steps, train on 1..{ntrain}, evaluate {ntrain + 1}..{nsteps}",
+        "criterion": f"||R|| <= {rtol:g} ||R(U_n)|| per step; tangent {prec}, preconditioner {pc}; arms other than "
+                     f"plain: Krylov to {ksp_factor:g} x the Newton threshold",
+        "grid": list(nb), "train_samples": tr["train_samples"], "val_samples": tr["val_samples"], "epochs": epochs,
+        "loss_first": tr["train_loss"][0], "loss_last": tr["train_loss"][-1], "val_loss_best": tr["val_loss_best"],
+        "val_loss_zero_correction": tr["val_loss_zero_correction"],
+        "val_residual_ratio_learned_over_quadratic": float(np.median(ratios_val)) if ratios_val else None,
+        "correction_used": use_corr, "t_generate_s": t_gen, "t_train_s": t_train,
+    }
+    for name in arms:
+        summary[name] = agg(logs[name])
+        summary[name]["final_rel_diff_vs_plain"] = float(np.max(np.abs(finals[name] - ref)) / scale)
+    if "plain" in arms:
+        for name in arms:
+            summary[name]["speedup_vs_plain"] = summary["plain"]["wall_ms_per_step"] / summary[name]["wall_ms_per_step"]
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--nelem", default="64,16,16")
@@ -476,10 +611,18 @@ def main():
     ap.add_argument("--gen-rtol", type=float, default=1e-12)
     ap.add_argument("--ksp-factor", type=float, default=0.1)
     ap.add_argument("--arms", default="plain,linear,quadratic,learned")
-    ap.add_argument("--pc", default="ilu0", choices=["ilu0", "mg"])
+    ap.add_argument("--pc", default="ilu0", choices=["ilu0", "mg", "jacobi"])
     ap.add_argument("--coarse-filter", type=int, default=1)
+    ap.add_argument("--transient", action="store_true", help="time steps of gradelasttime instead of load steps")
+    ap.add_argument("--dt", type=float, default=1e-3)
     a = ap.parse_args()
     nel = tuple(int(x) for x in a.nelem.split(","))
+    if a.transient:
+        s = run_transient(nel, a.steps, a.train, a.dt, a.epochs, a.lr, a.prec if a.prec != "mixed" else "f64",
+                          a.pc if a.pc != "ilu0" else "jacobi", rtol=a.rtol, gen_rtol=a.gen_rtol,
+                          ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), coarse_filter=bool(a.coarse_filter))
+        print(json.dumps(s), flush=True)
+        return
     s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
             ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc, coarse_filter=bool(a.coarse_filter))
     print(json.dumps(s), flush=True)
