@@ -322,7 +322,7 @@ class LearnedGuess:
 def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
         seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
         arms=("plain", "linear", "quadratic", "learned"), pc="ilu0", coarse_filter=True):
-    """Config-4 scenario: steps 1..ntrain generate the training trajectory
+    """Load-step comparison: steps 1..ntrain generate the training trajectory
     (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
     differences the network learns are not dominated by solver tolerance),
     the network is trained on it, then steps ntrain+1..nsteps are solved once
@@ -467,7 +467,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
 def run_transient(nelem=(128, 32, 32), nsteps=200, ntrain=100, dt=1e-3, epochs=60, lr=2e-3, prec="f32", pc="jacobi",
                   seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
                   arms=("plain", "linear", "quadratic", "learned"), coarse_filter=True):
-    """The same comparison on time steps of gradelasttime (the config-3
+    """The same comparison on time steps of gradelasttime (the transient
     workload: L = 4:1:1 beam, zero displacement and the 8-mode seeded initial
     velocity, the reference's parameters with time step dt): steps
     1..ntrain generate the training trajectory, steps ntrain+1..nsteps are

@@ -58,7 +58,7 @@ def test_newton_matches_cpu_newton(igap, oracle, problem, prec, order):
 
 
 def test_load_steps_reference_criterion(igap):
-    """Config-1 shape (16x4x4 beam, tip 0.05 in 10 load steps), each load step
+    """16x4x4 beam, tip 0.05 in 10 load steps, each load step
     to the reference's absolute criterion ||R|| < 1e-10."""
     from igap4_amd.loadstep import run_load_steps
     m = igap.MeshIGABeam(L=(4, 1, 1), nelem=(16, 4, 4))

@@ -116,7 +116,7 @@ def test_training_lowers_the_loss(ws):
 
 
 def test_warm_start_ablation_small_beam(ws, igap):
-    """Config-4 scenario at the config-1 size (16x4x4, a ramp of 60 steps to a
+    """The warm-start comparison on a 16x4x4 beam (a ramp of 60 steps to a
     large tip displacement): trained on steps 1..30, evaluated on 31..60 from
     the same state with the previous solution (plain FP64), linear and
     quadratic extrapolation and the learned guess (FP32 tangent).  Every arm
