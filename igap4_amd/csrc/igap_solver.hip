@@ -127,9 +127,11 @@ __global__ void k_point_cols(int64_t ncp, const int64_t *__restrict__ rp, const 
 // y = A x on the point-block pattern: one wave per control point, lane k
 // takes neighbour points k and k+64 (one index load, three x loads and the
 // nine values of the block for the three rows)
-template <typename T>
+// MODE 0: y = A x;  1: y = b - A x;  2: y -= A x
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_spmv_pb(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ pcol,
-                                                 const T *__restrict__ val, const double *__restrict__ x, double *__restrict__ y)
+                                                 const T *__restrict__ val, const double *__restrict__ x, const double *__restrict__ b,
+                                                 double *__restrict__ y)
 {
     const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -153,9 +155,19 @@ __global__ __launch_bounds__(256) void k_spmv_pb(int64_t ncp, const int64_t *__r
         s2 += __shfl_xor(s2, o, 64);
     }
     if (lane == 0) {
-        y[3 * p] = s0;
-        y[3 * p + 1] = s1;
-        y[3 * p + 2] = s2;
+        if (MODE == 0) {
+            y[3 * p] = s0;
+            y[3 * p + 1] = s1;
+            y[3 * p + 2] = s2;
+        } else if (MODE == 1) {
+            y[3 * p] = b[3 * p] - s0;
+            y[3 * p + 1] = b[3 * p + 1] - s1;
+            y[3 * p + 2] = b[3 * p + 2] - s2;
+        } else {
+            y[3 * p] -= s0;
+            y[3 * p + 1] -= s1;
+            y[3 * p + 2] -= s2;
+        }
     }
 }
 
@@ -455,8 +467,11 @@ __global__ void k_set_i32(int *p, int v) { *p = v; }
 // out(i0,i1,i2)[c] = sum_e w[e] in(i0,i1,i2 with the coordinate along dir
 // replaced by ci[e])[c] over the entries e of row (coordinate along dir) of a
 // 1-D operator in CSR form; three components per point, z fastest.
+// Last pass of a transfer: mask (if given) zeroes the Dirichlet rows and
+// accumulate adds into out instead of overwriting it.
 __global__ void k_apply1d(int dir, int o0, int o1, int o2, int min, const int *__restrict__ rp, const int *__restrict__ ci,
-                          const double *__restrict__ w, const double *__restrict__ in, double *__restrict__ out)
+                          const double *__restrict__ w, const double *__restrict__ in, double *__restrict__ out,
+                          const int32_t *__restrict__ mask, int accumulate)
 {
     const int64_t n = 3LL * o0 * o1 * o2;
     const int d1 = dir == 1 ? min : o1, d2 = dir == 2 ? min : o2;
@@ -474,7 +489,26 @@ __global__ void k_apply1d(int dir, int o0, int o1, int o2, int min, const int *_
                                          : ((int64_t)i0 * d1 + i1) * d2 + j;
             acc += w[e] * in[3 * idx + c];
         }
-        out[t] = acc;
+        if (mask && mask[t]) acc = 0.0;
+        if (accumulate) out[t] += acc;
+        else out[t] = acc;
+    }
+}
+
+// x = s D^-1 b, d = x, r = b   (first Chebyshev step from x = 0)
+__global__ void k_cheb_first(int64_t ncp, const double *__restrict__ Dinv, const double *__restrict__ b, double *__restrict__ x,
+                             double *__restrict__ d, double *__restrict__ r, double sc)
+{
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ncp; p += st) {
+        const double *m = Dinv + 9 * p;
+        const double b0 = b[3 * p], b1 = b[3 * p + 1], b2 = b[3 * p + 2];
+        const double z0 = sc * (m[0] * b0 + m[1] * b1 + m[2] * b2);
+        const double z1 = sc * (m[3] * b0 + m[4] * b1 + m[5] * b2);
+        const double z2 = sc * (m[6] * b0 + m[7] * b1 + m[8] * b2);
+        x[3 * p] = z0; x[3 * p + 1] = z1; x[3 * p + 2] = z2;
+        d[3 * p] = z0; d[3 * p + 1] = z1; d[3 * p + 2] = z2;
+        r[3 * p] = b0; r[3 * p + 1] = b1; r[3 * p + 2] = b2;
     }
 }
 
@@ -1109,7 +1143,8 @@ static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int order, const 
     return IGAP_OK;
 }
 
-static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y)
+// y = A x (mode 0), y = b - A x (mode 1), y -= A x (mode 2)
+static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y, int mode = 0, const double *b = nullptr)
 {
     const int64_t n = c->ndof, ncp = n / 3;
     if (!c->d_pcol) {
@@ -1118,13 +1153,20 @@ static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y)
         hipLaunchKernelGGL(k_point_cols, grid1(ncp), dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr,
                            (const int32_t *)c->d_col, c->d_pcol);
     }
-    const dim3 g((unsigned)((ncp + 3) / 4));
-    if (p == IGAP_F64)
-        hipLaunchKernelGGL(k_spmv_pb<double>, g, dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr, (const int32_t *)c->d_pcol,
-                           (const double *)c->d_val64, x, y);
-    else
-        hipLaunchKernelGGL(k_spmv_pb<float>, g, dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr, (const int32_t *)c->d_pcol,
-                           (const float *)c->d_val32, x, y);
+    const dim3 g((unsigned)((ncp + 3) / 4)), bs(256);
+    const int64_t *rp = (const int64_t *)c->d_rowptr;
+    const int32_t *pc = (const int32_t *)c->d_pcol;
+#define IGAP_SPMV(T, V, M) hipLaunchKernelGGL((k_spmv_pb<T, M>), g, bs, 0, c->stream, ncp, rp, pc, (const T *)V, x, b, y)
+    if (p == IGAP_F64) {
+        if (mode == 0) IGAP_SPMV(double, c->d_val64, 0);
+        else if (mode == 1) IGAP_SPMV(double, c->d_val64, 1);
+        else IGAP_SPMV(double, c->d_val64, 2);
+    } else {
+        if (mode == 0) IGAP_SPMV(float, c->d_val32, 0);
+        else if (mode == 1) IGAP_SPMV(float, c->d_val32, 1);
+        else IGAP_SPMV(float, c->d_val32, 2);
+    }
+#undef IGAP_SPMV
     HIPCHK(hipGetLastError());
     return IGAP_OK;
 }
@@ -1397,7 +1439,8 @@ static std::vector<double> apply3_host(const std::vector<double> &in, const int 
 
 // device: out = (op0 x op1 x op2) in on the control-point grid; which:
 // 0 = P (coarse -> fine of level l), 1 = P^T, 2 = T (fine -> coarse)
-static int mg_transfer(igap_gpu_ctx *c, MGHier *g, int l, int which, const double *in, double *out)
+static int mg_transfer(igap_gpu_ctx *c, MGHier *g, int l, int which, const double *in, double *out,
+                       const int32_t *mask = nullptr, int accumulate = 0)
 {
     MGLevel &F = g->lev[l];
     const igap_gpu_ctx *cf = F.c, *cc = g->lev[l + 1].c;
@@ -1417,8 +1460,9 @@ static int mg_transfer(igap_gpu_ctx *c, MGHier *g, int l, int which, const doubl
         const int64_t n = 3LL * od[0] * od[1] * od[2];
         int64_t blocks = (n + 255) / 256;
         if (blocks > 8192) blocks = 8192;
+        const bool last = q == nd - 1;
         hipLaunchKernelGGL(k_apply1d, dim3((unsigned)blocks), dim3(256), 0, c->stream, d, od[0], od[1], od[2], dims[d], o.rp, o.ci,
-                           o.w, src, dst);
+                           o.w, src, dst, last ? mask : (const int32_t *)nullptr, last ? accumulate : 0);
         src = dst;
         dims[d] = o.rows;
     }
@@ -1532,18 +1576,17 @@ static int mg_cheb(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *b, do
     const double lo = 0.1 * L.lmax, hi = 1.1 * L.lmax;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     int rc;
+    (void)gn;
     if (zero_init) {
-        HIPCHK(launch_fill_f64(x, 0.0, n, c->stream));
-        HIPCHK(hipMemcpyAsync(L.r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_cheb_first, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, b, x, L.t, L.r, 1.0 / theta);
     } else {
-        if ((rc = spmv(cl, p, x, L.r))) return rc;
-        hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, 1.0, b, -1.0, L.r);
+        if ((rc = spmv(cl, p, x, L.r, 1, b))) return rc;   // r = b - A x
+        hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, 0.0, 1.0 / theta,
+                           1);
     }
     double rho = 1.0 / sigma;
-    hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, 0.0, 1.0 / theta, 1);
     for (int k = 1; k < deg; k++) {
-        if ((rc = spmv(cl, p, L.t, g->tmp1))) return rc;
-        hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, -1.0, (const double *)g->tmp1, 1.0, L.r);
+        if ((rc = spmv(cl, p, L.t, L.r, 2))) return rc;     // r -= A d
         const double rn = 1.0 / (2.0 * sigma - rho);
         hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, rn * rho,
                            2.0 * rn / delta, 1);
@@ -1658,15 +1701,11 @@ static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const d
     MGLevel &C = g->lev[l + 1];
     if ((rc = mg_cheb(c, g, l, p, b, x, 1, MG_CHEB_DEG))) return rc;
     // r = b - A x, restricted (zero on coarse Dirichlet rows)
-    if ((rc = spmv(cl, p, x, L.r))) return rc;
-    hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, 1.0, b, -1.0, L.r);
-    if ((rc = mg_transfer(c, g, l, 1, L.r, C.b))) return rc;
-    hipLaunchKernelGGL(k_zero_masked, grid1(C.c->ndof), bs, 0, c->stream, C.c->ndof, C.c->d_mask, C.b);
+    if ((rc = spmv(cl, p, x, L.r, 1, b))) return rc;
+    if ((rc = mg_transfer(c, g, l, 1, L.r, C.b, C.c->d_mask, 0))) return rc;
     if ((rc = mg_cycle(c, s, p, order, l + 1, C.b, C.x))) return rc;
     // x += P x_c (not on Dirichlet rows), then post-smoothing
-    if ((rc = mg_transfer(c, g, l, 0, C.x, L.t))) return rc;
-    hipLaunchKernelGGL(k_zero_masked, gn, bs, 0, c->stream, n, cl->d_mask, L.t);
-    hipLaunchKernelGGL(k_axpby, gn, bs, 0, c->stream, n, 1.0, L.t, 1.0, x);
+    if ((rc = mg_transfer(c, g, l, 0, C.x, x, cl->d_mask, 1))) return rc;
     if ((rc = mg_cheb(c, g, l, p, b, x, 0, MG_CHEB_DEG))) return rc;
     HIPCHK(hipGetLastError());
     return IGAP_OK;
