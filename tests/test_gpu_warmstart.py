@@ -115,14 +115,17 @@ def test_training_lowers_the_loss(ws):
         assert tot < 0.1 * first
 
 
-def test_warm_start_ablation_small_beam(ws, igap):
+@pytest.mark.parametrize("solver", ["ilu0-f32", "mg-mixed"])
+def test_warm_start_ablation_small_beam(ws, igap, solver):
     """The warm-start comparison on a 16x4x4 beam (a ramp of 60 steps to a
     large tip displacement): trained on steps 1..30, evaluated on 31..60 from
     the same state with the previous solution (plain FP64), linear and
-    quadratic extrapolation and the learned guess (FP32 tangent).  Every arm
+    quadratic extrapolation and the learned guess (FP32 tangent with ILU(0),
+    or FP64 tangent with the FP32 multigrid preconditioner).  Every arm
     reaches the plain FP64 answer to 1e-8; the learned candidate is accepted
     against linear extrapolation on most steps, with a smaller residual."""
-    s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=80, lr=2e-3, verbose=False)
+    pc, prec = solver.split("-")
+    s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=80, lr=2e-3, verbose=False, pc=pc, prec=prec)
     for arm in ("plain", "linear", "quadratic", "learned"):
         print(arm, s[arm])
         assert s[arm]["all_converged"]
