@@ -75,9 +75,10 @@ def probe(nel, calls):
 
 
 def pmc_json(nel, fetch_csv, write_csv):
-    """HBM bytes per tangent-kernel launch: FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch) of k_tangent_elem<double,1>, each divided by what the same
-    counter reads per byte of the calibration copy."""
+    """HBM bytes per launch of the assembly kernels: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch, averaged over the dispatches of a kernel), each divided
+    by what the same counter reads per byte of the calibration copy (its one
+    256 MiB dispatch)."""
     import csv
 
     def per_kernel(path, counter):
@@ -89,15 +90,17 @@ def pmc_json(nel, fetch_csv, write_csv):
                 k = r["Kernel_Name"]
                 acc.setdefault(k, {}).setdefault(r["Dispatch_Id"], 0.0)
                 acc[k][r["Dispatch_Id"]] += float(r["Counter_Value"])
-        return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+        return {k: (sum(v.values()) / len(v), max(v.values())) for k, v in acc.items()}
 
     out = {"workload": f"{nel[0]}x{nel[1]}x{nel[2]} static beam, k_tangent_elem<double,1>",
-           "copy_bytes": COPY_BYTES}
+           "copy_bytes": COPY_BYTES, "kernels": {}}
     total = 0.0
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         d = per_kernel(path, name)
-        copy = [v for k, v in d.items() if "copy" in k.lower() or "elementwise" in k.lower()]
-        tang = [v for k, v in d.items() if "k_tangent_elem<double, 1>" in k]
+        # the calibration is the single 256 MiB copy dispatch (the largest of the
+        # copy kernels; other copies of the run are a few bytes)
+        copy = [mx for k, (_, mx) in d.items() if "copy" in k.lower() or "elementwise" in k.lower()]
+        tang = [mean for k, (mean, _) in d.items() if "k_tangent_elem<double, 1>" in k]
         if not copy or not tang:
             raise SystemExit(f"{path}: no copy or tangent dispatch found for {name}")
         factor = max(copy) * 1024.0 / COPY_BYTES
@@ -105,6 +108,14 @@ def pmc_json(nel, fetch_csv, write_csv):
         out[name] = {"tangent_kib_per_dispatch": tang[0], "copy_kib_per_dispatch": max(copy),
                      "calibration_factor": factor, "tangent_bytes_per_dispatch": b}
         total += b
+        # every assembly kernel of the probe, same calibration
+        for k, (v, _) in d.items():
+            if "igap::" not in k:
+                continue
+            short = k.split("(")[0].replace("void ", "").strip()
+            e = out["kernels"].setdefault(short, {})
+            e[name + "_kib_per_dispatch"] = v
+            e[name + "_bytes_per_dispatch"] = v * 1024.0 / factor
     out["tangent_elem_bytes_per_launch"] = total
     path = os.path.join(ROOT, "profiles", f"tangent_elem_{nel[0]}x{nel[1]}x{nel[2]}_pmc.json")
     with open(path, "w") as f:
