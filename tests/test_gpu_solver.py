@@ -157,3 +157,29 @@ def test_coloured_fallback_and_fp32_preconditioner(igap, oracle):
     u0 = out["natural"][0]
     for name in ("coloured_fallback", "fp32_pc"):
         assert np.max(np.abs(out[name][0] - u0)) <= 1e-10 * np.max(np.abs(u0))
+
+
+@pytest.mark.parametrize("case", ["single_level_5x3x3", "transient_16x4x4"])
+def test_multigrid_pc_other_hierarchies(igap, oracle, case):
+    """Multigrid where no direction can be halved (one level: the cycle is
+    Chebyshev/block-Jacobi smoothing) and on the transient physics (three
+    history levels restricted to the coarse levels): Newton reaches the
+    ILU(0)-preconditioned answer to 1e-10 of the displacement scale."""
+    if case == "single_level_5x3x3":
+        m = igap.MeshIGABeam(L=(4, 1, 1), nelem=(5, 3, 3))
+        make = lambda: igap.Assm(m, igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCClampedTip(tip=0.02))
+        U0 = lambda A: np.zeros(A.ndof)
+    else:
+        m = igap.MeshIGABeam(L=(4, 1, 1), nelem=(16, 4, 4))
+        make = lambda: igap.Assm(m, igap.PhysGradElastTime(), igap.BCDefault())
+        U0 = lambda A: oracle.glibc_rand_state(3 * A.ndof)
+    out = {}
+    for pc in ("ilu0", "mg"):
+        with make() as A:
+            A.set_uhist(U0(A))
+            st = A.snes_solve(snes_rtol=1e-12, snes_atol=1e-30, pc=pc, pc_order=1)
+            out[pc] = (A.get_uhist()[:A.ndof].copy(), st)
+        print(f"  {case} {pc}: converged {st['converged']} newton {st['newton_its']} ksp {st['ksp_its']} |F| {st['fnorm']:.2e}")
+        assert st["converged"] > 0
+    u0, u1 = out["ilu0"][0], out["mg"][0]
+    assert np.max(np.abs(u1 - u0)) <= 1e-10 * np.max(np.abs(u0))
