@@ -68,6 +68,61 @@ __device__ __forceinline__ void rs_track(int N, bool up, int &base, int &cnt)
 // factorisation the physical u-jets of every history level
 // (sJet[q][h][k][a], par_periodic added) and the weight, G^-1, d2x/dxi2
 // and W jets (sGeo[q][38]).  Ends with a block barrier.
+// z and y passes of the Gauss-point field sum factorisation:
+//   sA[f][bx*3+by][qz][dz] = sum_bz c_f(b) tz[qz][dz][bz]
+//   sB[f][bx][qy][qz][s6]  = sum_by sA[f][bx*3+by][qz][dz(s6)] ty[qy][dy(s6)][by]
+// with c = w, w X, w U (per field f) and the y-z jet slots s6 = (00, 10, 01,
+// 20, 11, 02) in (dy, dz).  One thread per (f, bx*3+by) for z and per (f, bx,
+// qz) for y: the inputs of a thread are loaded once and all its outputs
+// (12 and 24) follow, with the reference's summation order per output.
+template <int NF, int STR>
+__device__ __forceinline__ void sf_zy(const double (*sT)[4][3][3], const double *sCP, double *sA, double *sB)
+{
+    static_assert(NF * 12 <= 256, "one thread per (field, bx, qz) in a 256-thread workgroup");
+    const int tid = threadIdx.x;
+    if (tid < NF * 9) {
+        const int f = tid / 9, bxy = tid - 9 * f;
+        double cf[3];
+#pragma unroll
+        for (int bz = 0; bz < 3; bz++) {
+            const double *c = sCP + STR * (3 * bxy + bz);
+            cf[bz] = (f == 0) ? c[3] : c[3] * ((f < 4) ? c[f - 1] : c[f]);
+        }
+        double *o = sA + 108 * f + 12 * bxy;
+#pragma unroll
+        for (int qz = 0; qz < 4; qz++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) {
+                const double *tz = &sT[2][qz][dz][0];
+                double acc = 0.0;
+#pragma unroll
+                for (int bz = 0; bz < 3; bz++) acc += cf[bz] * tz[bz];
+                o[3 * qz + dz] = acc;
+            }
+    }
+    __syncthreads();
+    if (tid < NF * 12) {
+        const int f = tid / 12, r = tid - 12 * f, bx = r >> 2, qz = r & 3;
+        const double *a = sA + 108 * f + 36 * bx + 3 * qz;   // a[12 by + dz]
+        double av[3][3];
+#pragma unroll
+        for (int by = 0; by < 3; by++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) av[by][dz] = a[12 * by + dz];
+        double *o = sB + 288 * f + 96 * bx + 6 * qz;         // o[24 qy + s6]
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++)
+#pragma unroll
+            for (int s6 = 0; s6 < 6; s6++) {
+                const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
+                const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
+                const double *ty = &sT[1][qy][dy][0];
+                o[24 * qy + s6] = av[0][dz] * ty[0] + av[1][dz] * ty[1] + av[2][dz] * ty[2];
+            }
+    }
+    __syncthreads();
+}
+
 template <int NH>
 __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restrict__ Uh, int64_t ndof, int e,
                                           double (*sT)[4][3][3], double *sCP, double *sWork, double *sJet, double *sGeo)
@@ -105,28 +160,7 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
     // fields at the Gauss points (z, then y, then x)
     {
         double *sA = sWork, *sB = sWork + NF * 108;
-        for (int i = tid; i < NF * 108; i += 256) {            // [f][bx*3+by][qz][dz]
-            const int f = i / 108, r = i - 108 * f, bxy = r / 12, qd = r - 12 * bxy, qz = qd / 3, dz = qd - 3 * qz;
-            const double *tz = &sT[2][qz][dz][0];
-            double acc = 0.0;
-#pragma unroll
-            for (int bz = 0; bz < 3; bz++) {
-                const double *c = sCP + STR * (3 * bxy + bz);
-                const double cf = (f == 0) ? c[3] : c[3] * ((f < 4) ? c[f - 1] : c[f]);
-                acc += cf * tz[bz];
-            }
-            sA[i] = acc;
-        }
-        __syncthreads();
-        for (int i = tid; i < NF * 288; i += 256) {            // [f][bx][qy][qz][s6]
-            const int f = i / 288, r = i - 288 * f, s6 = r % 6, t = r / 6, qz = t & 3, qy = (t >> 2) & 3, bx = t >> 4;
-            const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
-            const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
-            const double *ty = &sT[1][qy][dy][0];
-            const double *a = sA + 108 * f + 36 * bx + 3 * qz + dz;
-            sB[i] = a[0] * ty[0] + a[12] * ty[1] + a[24] * ty[2];
-        }
-        __syncthreads();
+        sf_zy<NF, STR>(sT, sCP, sA, sB);
 
         const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
         const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
@@ -540,28 +574,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     //      sum over b of c_b R_b in another order of summation.
     {
         double *sA = reinterpret_cast<double *>(sWork), *sB = sA + NF * 108;
-        for (int i = tid; i < NF * 108; i += 256) {            // [f][bx*3+by][qz][dz]
-            const int f = i / 108, r = i - 108 * f, bxy = r / 12, qd = r - 12 * bxy, qz = qd / 3, dz = qd - 3 * qz;
-            const double *tz = &sT[2][qz][dz][0];
-            double acc = 0.0;
-#pragma unroll
-            for (int bz = 0; bz < 3; bz++) {
-                const double *c = sCP + STR * (3 * bxy + bz);
-                const double cf = (f == 0) ? c[3] : c[3] * ((f < 4) ? c[f - 1] : c[f]);
-                acc += cf * tz[bz];
-            }
-            sA[i] = acc;
-        }
-        __syncthreads();
-        for (int i = tid; i < NF * 288; i += 256) {            // [f][bx][qy][qz][s6]
-            const int f = i / 288, r = i - 288 * f, s6 = r % 6, t = r / 6, qz = t & 3, qy = (t >> 2) & 3, bx = t >> 4;
-            const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
-            const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
-            const double *ty = &sT[1][qy][dy][0];
-            const double *a = sA + 108 * f + 36 * bx + 3 * qz + dz;
-            sB[i] = a[0] * ty[0] + a[12] * ty[1] + a[24] * ty[2];
-        }
-        __syncthreads();
+        sf_zy<NF, STR>(sT, sCP, sA, sB);
 
         const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
         const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
