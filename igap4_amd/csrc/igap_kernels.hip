@@ -674,11 +674,48 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             mu.m2 += P.ct[j] * P.s[j] * P.s[j];
         }
     }
+    T Rn[10];   // static: basis jets of the next chunk's point, built by lanes 32..58 on even chunks
     for (int chunk = 0; chunk < 64 / QC; chunk++) {
         const int q0 = chunk * QC;
         // ---- B1 (transient: + the rule-node states): wave wv builds the basis
-        //      jets of Gauss point q0 + wv
-        {
+        //      jets of Gauss point q0 + wv.  Static: lanes 32..58 are otherwise
+        //      idle here, so on even chunks they build the jets of point
+        //      q0 + 4 + wv (the next chunk's) and store them on the odd chunk:
+        //      one B1 pass per two chunks.
+        if (NH == 1) {
+            const int ql = wv;
+            if ((chunk & 1) == 0) {
+                const int hi = lane >> 5, b = lane & 31, q = q0 + ql + 4 * hi;
+                if (b < 27) {
+                    const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+                    const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
+                    const double *g = sGeo + q * GEO;
+                    double N[10], R[10];
+                    tensor_jet(t0, t1, t2, b, N);
+                    nurbs_param(N, sCP[STR * b + 3], g + 28, R);
+                    phys_jet(R, g + 1, g + 10);
+                    if (hi == 0) {
+#pragma unroll
+                        for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)R[a];
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 10; a++) Rn[a] = (T)R[a];
+                    }
+                } else if (hi == 0) {
+#pragma unroll
+                    for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)0;
+                }
+            } else {
+                const int b = lane - 32;
+                if (b >= 0 && b < 27) {
+#pragma unroll
+                    for (int a = 0; a < 10; a++) sR[ql][a][b] = Rn[a];
+                } else if (lane >= 27 && lane < 32) {
+#pragma unroll
+                    for (int a = 0; a < 10; a++) sR[ql][a][lane] = (T)0;
+                }
+            }
+        } else {
             const int ql = wv, q = q0 + ql, b = lane;
             if (b < 27) {
                 const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
