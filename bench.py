@@ -138,6 +138,48 @@ def traffic_from_profiles(nel):
         return None
 
 
+def measured_bytes(nel, kernel):
+    """HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, calibrated) of one
+    kernel from the committed PMC summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"tangent_elem_{nel[0]}x{nel[1]}x{nel[2]}_pmc.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel]
+        return float(k["FETCH_SIZE_bytes_per_dispatch"] + k["WRITE_SIZE_bytes_per_dispatch"])
+    except Exception:
+        return None
+
+
+def kernel_rooflines(nel, nelem, nnz, avg, avg32):
+    """Every assembly kernel against the ceiling that bounds it.  Tangent:
+    F_K per element on the matrix cores; residual: F_R per element on the
+    FP64 vector ALUs (counted flops only: basis, geometry and the pointwise
+    physics come on top); gathers: the element matrices read once and the
+    CSR values written once (8 or 4 bytes each), with the CSR values alone
+    and the measured HBM bytes beside them."""
+    def flop(kernel_ms, f, peak, bound):
+        a = f * nelem / (kernel_ms * 1e-3) / 1e12
+        return {"bound": bound, "achieved": a, "peak": peak, "unit": "TFLOP/s", "frac": a / peak, "ms": kernel_ms}
+
+    def hbm(kernel_ms, nbytes, name, csr_bytes):
+        a = nbytes / (kernel_ms * 1e-3) / 1e9
+        mb = measured_bytes(nel, name)
+        return {"bound": "hbm", "achieved": a, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": a / PEAK_HBM_GBS,
+                "ms": kernel_ms, "algorithmic_bytes": nbytes, "csr_value_bytes": csr_bytes,
+                "csr_value_GBs": csr_bytes / (kernel_ms * 1e-3) / 1e9, "measured_bytes": mb,
+                "measured_GBs": None if mb is None else mb / (kernel_ms * 1e-3) / 1e9}
+
+    return {
+        "k_tangent_elem<double,1>": flop(avg[1], F_K, PEAK_FP64_TFLOPS, "mfma"),
+        "k_tangent_elem<float,1>": flop(avg32[1], F_K, PEAK_FP32_TFLOPS, "mfma"),
+        "k_residual_elem<1>": flop(avg[0], F_R_STATIC, PEAK_FP64_TFLOPS, "valu"),
+        "k_tangent_gather_rows<double,double>": hbm(avg[2], 8 * (6561 * nelem + nnz),
+                                                    "igap::k_tangent_gather_rows<double, double>", 8 * nnz),
+        "k_tangent_gather<float,float>": hbm(avg32[2], 4 * (6561 * nelem + nnz), "igap::k_tangent_gather<float, float>",
+                                             4 * nnz),
+    }
+
+
 def cpu_baseline(threads, nel, nx=None):
     """The oracle (CPU restatement with the reference's loop nest and naive
     81x81x100 contraction) on the same workload: one residual and one tangent
@@ -250,6 +292,7 @@ def main():
                        "residual_call": avg[3], "tangent_call_f64": avg[4],
                        "tangent_elem_f32": avg32[1], "tangent_gather_f32": avg32[2],
                        "tangent_call_f32": avg32[4]},
+        "kernel_rooflines": kernel_rooflines(nel, nelem, A.nnz, avg, avg32),
         "fp32_tangent": {"value": nelem / (ms32 * 1e-3), "ms_per_step": ms32,
                          "tangent_elem_tflops": F_K * nelem / (avg32[1] * 1e-3) / 1e12,
                          "peak": PEAK_FP32_TFLOPS},
