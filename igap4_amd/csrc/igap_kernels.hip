@@ -226,7 +226,7 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
 }
 
 template <int NH>
-__global__ __launch_bounds__(256) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+__global__ __launch_bounds__(256, (NH == 1 ? 4 : 1)) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                        double *__restrict__ Re, int nel)
 {
     constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
