@@ -563,48 +563,64 @@ __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, i
 // The transient rule weights the pointwise tangent of eight states
 // us_j = m0 + s_j d.  F and H are linear in the jet, F_j = F0 + s_j Fd and
 // H_j = H0 + s_j Hd, so every product of two of them sums over the nodes to
-//   sum_j ct_j X_j Y_j = mu0 X0 Y0 + mu1 (X0 Yd + Xd Y0) + mu2 Xd Yd
-// (mu_p = sum_j ct_j s_j^p) and the N-weighted ones to the same form with the
-// node sums N_p = sum_j ct_j s_j^p N_j; S and G enter linearly (Sb, Gb).
-// The moment state per Gauss point:
-constexpr int MF0 = 0, MFD = 9, MH0 = 18, MHD = 45, MS = 72, MG = 81, MN0 = 90, MN1 = 99, MN2 = 108, MW = 117;
+// the quadratic form
+//   sum_j ct_j X_j Y_j = (X0, Xd) M (Y0, Yd)^T,  M = [[mu0, mu1], [mu1, mu2]]
+// (mu_p = sum_j ct_j s_j^p), and the N-weighted ones to (X0, Xd) Nm (Y0, Yd)^T
+// with Nm = [[N_0, N_1], [N_1, N_2]], N_p = sum_j ct_j s_j^p N_j; S and G
+// enter linearly (Sb, Gb).  The rule integrates 1, s and s^2 exactly against
+// the positive weight s/2 on [0, 1], so M (= [[1/4, 1/6], [1/6, 1/8]] to
+// rounding) is positive definite; with its Cholesky factor M = L L^T and
+// X^ = L^T (X0, Xd) the first form is X^_1 Y^_1 + X^_2 Y^_2 and the second
+// X^^T (L^-1 Nm L^-T) Y^.  The moment state keeps the transformed quantities:
+//   MF1, MF2: F^ (9 each)      MH1, MH2: H^ (27 each)
+//   MS, MG:   Sb, Gb           MN11, MN12, MN22: L^-1 Nm L^-T (9 each)
+constexpr int MF1 = 0, MF2 = 9, MH1 = 18, MH2 = 45, MS = 72, MG = 81, MN11 = 90, MN12 = 99, MN22 = 108, MW = 117;
 
+// Cholesky factor of M: L = [[a11, 0], [a11 lf, a22]], so that
+// X^_1 = a11 (X0 + lf Xd), X^_2 = a22 Xd
 struct Mom {
     double m0, m1, m2;
-    __device__ __forceinline__ double mp(double x0, double xd, double y0, double yd) const
+    double lf, a11, a22, d2;
+    __device__ __forceinline__ void factor()
     {
-        return m0 * x0 * y0 + m1 * (x0 * yd + xd * y0) + m2 * xd * yd;
+        lf = m1 / m0;
+        a11 = sqrt(m0);
+        d2 = m2 - m1 * lf;
+        a22 = sqrt(d2);
     }
 };
 
-// c_row for the rule-weighted transient tangent, from the moment state ms
+__device__ __forceinline__ double dot2(double x1, double x2, double y1, double y2) { return x1 * y1 + x2 * y2; }
+
+// c_row for the rule-weighted transient tangent, from the transformed moment
+// state ms
 template <int A0>
-__device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, const Mom &mu, int k, int a, int l, double *row)
+__device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, int k, int a, int l, double *row)
 {
-    const double *F0 = ms + MF0, *Fd = ms + MFD, *H0 = ms + MH0, *Hd = ms + MHD;
+    const double *F1 = ms + MF1, *F2 = ms + MF2, *H1 = ms + MH1, *H2 = ms + MH2;
     const double g2 = 2 * P.le * P.le;
     const bool dia = (k == l);
     if (a >= 1 && a <= 3) {
         const int L = a - 1;
-        const double FkL0 = F0[3 * k + L], FkLd = Fd[3 * k + L];
+        const double FkL1 = F1[3 * k + L], FkL2 = F2[3 * k + L];
 #pragma unroll
         for (int M = 0; M < 3; M++) {
-            const double Fl0 = F0[3 * l + M], Fld = Fd[3 * l + M];
-            double v = ms[MN0 + 3 * L + M] * FkL0 * Fl0 + ms[MN1 + 3 * L + M] * (FkL0 * Fld + FkLd * Fl0)
-                     + ms[MN2 + 3 * L + M] * FkLd * Fld;
+            const double Fl1 = F1[3 * l + M], Fl2 = F2[3 * l + M];
+            double v = ms[MN11 + 3 * L + M] * FkL1 * Fl1 + ms[MN12 + 3 * L + M] * (FkL1 * Fl2 + FkL2 * Fl1)
+                     + ms[MN22 + 3 * L + M] * FkL2 * Fl2;
             double e = 0.0;
             if (M == L) {
 #pragma unroll
                 for (int J = 0; J < 3; J++)
-                    if (J != L) e += mu.mp(F0[3 * k + J], Fd[3 * k + J], F0[3 * l + J], Fd[3 * l + J]);
+                    if (J != L) e += dot2(F1[3 * k + J], F2[3 * k + J], F1[3 * l + J], F2[3 * l + J]);
             } else {
-                e = mu.mp(F0[3 * k + M], Fd[3 * k + M], F0[3 * l + L], Fd[3 * l + L]);
+                e = dot2(F1[3 * k + M], F2[3 * k + M], F1[3 * l + L], F2[3 * l + L]);
             }
             v += 0.5 * P.e345 * e;
             double hh = 0.0;
 #pragma unroll
             for (int c = 0; c < 3; c++)
-                hh += mu.mp(H0[9 * k + 3 * L + c], Hd[9 * k + 3 * L + c], H0[9 * l + 3 * M + c], Hd[9 * l + 3 * M + c]);
+                hh += dot2(H1[9 * k + 3 * L + c], H2[9 * k + 3 * L + c], H1[9 * l + 3 * M + c], H2[9 * l + 3 * M + c]);
             v += g2 * ((M == L) ? 2.0 / 3.0 : -1.0 / 3.0) * hh;
             if (dia) v += ms[MS + 3 * L + M];
             row[1 + M - A0] += v;
@@ -614,10 +630,10 @@ __device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, cons
             constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
             const int I = PI[pp], K = PK[pp];
             double v = g2 * ((L == I) ? 2.0 / 3.0 : -1.0 / 3.0)
-                     * mu.mp(H0[9 * k + 3 * L + K], Hd[9 * k + 3 * L + K], F0[3 * l + I], Fd[3 * l + I]);
+                     * dot2(H1[9 * k + 3 * L + K], H2[9 * k + 3 * L + K], F1[3 * l + I], F2[3 * l + I]);
             if (I != K)
                 v += g2 * ((L == K) ? 2.0 / 3.0 : -1.0 / 3.0)
-                   * mu.mp(H0[9 * k + 3 * L + I], Hd[9 * k + 3 * L + I], F0[3 * l + K], Fd[3 * l + K]);
+                   * dot2(H1[9 * k + 3 * L + I], H2[9 * k + 3 * L + I], F1[3 * l + K], F2[3 * l + K]);
             if (dia) {
                 if (L == I) v += ms[MG + 3 * I + K];
                 if (I != K && L == K) v += ms[MG + 3 * K + I];
@@ -628,11 +644,11 @@ __device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, cons
         constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
         const int I = PI[a - 4], K = PK[a - 4];
         const bool off = (I != K);
-        const double FkI0 = F0[3 * k + I], FkId = Fd[3 * k + I], FkK0 = F0[3 * k + K], FkKd = Fd[3 * k + K];
+        const double FkI1 = F1[3 * k + I], FkI2 = F2[3 * k + I], FkK1 = F1[3 * k + K], FkK2 = F2[3 * k + K];
 #pragma unroll
         for (int M = 0; M < 3; M++) {
-            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(H0[9 * l + 3 * M + K], Hd[9 * l + 3 * M + K], FkI0, FkId);
-            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(H0[9 * l + 3 * M + I], Hd[9 * l + 3 * M + I], FkK0, FkKd);
+            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(H1[9 * l + 3 * M + K], H2[9 * l + 3 * M + K], FkI1, FkI2);
+            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(H1[9 * l + 3 * M + I], H2[9 * l + 3 * M + I], FkK1, FkK2);
             if (dia) {
                 if (M == I) v += ms[MG + 3 * I + K];
                 if (off && M == K) v += ms[MG + 3 * K + I];
@@ -644,10 +660,10 @@ __device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, cons
             constexpr int PJ[6] = {0, 0, 0, 1, 1, 2}, PK2[6] = {0, 1, 2, 1, 2, 2};
             const int J = PJ[pp], K2 = PK2[pp];
             double v = 0.0;
-            if (K == K2) v += ((I == J) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkI0, FkId, F0[3 * l + J], Fd[3 * l + J]);
-            if (J != K2 && K == J) v += ((I == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkI0, FkId, F0[3 * l + K2], Fd[3 * l + K2]);
-            if (off && I == K2) v += ((K == J) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkK0, FkKd, F0[3 * l + J], Fd[3 * l + J]);
-            if (off && J != K2 && I == J) v += ((K == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * mu.mp(FkK0, FkKd, F0[3 * l + K2], Fd[3 * l + K2]);
+            if (K == K2) v += ((I == J) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkI1, FkI2, F1[3 * l + J], F2[3 * l + J]);
+            if (J != K2 && K == J) v += ((I == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkI1, FkI2, F1[3 * l + K2], F2[3 * l + K2]);
+            if (off && I == K2) v += ((K == J) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkK1, FkK2, F1[3 * l + J], F2[3 * l + J]);
+            if (off && J != K2 && I == J) v += ((K == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkK1, FkK2, F1[3 * l + K2], F2[3 * l + K2]);
             row[4 + pp - A0] += g2 * v;
         }
     }

@@ -665,7 +665,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
 #pragma unroll
     for (int t = 0; t < 3; t++) { bi[t] = kBlkI[blk0 + t]; bj[t] = kBlkJ[blk0 + t]; }
 
-    Mom mu = {0.0, 0.0, 0.0};
+    Mom mu = {0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0};
     if (NH != 1) {
 #pragma unroll
         for (int j = 0; j < NNODE; j++) {
@@ -673,6 +673,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             mu.m1 += P.ct[j] * P.s[j];
             mu.m2 += P.ct[j] * P.s[j] * P.s[j];
         }
+        mu.factor();
     }
     T Rn[10];   // static: basis jets of the next chunk's point, built by lanes 32..58 on even chunks
     for (int chunk = 0; chunk < 64 / QC; chunk++) {
@@ -757,30 +758,37 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                     v[i] += __shfl_xor(v[i], 4, 64);
                 }
                 if (j == 0) {
+                    // Sb, Gb as summed; the N node sums as L^-1 Nm L^-T
                     double *ms = sSt + ql * MW;
 #pragma unroll
-                    for (int i = 0; i < 45; i++) ms[MS + i] = v[i];
+                    for (int i = 0; i < 18; i++) ms[MS + i] = v[i];
+                    const double c12 = 1.0 / (mu.a11 * mu.a22), c22 = 1.0 / mu.d2;
+#pragma unroll
+                    for (int i = 0; i < 9; i++) {
+                        const double n0 = v[18 + i], n1 = v[27 + i], n2 = v[36 + i];
+                        ms[MN11 + i] = n0 / mu.m0;
+                        ms[MN12 + i] = (n1 - mu.lf * n0) * c12;
+                        ms[MN22 + i] = ((mu.lf * n0 - n1) * mu.lf + (n2 - mu.lf * n1)) * c22;
+                    }
                 }
             } else if (NH != 1 && b >= 40) {
-                // F0, Fd, H0, Hd of the midpoint m0 = (u1+u0)/2 and the
-                // increment d = (u-u0)/2
+                // F^ and H^ (two components each) of the midpoint m0 = (u1+u0)/2
+                // and the increment d = (u-u0)/2: X^_1 = a11 (X0 + lf Xd),
+                // X^_2 = a22 Xd, for the 9 entries of F and the 27 of H
                 const double *u = sJet + q * 30 * NH;
                 double *ms = sSt + ql * MW;
-                for (int i = b - 40; i < 72; i += 24) {
-                    const bool isF = i < 18, dpart = isF ? (i >= 9) : (i >= 45);
-                    const int r = isF ? (i % 9) : ((i - 18) % 27);
-                    int slot, k;
-                    if (isF) {
-                        k = r / 3;
-                        slot = 10 * k + 1 + (r % 3);
-                    } else {
-                        k = r / 9;
-                        slot = 10 * k + sidx((r % 9) / 3, r % 3);
-                    }
-                    const double m0v = 0.5 * (u[30 + slot] + u[60 + slot]), dv = 0.5 * (u[slot] - u[60 + slot]);
-                    double val = dpart ? dv : m0v;
-                    if (isF && !dpart && (r / 3) == (r % 3)) val += 1.0;
-                    ms[i] = val;
+                for (int i = b - 40; i < 36; i += 24) {
+                    const bool isF = i < 9;
+                    const int r = isF ? i : i - 9;
+                    int slot;
+                    if (isF) slot = 10 * (r / 3) + 1 + (r % 3);
+                    else slot = 10 * (r / 9) + sidx((r % 9) / 3, r % 3);
+                    double x0 = 0.5 * (u[30 + slot] + u[60 + slot]);
+                    const double xd = 0.5 * (u[slot] - u[60 + slot]);
+                    if (isF && (r / 3) == (r % 3)) x0 += 1.0;
+                    const double x1 = mu.a11 * (x0 + mu.lf * xd), x2 = mu.a22 * xd;
+                    if (isF) { ms[MF1 + r] = x1; ms[MF2 + r] = x2; }
+                    else { ms[MH1 + r] = x1; ms[MH2 + r] = x2; }
                 }
             }
         }
@@ -796,7 +804,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
                 if (NH == 1) c_row<A0>(sSt + q * STW, P, k, ar + A0, l, 1.0, row);
-                else c_row_mom<A0>(sSt + ql * MW, P, mu, k, ar + A0, l, row);
+                else c_row_mom<A0>(sSt + ql * MW, P, k, ar + A0, l, row);
                 if (NH != 1 && ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
                 T *D = sD[ql] + p * NN + ar * RS;
 #pragma unroll
