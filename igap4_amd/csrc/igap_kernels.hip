@@ -75,18 +75,19 @@ __device__ __forceinline__ void rs_track(int N, bool up, int &base, int &cnt)
 // 20, 11, 02) in (dy, dz).  One thread per (f, bx*3+by) for z and per (f, bx,
 // qz) for y: the inputs of a thread are loaded once and all its outputs
 // (12 and 24) follow, with the reference's summation order per output.
-template <int NF, int STR>
+// The NF fields are F0 .. F0 + NF - 1 of that list, stored from field 0 on.
+template <int NF, int STR, int F0 = 0>
 __device__ __forceinline__ void sf_zy(const double (*sT)[4][3][3], const double *sCP, double *sA, double *sB)
 {
     static_assert(NF * 12 <= 256, "one thread per (field, bx, qz) in a 256-thread workgroup");
     const int tid = threadIdx.x;
     if (tid < NF * 9) {
-        const int f = tid / 9, bxy = tid - 9 * f;
+        const int f = tid / 9, bxy = tid - 9 * f, fg = F0 + f;
         double cf[3];
 #pragma unroll
         for (int bz = 0; bz < 3; bz++) {
             const double *c = sCP + STR * (3 * bxy + bz);
-            cf[bz] = (f == 0) ? c[3] : c[3] * ((f < 4) ? c[f - 1] : c[f]);
+            cf[bz] = (fg == 0) ? c[3] : c[3] * ((fg < 4) ? c[fg - 1] : c[fg]);
         }
         double *o = sA + 108 * f + 12 * bxy;
 #pragma unroll
@@ -505,10 +506,14 @@ __constant__ const int kBlkJ[6] = {0, 1, 2, 1, 2, 2};
 __device__ __forceinline__ int blk_index(int k, int l) { return k == 0 ? l : (k == 1 ? 2 + l : 5); }   // k <= l
 
 template <typename T, int NH>
-__global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh,
-                                                                         int64_t ndof, T *__restrict__ Ke, int nel)
+__global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                         T *__restrict__ Ke, int nel)
 {
-    constexpr int STR = 4 + 3 * NH;
+    // field levels held per control point: static U; transient the two
+    // combinations the rule evaluates, Um = (U^1 + U^0)/2 and Ud = (U - U^0)/2
+    // (rule node j: Um + s_j Ud)
+    constexpr int NL = (NH == 1) ? 1 : 2;
+    constexpr int STR = 4 + 3 * NL;
     constexpr int NNODE = (NH == 1) ? 1 : 8;
     constexpr int A0 = (NH == 1) ? 1 : 0;     // first jet slot in the contraction
     constexpr int NA = 10 - A0;               // jet slots per Gauss point in K
@@ -519,17 +524,22 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     constexpr int NN = NA * RS;                   // stride of one (i,j) block of D
     static_assert(QC == 4, "the MFMA k-row is the Gauss point of a 4-point chunk");
     // static: the strain-measure state of all 64 Gauss points is built once in
-    // stage A; transient: the u-jets of the three levels are kept and the
-    // eight rule-node states are built per chunk
+    // stage A; transient: the u-jets of Um and Ud are kept and the eight
+    // rule-node states are built per chunk.  Two workgroups share a CU, so
+    // the transient keeps its LDS under 80 KB: the field sum factorisation
+    // runs in two passes (geometry, then Um/Ud) and the weight jet W is
+    // rebuilt per chunk from the basis instead of being stored.
     constexpr int SST = (NH == 1) ? 64 * STW : QC * MW;   // static: states; transient: moment states
+    constexpr int GEOK = (NH == 1) ? GEO : GEO - 10;     // per Gauss point: wgt, Gi[9], X2[18] (+ W[10] static)
     __shared__ double sT[3][4][3][3];          // 1-D tables [d][q_d][order][i]
     __shared__ double sCP[27 * STR];
-    __shared__ double sGeo[64 * GEO];
-    __shared__ double sJet[(NH == 1) ? 1 : 64 * 30 * NH];
+    __shared__ double sGeo[64 * GEOK];
+    __shared__ double sJet[(NH == 1) ? 1 : 64 * 30 * NL];
     __shared__ double sSt[SST];
     // stage A's sum-factorisation buffers share memory with stage B's R and D
-    constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
-    constexpr int WORK_A = NF * (108 + 288) * (int)sizeof(double);
+    constexpr int NF = 4 + 3 * NL;             // fields w, w X (3), w U (3 per held level)
+    constexpr int NFP = (NH == 1) ? NF : 6;    // fields per sum-factorisation pass
+    constexpr int WORK_A = NFP * (108 + 288) * (int)sizeof(double);
     constexpr int WORK_B = QC * (10 * 32 + 6 * NN) * (int)sizeof(T);
     __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_B ? WORK_A : WORK_B];
     T(*sR)[10][32] = reinterpret_cast<T(*)[10][32]>(sWork);
@@ -552,10 +562,17 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
         sCP[STR * tid + 3] = M.w[id];
         int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ax, M.nb[0], M.periodic[0])
                    + (int64_t)M.nb[2] * own1(ay, M.nb[1], M.periodic[1]) + own1(az, M.nb[2], M.periodic[2]);
+        if (NH == 1) {
 #pragma unroll
-        for (int h = 0; h < NH; h++)
+            for (int k = 0; k < 3; k++) sCP[STR * tid + 4 + k] = Uh[3 * oc + k];
+        } else {
 #pragma unroll
-            for (int k = 0; k < 3; k++) sCP[STR * tid + 4 + 3 * h + k] = Uh[h * ndof + 3 * oc + k];
+            for (int k = 0; k < 3; k++) {
+                const double u0 = Uh[3 * oc + k], u1 = Uh[ndof + 3 * oc + k], u2 = Uh[2 * ndof + 3 * oc + k];
+                sCP[STR * tid + 4 + k] = 0.5 * (u1 + u2);
+                sCP[STR * tid + 7 + k] = 0.5 * (u0 - u2);
+            }
+        }
     } else if (tid >= 64 && tid < 64 + 12) {
         int d = (tid - 64) / 4, q = (tid - 64) % 4;
         const double *kV = M.knot[d] + ed[d];
@@ -573,8 +590,8 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
     //      rule, both linear in the basis, so the result is the reference's
     //      sum over b of c_b R_b in another order of summation.
     {
-        double *sA = reinterpret_cast<double *>(sWork), *sB = sA + NF * 108;
-        sf_zy<NF, STR>(sT, sCP, sA, sB);
+        double *sA = reinterpret_cast<double *>(sWork), *sB = sA + NFP * 108;
+        sf_zy<(NH == 1 ? NF : 4), STR>(sT, sCP, sA, sB);
 
         const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
         const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
@@ -583,14 +600,17 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
         double W[10], Jg[3][10];
         sf_jet(sB + boff, tx, W);
         // part 0: the three geometry fields; part 1 + k: displacement
-        // component k of every level
+        // component k of every held level (transient: second pass below)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int f = (part == 0) ? 1 + j : 4 + 3 * j + part - 1;
-            if (part == 0 || j < NH) {
+            if (part == 0 || (NH == 1 && j < NL)) {
                 double Pj[10];
                 sf_jet(sB + 288 * f + boff, tx, Pj);
                 nurbs_param(Pj, 1.0, W, Jg[j]);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 10; a++) Jg[j][a] = (a >= 1 && a <= 3 && a - 1 == j) ? 1.0 : 0.0;
             }
         }
         double G[9], Gi[9], X2[18];
@@ -613,15 +633,30 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
             dX[d] = kV[3] - kV[2];
         }
+        if (NH != 1) {
+            // second pass: the fields w Um (3) and w Ud (3)
+            __syncthreads();
+            sf_zy<6, STR, 4>(sT, sCP, sA, sB);
+            if (part > 0) {
+#pragma unroll
+                for (int h = 0; h < NL; h++) {
+                    double Pj[10];
+                    sf_jet(sB + 288 * (3 * h + part - 1) + boff, tx, Pj);
+                    nurbs_param(Pj, 1.0, W, Jg[h]);
+                }
+            }
+        }
         if (part > 0) {
             const int k = part - 1;
 #pragma unroll
-            for (int h = 0; h < NH; h++) {
+            for (int h = 0; h < NL; h++) {
                 phys_jet(Jg[h], Gi, X2);
-                add_periodic1(M, xq, k, Jg[h]);
+                // the periodic macro-field shifts every level alike: Um takes
+                // it, the difference Ud does not
+                if (h == 0) add_periodic1(M, xq, k, Jg[h]);
                 if (NH != 1) {
 #pragma unroll
-                    for (int a = 0; a < 10; a++) sJet[q * 30 * NH + 30 * h + 10 * k + a] = Jg[h][a];
+                    for (int a = 0; a < 10; a++) sJet[q * 30 * NL + 30 * h + 10 * k + a] = Jg[h][a];
                 }
             }
         }
@@ -644,14 +679,16 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             }
         }
         if (part == 0) {
-            double *g = sGeo + q * GEO;
+            double *g = sGeo + q * GEOK;
             g[0] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
 #pragma unroll
             for (int i = 0; i < 9; i++) g[1 + i] = Gi[i];
 #pragma unroll
             for (int i = 0; i < 18; i++) g[10 + i] = X2[i];
+            if (NH == 1) {
 #pragma unroll
-            for (int i = 0; i < 10; i++) g[28 + i] = W[i];
+                for (int i = 0; i < 10; i++) g[28 + i] = W[i];
+            }
         }
     }
     __syncthreads();
@@ -690,7 +727,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                 if (b < 27) {
                     const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
                     const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
-                    const double *g = sGeo + q * GEO;
+                    const double *g = sGeo + q * GEOK;
                     double N[10], R[10];
                     tensor_jet(t0, t1, t2, b, N);
                     nurbs_param(N, sCP[STR * b + 3], g + 28, R);
@@ -718,54 +755,65 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             }
         } else {
             const int ql = wv, q = q0 + ql, b = lane;
-            if (b < 27) {
+            if (b < 32) {
+                // basis jets; the weight jet W = sum_b w_b N_b is summed over
+                // lanes 0..31 (27..31 add zero)
                 const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
                 const double *t0 = &sT[0][qd[0]][0][0], *t1 = &sT[1][qd[1]][0][0], *t2 = &sT[2][qd[2]][0][0];
-                const double *g = sGeo + q * GEO;
-                double N[10], R[10];
-                tensor_jet(t0, t1, t2, b, N);
-                nurbs_param(N, sCP[STR * b + 3], g + 28, R);
-                phys_jet(R, g + 1, g + 10);
+                const double *g = sGeo + q * GEOK;
+                double N[10], W[10], R[10];
+                const double wb = (b < 27) ? sCP[STR * b + 3] : 0.0;
+                if (b < 27) {
+                    tensor_jet(t0, t1, t2, b, N);
+                } else {
 #pragma unroll
-                for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)R[a];
-            } else if (b < 32) {
+                    for (int a = 0; a < 10; a++) N[a] = 0.0;
+                }
 #pragma unroll
-                for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)0;
+                for (int a = 0; a < 10; a++) W[a] = wb * N[a];
+#pragma unroll
+                for (int o = 1; o < 32; o <<= 1)
+#pragma unroll
+                    for (int a = 0; a < 10; a++) W[a] += __shfl_xor(W[a], o, 64);
+                if (b < 27) {
+                    nurbs_param(N, wb, W, R);
+                    phys_jet(R, g + 1, g + 10);
+#pragma unroll
+                    for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)R[a];
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 10; a++) sR[ql][a][b] = (T)0;
+                }
             } else if (NH != 1 && b < 32 + NNODE) {
                 // rule node j: S, G and N at us_j, weighted and summed over
                 // the eight lanes 32..39 into the moment state
                 const int j = b - 32;
-                const double *u = sJet + q * 30 * NH;
+                const double *u = sJet + q * 30 * NL;
                 double us[30];
 #pragma unroll
-                for (int i = 0; i < 30; i++) us[i] = 0.5 * (u[60 + i] + u[30 + i]) + P.s[j] * (0.5 * (u[i] - u[60 + i]));
+                for (int i = 0; i < 30; i++) us[i] = u[i] + P.s[j] * u[30 + i];
                 State st;
                 phys_state(us, P, st);
                 const double w0 = P.ct[j], w1 = w0 * P.s[j], w2 = w1 * P.s[j];
-                double v[45];
+                // node sums over lanes 32..39, one entry at a time (few live
+                // registers); lane 32 stores Sb, Gb and the N sums as L^-1 Nm L^-T
+                double *ms = sSt + ql * MW;
+                const double c12 = 1.0 / (mu.a11 * mu.a22), c22 = 1.0 / mu.d2;
 #pragma unroll
                 for (int i = 0; i < 9; i++) {
-                    v[i] = w0 * st.S[i];
-                    v[9 + i] = w0 * st.G[i];
-                    v[18 + i] = w0 * st.N[i];
-                    v[27 + i] = w1 * st.N[i];
-                    v[36 + i] = w2 * st.N[i];
-                }
+                    double sv = w0 * st.S[i], gv = w0 * st.G[i];
+                    double n0 = w0 * st.N[i], n1 = w1 * st.N[i], n2 = w2 * st.N[i];
 #pragma unroll
-                for (int i = 0; i < 45; i++) {
-                    v[i] += __shfl_xor(v[i], 1, 64);
-                    v[i] += __shfl_xor(v[i], 2, 64);
-                    v[i] += __shfl_xor(v[i], 4, 64);
-                }
-                if (j == 0) {
-                    // Sb, Gb as summed; the N node sums as L^-1 Nm L^-T
-                    double *ms = sSt + ql * MW;
-#pragma unroll
-                    for (int i = 0; i < 18; i++) ms[MS + i] = v[i];
-                    const double c12 = 1.0 / (mu.a11 * mu.a22), c22 = 1.0 / mu.d2;
-#pragma unroll
-                    for (int i = 0; i < 9; i++) {
-                        const double n0 = v[18 + i], n1 = v[27 + i], n2 = v[36 + i];
+                    for (int o = 1; o < 8; o <<= 1) {
+                        sv += __shfl_xor(sv, o, 64);
+                        gv += __shfl_xor(gv, o, 64);
+                        n0 += __shfl_xor(n0, o, 64);
+                        n1 += __shfl_xor(n1, o, 64);
+                        n2 += __shfl_xor(n2, o, 64);
+                    }
+                    if (j == 0) {
+                        ms[MS + i] = sv;
+                        ms[MG + i] = gv;
                         ms[MN11 + i] = n0 / mu.m0;
                         ms[MN12 + i] = (n1 - mu.lf * n0) * c12;
                         ms[MN22 + i] = ((mu.lf * n0 - n1) * mu.lf + (n2 - mu.lf * n1)) * c22;
@@ -775,7 +823,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                 // F^ and H^ (two components each) of the midpoint m0 = (u1+u0)/2
                 // and the increment d = (u-u0)/2: X^_1 = a11 (X0 + lf Xd),
                 // X^_2 = a22 Xd, for the 9 entries of F and the 27 of H
-                const double *u = sJet + q * 30 * NH;
+                const double *u = sJet + q * 30 * NL;
                 double *ms = sSt + ql * MW;
                 for (int i = b - 40; i < 36; i += 24) {
                     const bool isF = i < 9;
@@ -783,8 +831,8 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
                     int slot;
                     if (isF) slot = 10 * (r / 3) + 1 + (r % 3);
                     else slot = 10 * (r / 9) + sidx((r % 9) / 3, r % 3);
-                    double x0 = 0.5 * (u[30 + slot] + u[60 + slot]);
-                    const double xd = 0.5 * (u[slot] - u[60 + slot]);
+                    double x0 = u[slot];
+                    const double xd = u[30 + slot];
                     if (isF && (r / 3) == (r % 3)) x0 += 1.0;
                     const double x1 = mu.a11 * (x0 + mu.lf * xd), x2 = mu.a22 * xd;
                     if (isF) { ms[MF1 + r] = x1; ms[MF2 + r] = x2; }
@@ -799,7 +847,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 2 : 1)) void k_tangent_elem(MeshK M
             const int ql = wv, q = q0 + ql;
             if (lane < 6 * NA) {
                 const int p = lane / NA, ar = lane - p * NA, k = kBlkI[p], l = kBlkJ[p];
-                const double wgt = sGeo[q * GEO];
+                const double wgt = sGeo[q * GEOK];
                 double row[NA];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
