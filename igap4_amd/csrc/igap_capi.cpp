@@ -485,19 +485,25 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
 extern "C" int igap_gpu_destroy(igap_gpu_ctx *c)
 {
     if (!c) return IGAP_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (int d = 0; d < 3; d++) hipFree(c->d_knot[d]);
+    // release everything even after a failure; report the first one
+    int rc = IGAP_OK;
+    auto note = [&rc](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == IGAP_OK) rc = fail(IGAP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    note(hipSetDevice(c->device), "hipSetDevice");
+    if (c->stream) note(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (int d = 0; d < 3; d++)
+        if (c->d_knot[d]) note(hipFree(c->d_knot[d]), "hipFree");
     void *bufs[] = {c->d_w, c->d_X, c->d_Uh, c->d_R, c->d_Re, c->d_Ke[0], c->d_Ke[1], c->d_rowptr, c->d_col, c->d_pcol, c->d_val64,
                     c->d_val32, c->d_mask, c->d_g, c->d_F, c->d_Hd, c->d_H};
     for (void *b : bufs)
-        if (b) hipFree(b);
+        if (b) note(hipFree(b), "hipFree");
     if (c->solver) solver_destroy(c->solver);
     for (int i = 0; i < 4; i++)
-        if (c->ev[i]) hipEventDestroy(c->ev[i]);
-    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+        if (c->ev[i]) note(hipEventDestroy(c->ev[i]), "hipEventDestroy");
+    if (c->stream && c->own_stream) note(hipStreamDestroy(c->stream), "hipStreamDestroy");
     delete c;
-    return IGAP_OK;
+    return rc;
 }
 
 extern "C" int igap_gpu_set_bc(igap_gpu_ctx *c, const igap_bc_desc *bc)
@@ -610,9 +616,10 @@ extern "C" float *igap_gpu_csr_val32(igap_gpu_ctx *c)
 }
 
 // ---------------------------------------------------------------- timing helpers
-static void t_begin(igap_gpu_ctx *c, int slot)
+static int t_begin(igap_gpu_ctx *c, int slot)
 {
-    if (c->timing) hipEventRecord(c->ev[slot], c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[slot], c->stream));
+    return IGAP_OK;
 }
 static int t_end(igap_gpu_ctx *c, int slot, int which)
 {
@@ -632,11 +639,12 @@ extern "C" int igap_gpu_residual(igap_gpu_ctx *c, const double *U, double *R)
     if (!c) return fail(IGAP_ERR_ARG, "null context");
     HIPCHK(hipSetDevice(c->device));
     if (!R) R = c->d_R;
-    if (c->timing) hipEventRecord(c->ev[2], c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (U && U != c->d_Uh) HIPCHK(launch_copy_f64(c->d_Uh, U, c->ndof, c->stream));   // U_hist[0] = U (:68)
-    t_begin(c, 0);
+    int rc = t_begin(c, 0);
+    if (rc) return rc;
     HIPCHK(launch_residual_elem(c->M, c->P, c->d_Uh, c->ndof, c->d_Re, (int)c->nel, c->stream));
-    int rc = t_end(c, 0, 0);
+    rc = t_end(c, 0, 0);
     if (rc) return rc;
     HIPCHK(launch_residual_gather(c->M, c->d_Re, c->d_Uh, c->d_F, c->d_mask, c->d_g, R, c->ncp, c->stream));
     if (c->timing) {
@@ -664,13 +672,15 @@ extern "C" int igap_gpu_tangent(igap_gpu_ctx *c, const double *U, void *vals, in
         hipError_t e = hipMalloc(&c->d_Ke[prec], es * 6561 * (size_t)c->nel);
         if (e != hipSuccess) return fail(IGAP_ERR_NOMEM, "element tangent buffer (%zu bytes): %s", es * 6561 * (size_t)c->nel, hipGetErrorString(e));
     }
-    if (c->timing) hipEventRecord(c->ev[2], c->stream);
+    if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
     if (U && U != c->d_Uh) HIPCHK(launch_copy_f64(c->d_Uh, U, c->ndof, c->stream));   // U_hist[0] = U (:557)
-    t_begin(c, 0);
-    HIPCHK(launch_tangent_elem(c->M, c->P, c->d_Uh, c->ndof, c->d_Ke[prec], prec, (int)c->nel, c->stream));
-    int rc = t_end(c, 0, 1);
+    int rc = t_begin(c, 0);
     if (rc) return rc;
-    t_begin(c, 0);
+    HIPCHK(launch_tangent_elem(c->M, c->P, c->d_Uh, c->ndof, c->d_Ke[prec], prec, (int)c->nel, c->stream));
+    rc = t_end(c, 0, 1);
+    if (rc) return rc;
+    rc = t_begin(c, 0);
+    if (rc) return rc;
     HIPCHK(launch_tangent_gather(c->M, c->d_Ke[prec], prec, c->d_rowptr, c->d_mask, vals, c->ncp, c->stream));
     rc = t_end(c, 0, 2);
     if (rc) return rc;
@@ -758,13 +768,16 @@ extern "C" int igap_gpu_selftest_mfma(double *out64, float *out32)
     if (!out64 || !out32) return fail(IGAP_ERR_ARG, "null argument");
     double *d64 = nullptr;
     float *d32 = nullptr;
-    HIPCHK(hipMalloc(&d64, 256 * sizeof(double)));
-    HIPCHK(hipMalloc(&d32, 256 * sizeof(float)));
-    HIPCHK(launch_selftest_mfma(d64, d32, 0));
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(out64, d64, 256 * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(out32, d32, 256 * sizeof(float), hipMemcpyDeviceToHost));
-    hipFree(d64);
-    hipFree(d32);
+    // one status for the whole sequence so that both buffers are freed on
+    // every path
+    hipError_t e = hipMalloc(&d64, 256 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d32, 256 * sizeof(float));
+    if (e == hipSuccess) e = launch_selftest_mfma(d64, d32, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out64, d64, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out32, d32, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    const hipError_t f64 = d64 ? hipFree(d64) : hipSuccess, f32 = d32 ? hipFree(d32) : hipSuccess;
+    if (e == hipSuccess) e = (f64 != hipSuccess) ? f64 : f32;
+    if (e != hipSuccess) return fail(IGAP_ERR_HIP, "MFMA self test: %s", hipGetErrorString(e));
     return IGAP_OK;
 }
