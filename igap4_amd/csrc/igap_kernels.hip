@@ -24,35 +24,6 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // ============================================================================
-// butterfly reduce-scatter across the 64 lanes of a wave: each step halves the
-// per-lane vector (lanes with the step's bit set keep the upper half), so
-// summing N values over 64 lanes costs ~N shuffles instead of 6N.
-// ============================================================================
-template <int N>
-__device__ __forceinline__ void rs_step(const double *in, double *out, int mask, bool up)
-{
-    constexpr int N1 = (N + 1) / 2, N2 = N - N1;
-#pragma unroll
-    for (int i = 0; i < N1; i++) {
-        double hi = (i < N2) ? in[N1 + i] : 0.0;
-        double send = up ? in[i] : hi;
-        double recv = __shfl_xor(send, mask, 64);
-        out[i] = (up ? hi : in[i]) + recv;
-    }
-}
-
-__device__ __forceinline__ void rs_track(int N, bool up, int &base, int &cnt)
-{
-    int N1 = (N + 1) / 2;
-    if (up) {
-        base += N1;
-        cnt = cnt - N1 > 0 ? cnt - N1 : 0;
-    } else {
-        cnt = cnt < N1 ? cnt : N1;
-    }
-}
-
-// ============================================================================
 // residual: one workgroup (4 waves) per element.
 //   A  the Gauss-point fields by sum factorisation (as in the tangent):
 //      geometry, the physical u-jets of every history level -> LDS
