@@ -317,16 +317,6 @@ __device__ __forceinline__ void sf_jet(const double *Bq, const double *tx, doubl
         J[a] = Bq[S6[a]] * tx[3 * DX[a]] + Bq[96 + S6[a]] * tx[3 * DX[a] + 1] + Bq[192 + S6[a]] * tx[3 * DX[a] + 2];
 }
 
-__device__ __forceinline__ void add_periodic(const MeshK &M, const double *xq, double *u)
-{
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        u[10 * k] += M.par_periodic[3 * k] * xq[0] + M.par_periodic[3 * k + 1] * xq[1] + M.par_periodic[3 * k + 2] * xq[2];
-#pragma unroll
-        for (int d = 0; d < 3; d++) u[10 * k + 1 + d] += M.par_periodic[3 * k + d];
-    }
-}
-
 // gradelast_residual: r = dPsi/du at the jet u[30]
 __device__ __forceinline__ void phys_residual_static(const PhysK &P, const double *u, double *r)
 {
@@ -335,28 +325,6 @@ __device__ __forceinline__ void phys_residual_static(const PhysK &P, const doubl
     State st;
     phys_state(u, P, st);
     phys_residual_acc(st, 1.0, r);
-}
-
-// gradelasttime_residual from the midpoint jet m0 = (u0+u1)/2, the increment
-// jet d = (u-u0)/2 and the value slots uv = {u, u1, u0}[k]:
-//   value rows  cii (u-u0)/(2dt) + rho (u+u0-2u1)/dt^2     (:4128,4138,4148)
-//   other rows  sum_j cr_j dPsi(m0 + s_j d)  (the truncated series of :4129-4158)
-__device__ __forceinline__ void phys_residual_transient(const PhysK &P, const double *m0, const double *dl, const double *uv,
-                                                        double *r)
-{
-#pragma unroll
-    for (int i = 0; i < 30; i++) r[i] = 0.0;
-    for (int g = 0; g < 8; g++) {
-        double us[30];
-#pragma unroll
-        for (int i = 0; i < 30; i++) us[i] = m0[i] + P.s[g] * dl[i];
-        State st;
-        phys_state(us, P, st);
-        phys_residual_acc(st, P.cr[g], r);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-        r[10 * k] = (P.cii * (uv[k] - uv[6 + k])) / (2. * P.dt) + (P.rho * (uv[k] + uv[6 + k] - 2 * uv[3 + k])) / (P.dt * P.dt);
 }
 
 // ---------------------------------------------------------------- densities
@@ -437,55 +405,8 @@ __device__ __forceinline__ void phys_density_transient(const double *J, const Ph
 }
 
 // ---------------------------------------------------------------- tangent entries
-// Entries of d2Psi/du2 at one state, by block type.  M = I - 11^T/3.
-__device__ __forceinline__ double Mdev(int a, int b) { return (a == b ? 2.0 : -1.0) / 3.0; }
-
 // The state as stored in LDS: F[0..8], H[9..35], S[36..44], G[45..53], N[54..62]
 constexpr int ST_F = 0, ST_H = 9, ST_S = 36, ST_G = 45, ST_N = 54, STW = 63;
-
-// first-gradient (k,L) x first-gradient (l,M)
-__device__ __forceinline__ double c_ff(const double *st, const PhysK &P, int k, int L, int l, int M)
-{
-    const double *F = st + ST_F, *H = st + ST_H;
-    double v = st[ST_N + 3 * L + M] * F[3 * k + L] * F[3 * l + M];
-    if (L == M) {
-        double t = F[3 * k] * F[3 * l] + F[3 * k + 1] * F[3 * l + 1] + F[3 * k + 2] * F[3 * l + 2] - F[3 * k + L] * F[3 * l + L];
-        v += 0.5 * P.e345 * t;
-    } else {
-        v += 0.5 * P.e345 * F[3 * k + M] * F[3 * l + L];
-    }
-    double hh = H[9 * k + 3 * L] * H[9 * l + 3 * M] + H[9 * k + 3 * L + 1] * H[9 * l + 3 * M + 1] + H[9 * k + 3 * L + 2] * H[9 * l + 3 * M + 2];
-    v += 2 * P.le * P.le * Mdev(L, M) * hh;
-    if (k == l) v += st[ST_S + 3 * L + M];
-    return v;
-}
-
-// first-gradient (k,L) x second-gradient (l,(I,K))
-__device__ __forceinline__ double c_fh(const double *st, const PhysK &P, int k, int L, int l, int I, int K)
-{
-    const double *F = st + ST_F, *H = st + ST_H, *G = st + ST_G;
-    double g2 = 2 * P.le * P.le;
-    double v = g2 * Mdev(L, I) * H[9 * k + 3 * L + K] * F[3 * l + I];
-    if (I != K) v += g2 * Mdev(L, K) * H[9 * k + 3 * L + I] * F[3 * l + K];
-    if (k == l) {
-        if (L == I) v += G[3 * I + K];
-        if (I != K && L == K) v += G[3 * K + I];
-    }
-    return v;
-}
-
-// second-gradient (k,(I,K)) x second-gradient (l,(J,K2)): each slot carries
-// the (component, gradient direction) pairs (I,K) and, when I != K, (K,I)
-__device__ __forceinline__ double c_hh(const double *st, const PhysK &P, int k, int I, int K, int l, int J, int K2)
-{
-    const double *F = st + ST_F;
-    double v = 0;
-    if (K == K2) v += Mdev(I, J) * F[3 * k + I] * F[3 * l + J];
-    if (J != K2 && K == J) v += Mdev(I, K2) * F[3 * k + I] * F[3 * l + K2];
-    if (I != K && I == K2) v += Mdev(K, J) * F[3 * k + K] * F[3 * l + J];
-    if (I != K && J != K2 && I == J) v += Mdev(K, K2) * F[3 * k + K] * F[3 * l + K2];
-    return 2 * P.le * P.le * v;
-}
 
 // One row of the (k,l) block of C at one state, accumulated as row[b - A0] +=
 // s * C[(k,a),(l,b)] for the slots b = A0..9.  The row slot a (1..9) is the

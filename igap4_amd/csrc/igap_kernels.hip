@@ -474,7 +474,6 @@ template <> struct Mfma<float> {
 __constant__ const int kBlkI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kBlkJ[6] = {0, 1, 2, 1, 2, 2};
 
-__device__ __forceinline__ int blk_index(int k, int l) { return k == 0 ? l : (k == 1 ? 2 + l : 5); }   // k <= l
 
 template <typename T, int NH>
 __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,

@@ -513,14 +513,6 @@ __global__ void k_cheb_first(int64_t ncp, const double *__restrict__ Dinv, const
     }
 }
 
-// x[i] = 0 where mask[i] != 0 (Dirichlet rows)
-__global__ void k_zero_masked(int64_t n, const int32_t *__restrict__ mask, double *__restrict__ x)
-{
-    const int64_t st = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st)
-        if (mask[i]) x[i] = 0.0;
-}
-
 // Inverses of the 3x3 diagonal blocks (one per control point) of a CSR
 // matrix whose rows 3p..3p+2 hold columns 3p..3p+2 (a singular block gets
 // the identity)
