@@ -95,12 +95,18 @@ __device__ __forceinline__ void sf_zy(const double (*sT)[4][3][3], const double 
     __syncthreads();
 }
 
+// fields per sum-factorisation pass of gp_fields: static all 7 at once;
+// transient the 9 history fields after the 4 geometry fields
+template <int NH>
+constexpr int gp_fields_per_pass() { return NH == 1 ? 4 + 3 * NH : 3 * NH; }
+
 template <int NH>
 __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restrict__ Uh, int64_t ndof, int e,
                                           double (*sT)[4][3][3], double *sCP, double *sWork, double *sJet, double *sGeo)
 {
     constexpr int STR = 4 + 3 * NH;
     constexpr int NF = 4 + 3 * NH;
+    constexpr int NFP = gp_fields_per_pass<NH>();
     constexpr int GEO_R = 38;
     const int tid = threadIdx.x;
     const int nz = M.nelem[2], ny = M.nelem[1];
@@ -129,10 +135,12 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
     }
     __syncthreads();
 
-    // fields at the Gauss points (z, then y, then x)
+    // fields at the Gauss points (z, then y, then x).  Transient: two passes,
+    // geometry (w, w X) then the history levels (w U), so that the buffers
+    // hold 9 fields instead of 13
     {
-        double *sA = sWork, *sB = sWork + NF * 108;
-        sf_zy<NF, STR>(sT, sCP, sA, sB);
+        double *sA = sWork, *sB = sWork + NFP * 108;
+        sf_zy<(NH == 1 ? NF : 4), STR>(sT, sCP, sA, sB);
 
         const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
         const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
@@ -143,10 +151,13 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int f = (part == 0) ? 1 + j : 4 + 3 * j + part - 1;
-            if (part == 0 || j < NH) {
+            if (part == 0 || (NH == 1 && j < NH)) {
                 double Pj[10];
                 sf_jet(sB + 288 * f + boff, tx, Pj);
                 nurbs_param(Pj, 1.0, W, Jg[j]);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 10; a++) Jg[j][a] = (a >= 1 && a <= 3 && a - 1 == j) ? 1.0 : 0.0;
             }
         }
         double G[9], Gi[9], X2[18];
@@ -161,6 +172,18 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = __shfl(Jg[d][4 + s2], lq, 64);
+        if (NH != 1) {
+            __syncthreads();
+            sf_zy<3 * NH, STR, 4>(sT, sCP, sA, sB);
+            if (part > 0) {
+#pragma unroll
+                for (int h = 0; h < NH; h++) {
+                    double Pj[10];
+                    sf_jet(sB + 288 * (3 * h + part - 1) + boff, tx, Pj);
+                    nurbs_param(Pj, 1.0, W, Jg[h]);
+                }
+            }
+        }
         if (part > 0) {
             const int k = part - 1;
             double xq[3];
@@ -198,12 +221,12 @@ __device__ __forceinline__ void gp_fields(const MeshK &M, const double *__restri
 }
 
 template <int NH>
-__global__ __launch_bounds__(256, (NH == 1 ? 4 : 1)) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+__global__ __launch_bounds__(256, (NH == 1 ? 4 : 2)) void k_residual_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                        double *__restrict__ Re, int nel)
 {
     constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
     constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
-    constexpr int WA = NF * (108 + 288);       // stage-A buffers
+    constexpr int WA = gp_fields_per_pass<NH>() * (108 + 288);   // stage-A buffers
     constexpr int GEO_R = 38;                  // wgt, Gi[9], X2[18], W[10]
     constexpr int WG = 64 * GEO_R;             // the Gauss-point geometry (after stage A)
     constexpr int WT = 864 + 324;              // x^T and y^T buffers (after the adjoint; rho lives in sJet)
@@ -333,7 +356,7 @@ __global__ __launch_bounds__(256, (NH == 1 ? 4 : 1)) void k_residual_elem(MeshK 
 // sums over the 64 lanes
 // ============================================================================
 template <int NH>
-__global__ __launch_bounds__(256) void k_density_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+__global__ __launch_bounds__(256, (NH == 1 ? 1 : 2)) void k_density_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                       double *__restrict__ Hd, int nel)
 {
     constexpr int STR = 4 + 3 * NH;
@@ -342,7 +365,7 @@ __global__ __launch_bounds__(256) void k_density_elem(MeshK M, PhysK P, const do
     constexpr int ND = (NH == 1) ? ND_STATIC : ND_TRANSIENT;
     __shared__ double sT[3][4][3][3];
     __shared__ double sCP[27 * STR];
-    constexpr int WA = NF * (108 + 288), WG = 64 * GEO_R;
+    constexpr int WA = gp_fields_per_pass<NH>() * (108 + 288), WG = 64 * GEO_R;
     __shared__ double sWork[WA > WG ? WA : WG];   // stage A, then sGeo
     __shared__ double sJet[64 * 30 * NH];
     double *sGeo = sWork;
