@@ -65,3 +65,37 @@ def test_bad_descriptor_is_an_error_not_a_crash(igap):
     d.porder = 3
     rc = _lib.lib().igap_gpu_create(C.byref(d), C.byref(ph), C.byref(bc), -1, 0, C.byref(ctx))
     assert rc == 1 and b"porder" in _lib.lib().igap_gpu_last_error()
+
+
+def test_null_context_is_an_error_not_a_crash(igap):
+    """Every entry point that takes a context rejects a null one on the host,
+    with IGAP_ERR_ARG and a message, before any device work; destroying a
+    null context is a no-op, as free(NULL)."""
+    from igap4_amd import _lib
+    L = _lib.lib()
+    d, f = C.c_double(), C.c_float()
+    i, i64 = C.c_int(), C.c_int64()
+    calls = {
+        "igap_gpu_residual": (None, None, None),
+        "igap_gpu_tangent": (None, None, None, 0),
+        "igap_gpu_set_bc": (None, None),
+        "igap_gpu_sizes": (None, None, None, None, None),
+        "igap_gpu_intdensity": (None, C.byref(d), C.byref(i)),
+        "igap_gpu_save_solution": (None, b"/nonexistent/x.bin"),
+        "igap_gpu_load_solution": (None, b"/nonexistent/x.bin"),
+        "igap_gpu_synchronize": (None,),
+        "igap_gpu_timing_enable": (None, 1),
+        "igap_gpu_timing_get": (None, 0, C.byref(d), C.byref(i64)),
+        "igap_gpu_timing_reset": (None,),
+        "igap_gpu_snes_solve": (None, None, None),
+        "igap_gpu_selftest_ilu": (None, 0, C.byref(i), C.byref(d)),
+        "igap_ws_nparams": (None, C.byref(i64)),
+        "igap_ws_forward": (None, None, None),
+        "igap_ws_train_step": (None, None, None, 0.0, C.byref(d)),
+    }
+    for name, args in calls.items():
+        rc = getattr(L, name)(*args)
+        assert rc == 1, name      # IGAP_ERR_ARG
+        assert L.igap_gpu_last_error(), name
+    assert L.igap_gpu_destroy(None) == _lib.IGAP_OK
+    assert L.igap_ws_destroy(None) == _lib.IGAP_OK
