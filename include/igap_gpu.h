@@ -100,14 +100,21 @@ int igap_mesh_igabeam(const double L[3], const int nelem[3], const int periodic[
  * device.  flags: reserved, pass 0. */
 int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc *phys, const igap_bc_desc *bc,
                     int device, int flags, igap_gpu_ctx **ctx);
+/* IGAPFinalize (src/interface.c:372-435) for the context: frees everything
+ * even after a failure and returns the first failure; NULL is a no-op. */
 int igap_gpu_destroy(igap_gpu_ctx *ctx);
+/* Message of the last failing call of this thread (no reference
+ * counterpart: the reference asserts or exits instead of returning). */
 const char *igap_gpu_last_error(void);
 
-/* Replace the boundary conditions (e.g. the Dirichlet load factor of a load
- * step) without rebuilding the context. */
+/* Replace the boundary conditions without rebuilding the context: what a
+ * reference driver does by changing bc->par_dirichlet (the load factor of a
+ * load step) between IGAPSolvSolve calls (src/interface.c:302-328). */
 int igap_gpu_set_bc(igap_gpu_ctx *ctx, const igap_bc_desc *bc);
 
 /* ---- sizes and device buffers owned by the context ---------------------- */
+/* core->nDof, the nonzeros of the setnz_iga pattern (src/assmcore_iga.c:
+ * 1054-1087), the element count and phys->torder */
 int igap_gpu_sizes(const igap_gpu_ctx *ctx, int64_t *ndof, int64_t *nnz, int64_t *nelem, int *torder);
 /* core->U_hist (include/coreimpl.h:17): (torder+1)*ndof doubles, level h at
  * offset h*ndof, 0 = current iterate (src/interface.c:127). */
@@ -128,13 +135,16 @@ int igap_gpu_intdensity(igap_gpu_ctx *ctx, double *HH, int *ndensity);
  * missing file is an error, not an abort. */
 int igap_gpu_save_solution(igap_gpu_ctx *ctx, const char *fname);
 int igap_gpu_load_solution(igap_gpu_ctx *ctx, const char *fname);
-/* device CSR pattern (rows = owned dofs dof = 3*cp+i, columns sorted) */
+/* device CSR pattern of the AIJ matrix IGAPAssmInit preallocates
+ * (src/interface.c:187-198): rows = owned dofs 3*cp+i, columns sorted */
 const int64_t *igap_gpu_csr_rowptr(const igap_gpu_ctx *ctx);
 const int32_t *igap_gpu_csr_col(const igap_gpu_ctx *ctx);
-/* matrix values owned by the context (FP64 and FP32 arrays of nnz entries) */
+/* values of that matrix (assm->TT) owned by the context, FP64 and FP32
+ * arrays of nnz entries, allocated on first use */
 double *igap_gpu_csr_val64(igap_gpu_ctx *ctx);
 float *igap_gpu_csr_val32(igap_gpu_ctx *ctx);
-/* Dirichlet row mask (int32, 1 = constrained) and inserted values g */
+/* Dirichlet row mask (int32, 1 = constrained) and the values g that
+ * Residual_iga inserts as U_d - g_d (dirichlet_iga, src/assmcore_iga.c:861-934) */
 const int32_t *igap_gpu_dirichlet_mask(const igap_gpu_ctx *ctx);
 const double *igap_gpu_dirichlet_values(const igap_gpu_ctx *ctx);
 
@@ -143,6 +153,7 @@ const double *igap_gpu_dirichlet_values(const igap_gpu_ctx *ctx);
  * R = volume integral - Neumann load, Dirichlet rows R_d = U_d - g_d.
  * R may be NULL to write the context's own R buffer (igap_gpu_rvec). */
 int igap_gpu_residual(igap_gpu_ctx *ctx, const double *U, double *R);
+/* the context's residual vector (assm->RR, src/interface.c:187-198) */
 double *igap_gpu_rvec(igap_gpu_ctx *ctx);
 /* Tangent_iga: copies U into U_hist level 0 (U may be NULL to use U_hist as
  * it is), zeroes the values, assembles the volume tangent deterministically
@@ -157,8 +168,9 @@ int igap_gpu_tangent(igap_gpu_ctx *ctx, const double *U, void *vals, int prec);
 /* IGAP_PC_MG (no counterpart in the reference, which uses ASM/ILU(0)):
  * geometric multigrid V-cycle on the patch's B-spline hierarchy (every even
  * direction halved per level; exact knot-insertion prolongation; coarse
- * tangents re-assembled at the interpolated state; ILU(0) smoothing in the
- * pc_order ordering).  Non-periodic patches with unit weights only. */
+ * tangents re-assembled at the interpolated state; degree-3 Chebyshev
+ * smoothing with 3x3 point-block Jacobi; the coarsest level applied as its
+ * explicit inverse).  Non-periodic patches with unit weights only. */
 enum { IGAP_PC_ILU0 = 0, IGAP_PC_JACOBI = 1, IGAP_PC_NONE = 2, IGAP_PC_MG = 3 };
 
 typedef struct {
@@ -198,7 +210,9 @@ int igap_gpu_solver_default_opts(igap_solver_opts *opts);
  * the last iterate and the context's R buffer its residual.  Synchronous. */
 int igap_gpu_snes_solve(igap_gpu_ctx *ctx, const igap_solver_opts *opts, igap_solver_stats *stats);
 
-/* ---- helpers ------------------------------------------------------------ */
+/* ---- helpers (no reference counterpart: device memory for callers that
+ * hold no HIP runtime of their own; all on the context's device and stream,
+ * the copies synchronous) -------------------------------------------------- */
 int igap_gpu_synchronize(igap_gpu_ctx *ctx);
 int igap_gpu_malloc(igap_gpu_ctx *ctx, void **ptr, int64_t bytes);
 int igap_gpu_free(igap_gpu_ctx *ctx, void *ptr);
@@ -206,15 +220,17 @@ int igap_gpu_memcpy_htod(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t 
 int igap_gpu_memcpy_dtoh(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
 int igap_gpu_memcpy_dtod(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
 
-/* Timing of the assembly kernels with HIP events recorded on the context's
- * stream around each launch.  which: 0 residual volume kernel, 1 tangent
+/* (No reference counterpart.)  Timing of the assembly kernels with HIP
+ * events recorded on the context's stream around each launch.
+ * which: 0 residual volume kernel, 1 tangent
  * volume kernel, 2 CSR gather kernel, 3 whole residual call, 4 whole tangent
  * call.  Accumulated milliseconds and launch counts since the last reset. */
 int igap_gpu_timing_enable(igap_gpu_ctx *ctx, int on);
 int igap_gpu_timing_get(igap_gpu_ctx *ctx, int which, double *ms_total, int64_t *count);
 int igap_gpu_timing_reset(igap_gpu_ctx *ctx);
 
-/* Self test of the matrix-core operand layout the tangent kernels assume:
+/* (No reference counterpart.)  Self test of the matrix-core operand layout
+ * the tangent kernels assume:
  * one 16x16x4 product with V_MFMA_F64_16X16X4F64 and one with
  * V_MFMA_F32_16X16X4F32 of A[i][k] = 1 + i + k/4, B[k][j] = 2 - j/2 + k,
  * lane l supplying A[l&15][l>>4] and B[l>>4][l&15]; out64/out32 receive the
