@@ -11,14 +11,20 @@
 //     one reorthogonalisation (two batched GEMVs per step instead of j
 //     dependent dot products; m is bounded because 10000 Krylov vectors of
 //     the large patches do not fit),
-//   * ILU(0) of the assembled tangent (rocSPARSE csrilu0 + csrsv) in the
-//     tangent's precision, FP64 or FP32, in the natural dof order (the
-//     reference's) or in a 27-colour order of the control points (colour
-//     9 (i mod 3) + 3 (j mod 3) + (k mod 3), points of one colour share no
-//     stencil entry, the three dofs of a point stay together): the natural
-//     order's triangular solves have ~9 n_x levels of dependent rows, the
-//     coloured order's at most 81,
-//   * a wave-per-row CSR SpMV of this file.
+//   * ILU(0) of the assembled tangent in the tangent's precision (or FP32
+//     with pc_fp32), FP64 or FP32, in the natural dof order (the
+//     reference's; rocSPARSE csrilu0 + csrsv) or in a 27-colour order of the
+//     control points (colour 9 (i mod 3) + 3 (j mod 3) + (k mod 3), points of
+//     one colour share no stencil entry, the three dofs of a point stay
+//     together), factored and solved by this file's point-block kernels: the
+//     natural order's triangular solves have ~9 n_x levels of dependent rows,
+//     the coloured order's 27 colour sweeps,
+//   * or a geometric multigrid V-cycle on the patch's B-spline hierarchy
+//     (knot-insertion transfers, coarse tangents re-assembled by the same
+//     kernels, Chebyshev / point-block Jacobi smoothing, explicit inverse on
+//     the coarsest level),
+//   * a point-block SpMV (one wave per control point, a lane per neighbour
+//     point, the nine values of the 3x3 block per lane).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsparse/rocsparse.h>
