@@ -11,3 +11,7 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
     -lrocsparse -lrocblas -lrocsolver \
     -o "$HERE/libigap_gpu.so.tmp"
 mv "$HERE/libigap_gpu.so.tmp" "$HERE/libigap_gpu.so"
+# the plain-C host of the ABI (tests/c_host), linked against the library in place
+gcc -O2 -std=c99 -Wall -Wextra -I "$HERE/../include" "$HERE/../tests/c_host/igap_c_host.c" \
+    -L "$HERE" -ligap_gpu -Wl,-rpath,'$ORIGIN/../../igap4_amd' -Wl,-rpath-link,/opt/rocm/lib -lm \
+    -o "$HERE/../tests/c_host/igap_c_host"
