@@ -359,6 +359,15 @@ static int dalloc(T **p, int64_t n)
             return r_;               \
         }                            \
     } while (0)
+// a HIP call inside igap_gpu_create: release the partly built context c on failure
+#define HIPTRY(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) {                                                                      \
+            igap_gpu_destroy(c);                                                                     \
+            return fail(IGAP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));                          \
+        }                                                                                            \
+    } while (0)
 
 extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc *phys, const igap_bc_desc *bc, int device, int flags,
                                igap_gpu_ctx **out)
@@ -379,7 +388,7 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     if (device >= 0) HIPCHK(hipSetDevice(device));
 
     igap_gpu_ctx *c = new igap_gpu_ctx();
-    HIPCHK(hipGetDevice(&c->device));
+    HIPTRY(hipGetDevice(&c->device));
     c->torder = phys->kind;
     for (int d = 0; d < 3; d++) {
         c->nelem[d] = mesh->nelem[d];
@@ -427,12 +436,12 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     // mesh upload
     for (int d = 0; d < 3; d++) {
         TRY(dalloc(&c->d_knot[d], c->nelem[d] + 5));
-        HIPCHK(hipMemcpy(c->d_knot[d], c->knot_h[d].data(), sizeof(double) * (c->nelem[d] + 5), hipMemcpyHostToDevice));
+        HIPTRY(hipMemcpy(c->d_knot[d], c->knot_h[d].data(), sizeof(double) * (c->nelem[d] + 5), hipMemcpyHostToDevice));
     }
     TRY(dalloc(&c->d_w, c->ncp_active));
     TRY(dalloc(&c->d_X, 3 * c->ncp_active));
-    HIPCHK(hipMemcpy(c->d_w, mesh->w, sizeof(double) * c->ncp_active, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_X, mesh->X, sizeof(double) * 3 * c->ncp_active, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->d_w, mesh->w, sizeof(double) * c->ncp_active, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->d_X, mesh->X, sizeof(double) * 3 * c->ncp_active, hipMemcpyHostToDevice));
     c->w_h.assign(mesh->w, mesh->w + c->ncp_active);
     c->X_h.assign(mesh->X, mesh->X + 3 * c->ncp_active);
     c->phys_h = *phys;
@@ -464,8 +473,8 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     if (c->nnz > 0x7fffffffLL * 4) { igap_gpu_destroy(c); return fail(IGAP_ERR_ARG, "nnz %lld too large", (long long)c->nnz); }
     TRY(dalloc(&c->d_rowptr, c->ndof + 1));
     TRY(dalloc(&c->d_col, c->nnz));
-    HIPCHK(hipMemcpy(c->d_rowptr, rowptr.data(), sizeof(int64_t) * (c->ndof + 1), hipMemcpyHostToDevice));
-    HIPCHK(launch_csr_cols(M, c->d_rowptr, c->d_col, c->ncp, c->stream));
+    HIPTRY(hipMemcpy(c->d_rowptr, rowptr.data(), sizeof(int64_t) * (c->ndof + 1), hipMemcpyHostToDevice));
+    HIPTRY(launch_csr_cols(M, c->d_rowptr, c->d_col, c->ncp, c->stream));
 
     // state and BC buffers
     int nh = c->torder + 1;
@@ -475,9 +484,9 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     TRY(dalloc(&c->d_mask, c->ndof));
     TRY(dalloc(&c->d_g, c->ndof));
     TRY(dalloc(&c->d_F, c->ndof));
-    HIPCHK(launch_fill_f64(c->d_Uh, 0.0, nh * c->ndof, c->stream));
+    HIPTRY(launch_fill_f64(c->d_Uh, 0.0, nh * c->ndof, c->stream));
     TRY(setup_bc(c, bc));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
     *out = c;
     return IGAP_OK;
 }
