@@ -1,15 +1,21 @@
-"""Benchmark: elements assembled per second (residual + tangent) on one MI355X.
+"""Benchmark: elements assembled per second (residual + tangent) on one MI355X,
+and wall time per Newton load step.
 
-Workload (default): the 64x16x16-element quadratic C1 B-spline beam patch
-(L = 4:1:1, 21,384 control points, 64,152 dofs, 20.6M CSR nonzeros), static
-gradient elasticity with St Venant-Kirchhoff lambda = mu = 1 and strain-
-gradient length 0.1, clamped at x = 0 with a tip displacement on x = L,
-4x4x4 Gauss points.  A step is one full residual (volume + Neumann +
-Dirichlet rows) plus one full tangent (element tangents on the matrix cores,
-deterministic CSR gather, MatZeroRowsColumns) in FP64, on a smooth synthetic
-displacement resident in HBM.
+Workload (default): the largest single-GPU patch, 256x64x64 elements of the
+quadratic C1 B-spline beam (L = 4:1:1, 1,123,848 control points, 3,371,544
+dofs, 1.21G CSR nonzeros), static gradient elasticity with St Venant-Kirchhoff
+lambda = mu = 1 and strain-gradient length 0.1, clamped at x = 0 with a tip
+displacement on x = L, 4x4x4 Gauss points.  A step is one full residual
+(volume + Neumann + Dirichlet rows) plus one full tangent (element tangents on
+the matrix cores, deterministic CSR gather, MatZeroRowsColumns) in FP64, on a
+smooth synthetic displacement resident in HBM.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--nelem 64,16,16]
+The second measure is the wall time per Newton load step of the 40-step ramp
+to tip 0.1, solved to ||R|| <= 1e-10 ||R0|| (R0: the residual of the previous
+solution under the step's load), for several arms on the same steps and with
+the same Krylov stopping rule (the reference's GMRES rtol 1e-5).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--nelem 256,64,64]
 """
 from __future__ import annotations
 
@@ -39,11 +45,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nelem", default="64,16,16")
+    ap.add_argument("--nelem", default="256,64,64")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--newton-steps", type=int, default=3,
-                    help="load steps of the Newton ramp timed after the assembly bench (0: skip)")
+    ap.add_argument("--newton-steps", type=int, default=2,
+                    help="load steps of the Newton ramp timed per arm after the assembly bench (0: skip)")
+    ap.add_argument("--ramp-steps", type=int, default=40, help="load steps of the whole ramp")
+    ap.add_argument("--ramp-tip", type=float, default=0.1, help="tip displacement at the end of the ramp")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling run for rocprofv3 --pmc: one 256 MiB device copy (the HBM "
                          "counter calibration) then N residual+tangent calls; prints no JSON")
@@ -54,6 +64,11 @@ def parse():
 
 
 COPY_BYTES = 1 << 28   # the calibration copy of --probe
+
+
+def say(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(msg, file=sys.stderr, flush=True)
 
 
 def probe(nel, calls):
@@ -180,29 +195,76 @@ def kernel_rooflines(nel, nelem, nnz, avg, avg32):
     }
 
 
-def cpu_baseline(threads, nel, nx=None):
-    """The oracle (CPU restatement with the reference's loop nest and naive
-    81x81x100 contraction) on the same workload: one residual and one tangent
-    over the elements x < nx (default: all) of the same patch, physics and
-    state (about 14 s with 16 threads at 64x16x16); elements/s (R+K)."""
+def host_cpu():
+    """The host's CPU model, logical CPU count and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU this process may run on, capped
+    by OMP_NUM_THREADS where the host sets it (the CPU share of one GPU slot)."""
+    _, _, avail = host_cpu()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        return min(avail, int(omp))
+    return avail
+
+
+def cpu_baseline(threads, nel, target_s=15.0):
+    """The oracle (CPU restatement with the reference's loop nest, the naive
+    81x81x100 element contraction and a per-row column search for every
+    MatSetValues entry) on a slab of the same workload: one residual and one
+    tangent over the elements x < nx of the same patch, physics and state,
+    the tangent into the CSR rows of the control points those elements touch.
+    Volume integrals only: the boundary terms (face integrals of the whole
+    patch, Dirichlet rows) are a per-face cost that a slab does not scale.
+    nx is sized from a one-layer calibration run so that the sample takes
+    about target_s seconds; elements/s (R+K)."""
     from oracle import oracle as O
-    nx = nel[0] if nx is None else min(nx, nel[0])
     m = O.Mesh(nel, L=(4, 1, 1))
     ph = O.make_phys(0, coef=O.SVK_COEF)
     bc = O.clamped_tip_bc(tip=0.05)
     U = O.smooth_state(m, amp=0.01)
-    pat = O.csr_pattern(m)
-    box = [0, nx, 0, nel[1], 0, nel[2]]
-    t0 = time.perf_counter()
-    O.residual(m, ph, bc, U, box=box, nthreads=threads)
-    t1 = time.perf_counter()
-    O.tangent(m, ph, bc, U, pattern=pat, box=box, nthreads=threads)
-    t2 = time.perf_counter()
-    ns = nx * nel[1] * nel[2]
-    return {"value": ns / (t2 - t0), "unit": "elements/s", "cores": threads, "kind": "port",
+    per_layer = nel[1] * nel[2]
+
+    def run(nx):
+        rows = 3 * (nx + 2) * m.nb[1] * m.nb[2]
+        pat = O.csr_pattern(m, nrows=rows)
+        box = [0, nx, 0, nel[1], 0, nel[2]]
+        t0 = time.perf_counter()
+        O.residual(m, ph, bc, U, box=box, apply_bc=False, nthreads=threads)
+        t1 = time.perf_counter()
+        O.tangent(m, ph, bc, U, pattern=pat, box=box, apply_bc=False, nthreads=threads)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    tr, tk = run(1)
+    nx = 1
+    if tr + tk < 0.5 * target_s:
+        nx = int(max(1, min(nel[0], round(target_s / max(1e-3, tr + tk)))))
+        if nx > 1:
+            tr, tk = run(nx)
+    ns = nx * per_layer
+    model, ncpu, avail = host_cpu()
+    return {"value": ns / (tr + tk), "unit": "elements/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_logical_cpus": ncpu, "cpus_available": avail,
             "sample": f"one residual + one tangent over the {ns} elements x < {nx} of the {nel[0]}x{nel[1]}x{nel[2]} "
-                      f"patch (same physics and state, FP64, {threads} OpenMP threads): t_R = {t1 - t0:.2f} s, "
-                      f"t_K = {t2 - t1:.2f} s"}
+                      f"patch, volume integrals (same physics and state, FP64, {threads} OpenMP threads on {model}; "
+                      f"{avail} CPUs available to the process, {ncpu} on the host): t_R = {tr:.2f} s, "
+                      f"t_K = {tk:.2f} s"}
 
 
 def main():
@@ -228,6 +290,7 @@ def main():
     A = ig.Assm(mesh, phys, bc, device=0)
     A.set_uhist(U)
     nelem = A.nel
+    say(f"bench: context of {nelem} elements, {A.nnz} nonzeros")
 
     def step(prec):
         A.residual_dev(None, None)
@@ -269,7 +332,7 @@ def main():
     traffic = traffic_from_profiles(nel)
 
     line = {
-        "metric": "elements assembled/s (R+K)",
+        "metric": "elements assembled/s (R+K); wall time per Newton load step at ||R||/||R0||<=1e-10",
         "value": nelem / (ms64 * 1e-3),
         "unit": "elements/s",
         "n_gpus": 1,
@@ -283,7 +346,8 @@ def main():
         "data": "synthetic (smooth displacement field on the igabeam patch)",
         "config": {"workload": f"static gradient elasticity, {nel[0]}x{nel[1]}x{nel[2]}-element quadratic "
                                f"C1 B-spline beam 4:1:1, SVK lambda=mu=1 + gradient l=0.1, clamped + tip "
-                               f"Dirichlet, 4x4x4 Gauss points",
+                               f"Dirichlet, 4x4x4 Gauss points; load steps: tip {a.ramp_tip} in {a.ramp_steps} "
+                               f"steps",
                    "nelem": nelem, "ndof": A.ndof, "nnz": A.nnz},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
@@ -298,37 +362,119 @@ def main():
                          "peak": PEAK_FP32_TFLOPS},
     }
     A.close()
+    say(f"bench: R+K {line['value']:.4g} elements/s, {ms64:.1f} ms per step")
     if a.newton_steps > 0:
-        line["newton_load_step"] = newton_load_steps(nel, a.newton_steps)
+        line["newton_load_step"] = newton_load_steps(nel, a.newton_steps, nsteps=a.ramp_steps, tip=a.ramp_tip)
     if a.cpu_baseline:
-        th = a.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(th, nel)
+        th = a.cpu_threads or cpu_threads()
+        line["cpu_baseline"] = cpu_baseline(th, nel, target_s=a.cpu_seconds)
+        say(f"bench: cpu baseline {line['cpu_baseline']['value']:.4g} elements/s")
     print(json.dumps(line), flush=True)
 
 
-def newton_load_steps(nel, k, nsteps=20, tip=0.05):
+def newton_load_steps(nel, timed=2, nsteps=40, tip=0.1, ramp_in=2, fp32_maxit=15):
     """The second measure of the workload: wall time per Newton load step of
-    the clamped-tip ramp (tip 0.05 in 20 load steps) to ||R||/||R0|| <= 1e-10
-    from the previous solution (the reference's start), device Newton with
-    GMRES preconditioned by the geometric multigrid cycle (set up once per
-    load step: every step here takes 4 Newton iterations), FP64 tangent; the
-    first k steps of the ramp (the first includes one-time set-up)."""
+    the clamped-tip ramp (tip ``tip`` in ``nsteps`` load steps), every step
+    solved to ||R|| <= 1e-10 ||R0|| with R0 the residual of the previous
+    solution under the step's load (the first residual of the reference's
+    iteration).  After ``ramp_in`` steps, each of the next ``timed`` steps is
+    solved from the same converged history by every arm:
+
+      plain         the previous solution as the initial iterate (the
+                    reference's start), FP64 tangent and preconditioner
+      quadratic     the warm start without its network: quadratic
+                    extrapolation of the last three solutions, accepted
+                    against linear extrapolation by residual norm; FP64
+      fast          the same start, FP64 tangent, preconditioner in FP32
+      fp32_tangent  the same start, Newton on the FP32 matrix-core tangent
+                    (FP32 preconditioner), at most fp32_maxit iterations
+
+    Every arm uses the same Newton test and the same Krylov rule (GMRES(200)
+    to the reference's rtol 1e-5) and the same preconditioner (geometric
+    multigrid on the B-spline hierarchy, set up every 4th Newton iteration);
+    wall time includes building the guess.  The history continues from the
+    plain arm's solution.  Displacements of each arm are compared with the
+    plain arm's at every timed step."""
     import igap4_amd as ig
-    from igap4_amd.loadstep import run_load_steps
+    from igap4_amd.loadstep import run_load_steps, tip_bc
+    from igap4_amd.warmstart import LearnedGuess
+
     mesh = ig.MeshIGABeam(L=(4.0, 1.0, 1.0), nelem=nel)
+    common = dict(snes_rtol=0.0, snes_atol=1e-30, pc="mg", pc_order=1, pc_lag=4, ksp_rtol=1e-5)
+    arms = {
+        "plain": dict(tangent_prec="f64", pc_fp32=0),
+        "quadratic": dict(tangent_prec="f64", pc_fp32=0),
+        "fast": dict(tangent_prec="f64", pc_fp32=1),
+        "fp32_tangent": dict(tangent_prec="f32", pc_fp32=1, snes_maxit=fp32_maxit),
+    }
+    out = {"ramp": f"tip {tip} in {nsteps} load steps", "criterion": "||R|| <= 1e-10 ||R(U_prev; load_k)||",
+           "krylov": "GMRES(200), right preconditioned, rtol 1e-5 (every arm)",
+           "preconditioner": "geometric multigrid on the B-spline hierarchy (Chebyshev/point-block-Jacobi "
+                             "smoothing, dense coarsest inverse), set up every 4th Newton iteration",
+           "arms": {k: {"ms_per_step": [], "newton_its": [], "ksp_its": [], "converged": [],
+                        "final_rel_residual": [], "max_rel_diff_vs_plain": []} for k in arms}}
     with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.0)) as A:
-        A.set_uhist(np.zeros(A.ndof))
-        opts = ig.assm.default_solver_opts(snes_rtol=1e-10, snes_atol=1e-30, tangent_prec="f64", pc="mg", pc_order=1,
-                                           pc_lag=4)
-        st = run_load_steps(A, nsteps, tip_total=tip, opts=opts, max_steps=k)
-    w = [s["wall_ms"] for s in st]
-    return {"ms_per_step_median": float(np.median(w)), "ms_per_step": w, "steps_timed": len(st),
-            "newton_its": [s["newton_its"] for s in st], "ksp_its": [s["ksp_its"] for s in st],
-            "converged": bool(all(s["converged"] > 0 for s in st)),
-            "solver": "NEWTONLS+BT, GMRES(200) right-PC, geometric multigrid on the B-spline hierarchy "
-                      "(Chebyshev/point-block-Jacobi smoothing, dense coarsest LU) set up every 4th Newton "
-                      "iteration, FP64 tangent, ||R|| <= 1e-10 ||R0||",
-            "ramp": f"tip {tip} in {nsteps} load steps"}
+        n = A.ndof
+        A.set_uhist(np.zeros(n))
+        prev, G = [np.zeros(n)], [A.dirichlet()[1]]
+        opts = {k: ig.assm.default_solver_opts(**common, **v) for k, v in arms.items()}
+        ramp = []
+
+        def keep(k, u, st):
+            prev.append(u.copy())
+            G.append(A.dirichlet()[1])
+
+        t0 = time.perf_counter()
+        for k in range(1, ramp_in + 1):
+            st = run_load_steps(A, nsteps, tip_total=tip, guess="extrapolate", opts=opts["fast"], start=k,
+                                max_steps=k, prev=prev[-3:], start_rtol=1e-10, callback=keep)
+            ramp.append({"step": k, "ms": st[0]["wall_ms"], "newton_its": st[0]["newton_its"],
+                         "ksp_its": st[0]["ksp_its"], "converged": st[0]["converged"]})
+            say(f"bench: ramp-in step {k}: {st[0]['wall_ms']:.0f} ms, newton {st[0]['newton_its']}, "
+                f"ksp {st[0]['ksp_its']}")
+            if st[0]["converged"] <= 0:
+                out["ramp_in"] = ramp
+                out["error"] = f"ramp-in step {k} did not converge"
+                return out
+        out["ramp_in"] = ramp
+        out["ramp_in_s"] = time.perf_counter() - t0
+        for k in range(ramp_in + 1, ramp_in + timed + 1):
+            sols = {}
+            for name in arms:
+                if name == "plain":
+                    guess = "previous"
+                else:
+                    guess = LearnedGuess(A, mode="quadratic")
+                    guess.set_previous_load(G[-1])
+                got = []
+                st = run_load_steps(A, nsteps, tip_total=tip, guess=guess, opts=opts[name], start=k, max_steps=k,
+                                    prev=prev[-3:], start_rtol=1e-10,
+                                    callback=lambda kk, u, x: got.append(u.copy()))
+                x = st[0]
+                sols[name] = got[0] if got else A.get_uhist()[:n].copy()
+                r = out["arms"][name]
+                r["ms_per_step"].append(x["wall_ms"])
+                r["newton_its"].append(x["newton_its"])
+                r["ksp_its"].append(x["ksp_its"])
+                r["converged"].append(x["converged"])
+                r["final_rel_residual"].append(x["fnorm"] / x["r0_start"] if x["r0_start"] > 0 else 0.0)
+                say(f"bench: step {k} {name}: {x['wall_ms']:.0f} ms, newton {x['newton_its']}, ksp {x['ksp_its']}, "
+                    f"converged {x['converged']}")
+            ref = sols["plain"]
+            scale = max(1e-300, float(np.max(np.abs(ref))))
+            for name in arms:
+                out["arms"][name]["max_rel_diff_vs_plain"].append(float(np.max(np.abs(sols[name] - ref)) / scale))
+            prev.append(ref)
+            A.set_bc(tip_bc(tip, k / nsteps))
+            G.append(A.dirichlet()[1])
+    for name, r in out["arms"].items():
+        r["ms_per_step_median"] = float(np.median(r["ms_per_step"])) if r["ms_per_step"] else None
+        r["all_converged"] = bool(r["converged"] and all(c > 0 for c in r["converged"]))
+    p = out["arms"]["plain"]["ms_per_step_median"]
+    for name, r in out["arms"].items():
+        r["speedup_vs_plain"] = (p / r["ms_per_step_median"]) if p and r["ms_per_step_median"] else None
+    out["steps_timed"] = list(range(ramp_in + 1, ramp_in + timed + 1))
+    return out
 
 
 if __name__ == "__main__":

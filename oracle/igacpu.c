@@ -565,9 +565,32 @@ int64_t oc_csr_nnz(const oc_mesh *m)
     return 9 * tot[0] * tot[1] * tot[2];
 }
 
-void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col)
+/* the first nrows rows of the pattern (nrows < 0: all of them) */
+int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t nrows)
 {
-    int64_t r = 0, pos = 0;
+    int64_t nd = oc_ndof(m), tot = 0, r = 0;
+    if (nrows < 0 || nrows > nd) nrows = nd;
+    int s[5];
+    for (int ax = 0; ax < m->nb[0]; ax++) {
+        int nx = stencil1(m, 0, ax, s);
+        for (int ay = 0; ay < m->nb[1]; ay++) {
+            int ny = stencil1(m, 1, ay, s);
+            for (int az = 0; az < m->nb[2]; az++) {
+                int nz = stencil1(m, 2, az, s);
+                for (int i = 0; i < 3; i++, r++) {
+                    if (r >= nrows) return tot;
+                    tot += 3 * nx * ny * nz;
+                }
+            }
+        }
+    }
+    return tot;
+}
+
+void oc_csr_pattern_rows(const oc_mesh *m, int64_t nrows, int64_t *rowptr, int32_t *col)
+{
+    int64_t nd = oc_ndof(m), r = 0, pos = 0;
+    if (nrows < 0 || nrows > nd) nrows = nd;
     int sx[5], sy[5], sz[5];
     rowptr[0] = 0;
     for (int ax = 0; ax < m->nb[0]; ax++) {
@@ -577,6 +600,7 @@ void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col)
             for (int az = 0; az < m->nb[2]; az++) {
                 int nz = stencil1(m, 2, az, sz);
                 for (int i = 0; i < 3; i++) {
+                    if (r >= nrows) return;
                     for (int x = 0; x < nx; x++)
                         for (int y = 0; y < ny; y++)
                             for (int z = 0; z < nz; z++)
@@ -588,6 +612,8 @@ void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col)
         }
     }
 }
+
+void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col) { oc_csr_pattern_rows(m, -1, rowptr, col); }
 
 void oc_setnz(const oc_mesh *m, int32_t *d_nz)
 {
@@ -940,10 +966,22 @@ static int colour1(const oc_mesh *m, int d, int e)
     return e < reg ? e % 3 : 3 + (e - reg);
 }
 
-static int in_box(const int *box, int ex, int ey, int ez)
+/* the element indices of each direction that lie in the box and have the
+ * given colour: ls[d][0..nl[d]) (storage from el, nelem[0]+nelem[1]+nelem[2]) */
+static void box_colour_lists(const oc_mesh *m, const int *box, int cx, int cy, int cz, int *el, int *ls[3], int nl[3])
 {
-    if (!box) return 1;
-    return ex >= box[0] && ex < box[1] && ey >= box[2] && ey < box[3] && ez >= box[4] && ez < box[5];
+    const int cc[3] = {cx, cy, cz};
+    int off = 0;
+    for (int d = 0; d < 3; d++) {
+        int lo = box ? box[2 * d] : 0, hi = box ? box[2 * d + 1] : m->nelem[d];
+        if (lo < 0) lo = 0;
+        if (hi > m->nelem[d]) hi = m->nelem[d];
+        ls[d] = el + off;
+        nl[d] = 0;
+        for (int e = lo; e < hi; e++)
+            if (colour1(m, d, e) == cc[d]) ls[d][nl[d]++] = e;
+        off += m->nelem[d];
+    }
 }
 
 static void set_threads(int nthreads)
@@ -962,14 +1000,15 @@ void oc_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const dou
     set_threads(nthreads);
     for (int64_t i = 0; i < nd; i++) R[i] = 0.0;   /* VecSet(RR,0) :105 */
     int ncx = ncolour1(m, 0), ncy = ncolour1(m, 1), ncz = ncolour1(m, 2);
-    int64_t nel = (int64_t)m->nelem[0] * m->nelem[1] * m->nelem[2];
+    int *el = (int *)malloc(sizeof(int) * (m->nelem[0] + m->nelem[1] + m->nelem[2]));
     for (int col = 0; col < ncx * ncy * ncz; col++) {
         int cx = col / (ncy * ncz), cy = (col / ncz) % ncy, cz = col % ncz;
+        int *ls[3], nl[3];
+        box_colour_lists(m, box, cx, cy, cz, el, ls, nl);
+        int64_t nt = (int64_t)nl[0] * nl[1] * nl[2];
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t t = 0; t < nel; t++) {
-            int ex = (int)(t / ((int64_t)m->nelem[1] * m->nelem[2])), ey = (int)((t / m->nelem[2]) % m->nelem[1]), ez = (int)(t % m->nelem[2]);
-            if (colour1(m, 0, ex) != cx || colour1(m, 1, ey) != cy || colour1(m, 2, ez) != cz) continue;
-            if (!in_box(box, ex, ey, ez)) continue;
+        for (int64_t t = 0; t < nt; t++) {
+            int ex = ls[0][t / ((int64_t)nl[1] * nl[2])], ey = ls[1][(t / nl[2]) % nl[1]], ez = ls[2][t % nl[2]];
             double val[81];
             oc_element_residual(m, ph, bc, U_hist, ex, ey, ez, val);
             for (int bx = 0; bx < 3; bx++)
@@ -979,6 +1018,7 @@ void oc_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const dou
                             R[dof_of_active(m, ex + bx, ey + by, ez + bz, k)] += val[3 * (9 * bx + 3 * by + bz) + k];
         }
     }
+    free(el);
     if (!apply_bc) return;
     double *F = (double *)malloc(sizeof(double) * nd);
     oc_neumann(m, bc, F);
@@ -1006,20 +1046,22 @@ static int64_t find_col(const int64_t *rowptr, const int32_t *col, int64_t row, 
 
 void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist,
                 const int64_t *rowptr, const int32_t *col, double *val,
-                const int *box, int apply_bc, int nthreads)
+                const int *box, int apply_bc, int nthreads, int64_t nrows)
 {
     int64_t nd = oc_ndof(m);
+    if (nrows < 0 || nrows > nd) nrows = nd;
     set_threads(nthreads);
-    for (int64_t i = 0; i < rowptr[nd]; i++) val[i] = 0.0;   /* MatZeroEntries :594 */
+    for (int64_t i = 0; i < rowptr[nrows]; i++) val[i] = 0.0;   /* MatZeroEntries :594 */
     int ncx = ncolour1(m, 0), ncy = ncolour1(m, 1), ncz = ncolour1(m, 2);
-    int64_t nel = (int64_t)m->nelem[0] * m->nelem[1] * m->nelem[2];
+    int *el = (int *)malloc(sizeof(int) * (m->nelem[0] + m->nelem[1] + m->nelem[2]));
     for (int cc = 0; cc < ncx * ncy * ncz; cc++) {
         int cx = cc / (ncy * ncz), cy = (cc / ncz) % ncy, cz = cc % ncz;
+        int *ls[3], nl[3];
+        box_colour_lists(m, box, cx, cy, cz, el, ls, nl);
+        int64_t nt = (int64_t)nl[0] * nl[1] * nl[2];
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t t = 0; t < nel; t++) {
-            int ex = (int)(t / ((int64_t)m->nelem[1] * m->nelem[2])), ey = (int)((t / m->nelem[2]) % m->nelem[1]), ez = (int)(t % m->nelem[2]);
-            if (colour1(m, 0, ex) != cx || colour1(m, 1, ey) != cy || colour1(m, 2, ez) != cz) continue;
-            if (!in_box(box, ex, ey, ez)) continue;
+        for (int64_t t = 0; t < nt; t++) {
+            int ex = ls[0][t / ((int64_t)nl[1] * nl[2])], ey = ls[1][(t / nl[2]) % nl[1]], ez = ls[2][t % nl[2]];
             double *ke = (double *)malloc(6561 * sizeof(double));
             oc_element_tangent(m, ph, bc, U_hist, ex, ey, ez, ke);
             int64_t idx[81];
@@ -1029,20 +1071,23 @@ void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const doub
                         for (int k = 0; k < 3; k++)
                             idx[3 * (9 * bx + 3 * by + bz) + k] = dof_of_active(m, ex + bx, ey + by, ez + bz, k);
             /* MatSetValues(...,ADD_VALUES): per row, search each column */
-            for (int r = 0; r < 81; r++)
+            for (int r = 0; r < 81; r++) {
+                if (idx[r] >= nrows) continue;   /* rows outside the stored prefix */
                 for (int c = 0; c < 81; c++) {
                     int64_t pos = find_col(rowptr, col, idx[r], (int32_t)idx[c]);
                     if (pos >= 0) val[pos] += ke[81 * r + c];
                 }
+            }
             free(ke);
         }
     }
+    free(el);
     if (!apply_bc) return;
     /* MatZeroRowsColumns(P, rows, diag=1) (assmcore_iga.c:844) */
     int32_t *mask = (int32_t *)malloc(sizeof(int32_t) * nd);
     double *g = (double *)malloc(sizeof(double) * nd);
     oc_dirichlet(m, bc, mask, g);
-    for (int64_t r = 0; r < nd; r++)
+    for (int64_t r = 0; r < nrows; r++)
         for (int64_t p = rowptr[r]; p < rowptr[r + 1]; p++)
             if (mask[r] || mask[col[p]]) val[p] = (col[p] == r) ? 1.0 : 0.0;
     free(mask);

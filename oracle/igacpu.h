@@ -90,6 +90,9 @@ int64_t oc_ndof(const oc_mesh *m);
 /* CSR pattern of the AIJ matrix: rows are owned dofs, columns sorted */
 int64_t oc_csr_nnz(const oc_mesh *m);
 void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col);
+/* the first nrows rows of that pattern (nrows < 0: all) */
+int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t nrows);
+void oc_csr_pattern_rows(const oc_mesh *m, int64_t nrows, int64_t *rowptr, int32_t *col);
 void oc_setnz(const oc_mesh *m, int32_t *d_nz);   /* assmcore_iga.c:1054-1087, one rank */
 
 /* Dirichlet rows: mask[ndof] (1 if constrained) and g[ndof] (value inserted,
@@ -103,10 +106,12 @@ void oc_neumann(const oc_mesh *m, const oc_bc *bc, double *F);
  * (NULL: all).  apply_bc = 0 skips Neumann/Dirichlet. */
 void oc_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist,
                  double *R, const int *box, int apply_bc, int nthreads);
-/* Tangent_iga (assmcore_iga.c:507-858) into the CSR pattern of oc_csr_pattern. */
+/* Tangent_iga (assmcore_iga.c:507-858) into the CSR pattern of oc_csr_pattern,
+ * or of its first nrows rows (oc_csr_pattern_rows; nrows < 0: all rows):
+ * contributions to later rows are dropped. */
 void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist,
                 const int64_t *rowptr, const int32_t *col, double *val,
-                const int *box, int apply_bc, int nthreads);
+                const int *box, int apply_bc, int nthreads, int64_t nrows);
 
 /* per-element kernels (one element, reference local ordering 3*ibpe+idof) */
 void oc_element_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc,

@@ -50,6 +50,7 @@ def lib():
             raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
         L = C.CDLL(path)
         L.oc_csr_nnz.restype = C.c_int64
+        L.oc_csr_nnz_rows.restype = C.c_int64
         L.oc_ndof.restype = C.c_int64
         L.oc_energy.restype = C.c_double
         L.oc_volume.restype = C.c_double
@@ -200,11 +201,14 @@ def clamped_tip_bc(tip=0.0, tip_dof=2):
 
 
 # ---------------------------------------------------------------- assembly
-def csr_pattern(mesh):
-    nnz = lib().oc_csr_nnz(C.byref(mesh.c))
-    rowptr = np.zeros(mesh.ndof + 1, dtype=np.int64)
+def csr_pattern(mesh, nrows=None):
+    """CSR pattern of the tangent (rowptr, col); ``nrows`` keeps only the first
+    rows (a row-prefix pattern for tangents of elements near x = 0)."""
+    nr = mesh.ndof if nrows is None else min(int(nrows), mesh.ndof)
+    nnz = lib().oc_csr_nnz_rows(C.byref(mesh.c), C.c_int64(nr))
+    rowptr = np.zeros(nr + 1, dtype=np.int64)
     col = np.zeros(nnz, dtype=np.int32)
-    lib().oc_csr_pattern(C.byref(mesh.c), _p(rowptr, c_i64_p), _p(col, c_i32_p))
+    lib().oc_csr_pattern_rows(C.byref(mesh.c), C.c_int64(nr), _p(rowptr, c_i64_p), _p(col, c_i32_p))
     return rowptr, col
 
 
@@ -244,13 +248,15 @@ def residual(mesh, ph, bc, U_hist, box=None, apply_bc=True, nthreads=0):
 
 
 def tangent(mesh, ph, bc, U_hist, pattern=None, box=None, apply_bc=True, nthreads=0):
+    """Tangent_iga into ``pattern`` (default: the full pattern); a row-prefix
+    pattern from csr_pattern(mesh, nrows) keeps only those rows."""
     U = np.ascontiguousarray(U_hist, dtype=np.float64)
     rowptr, col = csr_pattern(mesh) if pattern is None else pattern
     val = np.zeros(len(col))
     b = _box(box)
     lib().oc_tangent(C.byref(mesh.c), C.byref(ph), C.byref(bc), _p(U), _p(rowptr, c_i64_p),
                      _p(col, c_i32_p), _p(val), _p(b, c_int_p) if b is not None else None,
-                     int(apply_bc), int(nthreads))
+                     int(apply_bc), int(nthreads), C.c_int64(len(rowptr) - 1))
     return rowptr, col, val
 
 
