@@ -13,6 +13,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libigap_gpu.so")
 
 IGAP_OK = 0
+IGAP_ERR_ARG = 1
+IGAP_ERR_HIP = 2
+IGAP_ERR_NOMEM = 3
+IGAP_ERR_STATE = 4
 IGAP_PHYS_GRADELAST = 0
 IGAP_PHYS_GRADELASTTIME = 2
 IGAP_F64 = 0
@@ -104,6 +108,8 @@ _SIGS = {
     "igap_gpu_timing_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_i64_p]),
     "igap_gpu_timing_reset": (C.c_int, [C.c_void_p]),
     "igap_gpu_selftest_mfma": (C.c_int, [c_double_p, c_float_p]),
+    "igap_gpu_debug_fail_alloc": (C.c_int, [C.c_int]),
+    "igap_gpu_phys_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "igap_gpu_selftest_ilu": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_double_p]),
     "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),
     "igap_gpu_snes_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),

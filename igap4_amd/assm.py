@@ -261,6 +261,36 @@ class Assm:
             vals = self.dtoh(lib().igap_gpu_csr_val32(self.ctx), self.nnz, np.float32)
         return rowptr, col, vals
 
+    # ------------------------------------------------------------ pointwise physics
+    def phys_eval(self, u, residual=True, tangent=True, density=True):
+        """The Phys callbacks (residual / tangent / density, include/physimpl.h:6-9)
+        of this context's physics on the GPU at host jets u[npts][30*(torder+1)];
+        returns (r[npts][30], C[npts][30][30], hh[npts][nd]) (None where not asked)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        w = 30 * (self.torder + 1)
+        u = u.reshape(-1, w)
+        n = u.shape[0]
+        nd = 13 if self.torder == 0 else 2
+        sizes = {"u": u.size, "r": 30 * n if residual else 0, "C": 900 * n if tangent else 0,
+                 "hh": nd * n if density else 0}
+        ptrs = {}
+        try:
+            for k, m in sizes.items():
+                if m:
+                    p = C.c_void_p()
+                    check(lib().igap_gpu_malloc(self.ctx, C.byref(p), 8 * m))
+                    ptrs[k] = p
+            self.htod(ptrs["u"], u)
+            check(lib().igap_gpu_phys_eval(self.ctx, n, ptrs["u"], ptrs.get("r"), ptrs.get("C"), ptrs.get("hh")))
+            self.synchronize()
+            r = self.dtoh(ptrs["r"], 30 * n).reshape(n, 30) if residual else None
+            Cm = self.dtoh(ptrs["C"], 900 * n).reshape(n, 30, 30) if tangent else None
+            hh = self.dtoh(ptrs["hh"], nd * n).reshape(n, nd) if density else None
+        finally:
+            for p in ptrs.values():
+                lib().igap_gpu_free(self.ctx, p)
+        return r, Cm, hh
+
     # ------------------------------------------------------------ Newton-Krylov
     def snes_solve(self, opts=None, **kw):
         """Newton solve from U_hist level 0 (SNESSolve driven by IGAPSolvSolve,

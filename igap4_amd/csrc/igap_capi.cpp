@@ -6,6 +6,7 @@
 // U-independent Neumann load (:273-405) and stream-ordered launches.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -20,6 +21,29 @@
 using namespace igap;
 
 static thread_local std::string g_err;
+// allocations still to succeed before one fails (< 0: off); a test hook
+static std::atomic<int> g_fail_alloc{-1};
+
+hipError_t igap::dmalloc_raw(void **p, size_t bytes)
+{
+    int n = g_fail_alloc.load();
+    while (n >= 0) {
+        if (g_fail_alloc.compare_exchange_weak(n, n - 1)) {
+            if (n == 0) {
+                *p = nullptr;
+                return hipErrorOutOfMemory;
+            }
+            break;
+        }
+    }
+    return hipMalloc(p, bytes);
+}
+
+extern "C" int igap_gpu_debug_fail_alloc(int n)
+{
+    g_fail_alloc.store(n < 0 ? -1 : n);
+    return IGAP_OK;
+}
 
 int igap::fail(int code, const char *fmt, ...)
 {
@@ -346,7 +370,7 @@ template <typename T>
 static int dalloc(T **p, int64_t n)
 {
     if (n <= 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+    hipError_t e = igap::dmalloc(p, sizeof(T) * n);
     if (e != hipSuccess) return fail(IGAP_ERR_NOMEM, "hipMalloc(%lld bytes): %s", (long long)(sizeof(T) * n), hipGetErrorString(e));
     return IGAP_OK;
 }
@@ -540,8 +564,16 @@ extern "C" int igap_gpu_intdensity(igap_gpu_ctx *c, double *HH, int *ndensity)
     const int nd = c->torder == 0 ? 13 : 2;
     HIPCHK(hipSetDevice(c->device));
     if (!c->d_Hd) {
-        HIPCHK(hipMalloc(&c->d_Hd, sizeof(double) * (size_t)c->nel * nd));
-        HIPCHK(hipMalloc(&c->d_H, sizeof(double) * nd));
+        // both buffers or neither: a failed call leaves the context as it was
+        double *hd = nullptr, *h = nullptr;
+        hipError_t e = igap::dmalloc(&hd, sizeof(double) * (size_t)c->nel * nd);
+        if (e == hipSuccess) e = igap::dmalloc(&h, sizeof(double) * nd);
+        if (e != hipSuccess) {
+            if (hd) (void)hipFree(hd);
+            return fail(IGAP_ERR_NOMEM, "density buffers: %s", hipGetErrorString(e));
+        }
+        c->d_Hd = hd;
+        c->d_H = h;
     }
     HIPCHK(launch_density(c->M, c->P, c->d_Uh, c->ndof, c->d_Hd, c->d_H, (int)c->nel, c->stream));
     std::vector<double> H(nd), U((size_t)(c->torder + 1) * c->ndof), F(c->ndof);
@@ -678,7 +710,7 @@ extern "C" int igap_gpu_tangent(igap_gpu_ctx *c, const double *U, void *vals, in
     }
     if (!c->d_Ke[prec]) {
         size_t es = prec == IGAP_F64 ? sizeof(double) : sizeof(float);
-        hipError_t e = hipMalloc(&c->d_Ke[prec], es * 6561 * (size_t)c->nel);
+        hipError_t e = igap::dmalloc(&c->d_Ke[prec], es * 6561 * (size_t)c->nel);
         if (e != hipSuccess) return fail(IGAP_ERR_NOMEM, "element tangent buffer (%zu bytes): %s", es * 6561 * (size_t)c->nel, hipGetErrorString(e));
     }
     if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
@@ -704,6 +736,18 @@ extern "C" int igap_gpu_tangent(igap_gpu_ctx *c, const double *U, void *vals, in
     return IGAP_OK;
 }
 
+// ---------------------------------------------------------------- pointwise physics
+extern "C" int igap_gpu_phys_eval(igap_gpu_ctx *c, int64_t npts, const double *u, double *r, double *C, double *hh)
+{
+    if (!c) return fail(IGAP_ERR_ARG, "null context");
+    if (npts < 0) return fail(IGAP_ERR_ARG, "npts = %lld < 0", (long long)npts);
+    if (npts > 0 && !u) return fail(IGAP_ERR_ARG, "null jets");
+    if (npts > ((int64_t)1 << 31) / 30) return fail(IGAP_ERR_ARG, "npts = %lld too large for one launch", (long long)npts);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_phys_eval(c->P, npts, u, r, C, hh, c->stream));
+    return IGAP_OK;
+}
+
 // ---------------------------------------------------------------- helpers
 extern "C" int igap_gpu_synchronize(igap_gpu_ctx *c)
 {
@@ -716,7 +760,7 @@ extern "C" int igap_gpu_malloc(igap_gpu_ctx *c, void **ptr, int64_t bytes)
 {
     if (!c || !ptr) return fail(IGAP_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(c->device));
-    hipError_t e = hipMalloc(ptr, bytes > 0 ? bytes : 1);
+    hipError_t e = igap::dmalloc(ptr, bytes > 0 ? bytes : 1);
     if (e != hipSuccess) return fail(IGAP_ERR_NOMEM, "hipMalloc(%lld): %s", (long long)bytes, hipGetErrorString(e));
     return IGAP_OK;
 }

@@ -13,6 +13,14 @@ namespace igap {
 struct Solver;
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void solver_destroy(Solver *s);
+// hipMalloc of every device buffer the library owns, through the test hook of
+// igap_gpu_debug_fail_alloc (an armed countdown makes one allocation fail)
+hipError_t dmalloc_raw(void **p, size_t bytes);
+template <typename T>
+inline hipError_t dmalloc(T **p, size_t bytes)
+{
+    return dmalloc_raw(reinterpret_cast<void **>(p), bytes);
+}
 }  // namespace igap
 
 #define HIPCHK(x)                                                                                        \

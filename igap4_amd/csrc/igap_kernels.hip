@@ -1148,6 +1148,129 @@ __global__ void k_selftest_mfma(double *o64, float *o32)
 }
 
 // ============================================================================
+// Pointwise physics at given jets: the device counterpart of the Phys
+// callbacks residual / tangent / density (include/physimpl.h:6-9; gradelast,
+// phys/gradelast.c; gradelasttime, phys/gradelasttime.c), through the same
+// device functions the assembly kernels use (phys_state, phys_residual_acc,
+// c_row, c_row_mom, phys_density_*).  Jet layout of one point:
+// u[30 h + 10 k + a] with h = 0 the current level, 1 the previous and 2 the
+// one before (the reference's u, u1 = u + 30, u0 = u + 60).  Thread t
+// evaluates row (k, a) = t % 30 of point t / 30: r[30 p + row],
+// C[900 p + 30 row + 10 l + b] and, on row 0, the densities hh[nd p + i].
+// ============================================================================
+// the moment state of the transient tangent (c_row_mom) for the jets
+// Um = (u1 + u0)/2 and Ud = (u - u0)/2, summed over the rule nodes in order
+__device__ void moment_state_seq(const double *Um, const double *Ud, const PhysK &P, const Mom &mu, double *ms)
+{
+    double S[9], G[9], n0[9], n1[9], n2[9];
+    for (int i = 0; i < 9; i++) S[i] = G[i] = n0[i] = n1[i] = n2[i] = 0.0;
+    for (int j = 0; j < P.nnode; j++) {
+        double us[30];
+        for (int i = 0; i < 30; i++) us[i] = Um[i] + P.s[j] * Ud[i];
+        State st;
+        phys_state(us, P, st);
+        const double w0 = P.ct[j], w1 = w0 * P.s[j], w2 = w1 * P.s[j];
+        for (int i = 0; i < 9; i++) {
+            S[i] += w0 * st.S[i];
+            G[i] += w0 * st.G[i];
+            n0[i] += w0 * st.N[i];
+            n1[i] += w1 * st.N[i];
+            n2[i] += w2 * st.N[i];
+        }
+    }
+    const double c12 = 1.0 / (mu.a11 * mu.a22), c22 = 1.0 / mu.d2;
+    for (int i = 0; i < 9; i++) {
+        ms[MS + i] = S[i];
+        ms[MG + i] = G[i];
+        ms[MN11 + i] = n0[i] / mu.m0;
+        ms[MN12 + i] = (n1[i] - mu.lf * n0[i]) * c12;
+        ms[MN22 + i] = ((mu.lf * n0[i] - n1[i]) * mu.lf + (n2[i] - mu.lf * n1[i])) * c22;
+    }
+    for (int i = 0; i < 36; i++) {
+        const bool isF = i < 9;
+        const int r = isF ? i : i - 9;
+        const int slot = isF ? 10 * (r / 3) + 1 + (r % 3) : 10 * (r / 9) + sidx((r % 9) / 3, r % 3);
+        double x0 = Um[slot];
+        const double xd = Ud[slot];
+        if (isF && (r / 3) == (r % 3)) x0 += 1.0;
+        const double x1 = mu.a11 * (x0 + mu.lf * xd), x2 = mu.a22 * xd;
+        if (isF) { ms[MF1 + r] = x1; ms[MF2 + r] = x2; }
+        else { ms[MH1 + r] = x1; ms[MH2 + r] = x2; }
+    }
+}
+
+template <int NH>
+__global__ __launch_bounds__(128) void k_phys_eval(PhysK P, int64_t npts, const double *__restrict__ u, double *__restrict__ r,
+                                                  double *__restrict__ C, double *__restrict__ hh)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts * 30) return;
+    const int64_t p = t / 30;
+    const int row = (int)(t - 30 * p), k = row / 10, a = row % 10;
+    const double *J = u + p * 30 * NH;
+    if (r) {
+        double rr[30];
+        if (NH == 1) {
+            phys_residual_static(P, J, rr);
+        } else {
+            for (int i = 0; i < 30; i++) rr[i] = 0.0;
+            for (int g = 0; g < P.nnode; g++) {
+                double us[30];
+                for (int i = 0; i < 30; i++) us[i] = 0.5 * (J[30 + i] + J[60 + i]) + P.s[g] * (0.5 * (J[i] - J[60 + i]));
+                State st;
+                phys_state(us, P, st);
+                phys_residual_acc(st, P.cr[g], rr);
+            }
+            for (int kk = 0; kk < 3; kk++)
+                rr[10 * kk] = (P.cii * (J[10 * kk] - J[60 + 10 * kk])) / (2. * P.dt)
+                            + (P.rho * (J[10 * kk] + J[60 + 10 * kk] - 2 * J[30 + 10 * kk])) / (P.dt * P.dt);
+        }
+        r[p * 30 + row] = rr[row];
+    }
+    if (C) {
+        double out[30];
+        for (int i = 0; i < 30; i++) out[i] = 0.0;
+        if (NH == 1) {
+            State st;
+            phys_state(J, P, st);
+            double sp[STW];
+            for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+            for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+            for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
+            for (int l = 0; l < 3; l++) c_row<0>(sp, P, k, a, l, 1.0, out + 10 * l);
+        } else {
+            Mom mu = {0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+            for (int j = 0; j < P.nnode; j++) {
+                mu.m0 += P.ct[j];
+                mu.m1 += P.ct[j] * P.s[j];
+                mu.m2 += P.ct[j] * P.s[j] * P.s[j];
+            }
+            mu.factor();
+            double Um[30], Ud[30], ms[MW];
+            for (int i = 0; i < 30; i++) {
+                Um[i] = 0.5 * (J[30 + i] + J[60 + i]);
+                Ud[i] = 0.5 * (J[i] - J[60 + i]);
+            }
+            moment_state_seq(Um, Ud, P, mu, ms);
+            for (int l = 0; l < 3; l++) c_row_mom<0>(ms, P, k, a, l, out + 10 * l);
+            if (a == 0) out[10 * k] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
+        }
+        for (int i = 0; i < 30; i++) C[p * 900 + 30 * row + i] = out[i];
+    }
+    if (hh && row == 0) {
+        if (NH == 1) {
+            double h[ND_STATIC];
+            phys_density_static(J, P, h);
+            for (int i = 0; i < ND_STATIC; i++) hh[p * ND_STATIC + i] = h[i];
+        } else {
+            double h[ND_TRANSIENT];
+            phys_density_transient(J, P, h);
+            for (int i = 0; i < ND_TRANSIENT; i++) hh[p * ND_TRANSIENT + i] = h[i];
+        }
+    }
+}
+
+// ============================================================================
 // launchers (host, C++ linkage; called by the C ABI in igap_capi.cpp)
 // ============================================================================
 }  // namespace igap
@@ -1230,6 +1353,16 @@ hipError_t launch_copy_f64(double *dst, const double *src, int64_t n, hipStream_
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_eval(const PhysK &P, int64_t npts, const double *u, double *r, double *C, double *hh, hipStream_t s)
+{
+    if (npts <= 0) return hipSuccess;
+    const int64_t nt = 30 * npts;
+    dim3 grid((unsigned)((nt + 127) / 128)), block(128);
+    if (P.torder == 0) hipLaunchKernelGGL(k_phys_eval<1>, grid, block, 0, s, P, npts, u, r, C, hh);
+    else hipLaunchKernelGGL(k_phys_eval<3>, grid, block, 0, s, P, npts, u, r, C, hh);
     return hipGetLastError();
 }
 

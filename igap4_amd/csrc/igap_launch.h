@@ -21,5 +21,6 @@ hipError_t launch_csr_cols(const MeshK &M, const int64_t *rowptr, int32_t *col, 
 hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t s);
 hipError_t launch_copy_f64(double *dst, const double *src, int64_t n, hipStream_t s);
 hipError_t launch_selftest_mfma(double *o64, float *o32, hipStream_t s);
+hipError_t launch_phys_eval(const PhysK &P, int64_t npts, const double *u, double *r, double *C, double *hh, hipStream_t s);
 
 }  // namespace igap

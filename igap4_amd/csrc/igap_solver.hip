@@ -71,6 +71,7 @@ struct Solver {
     int mc_sweep_ok = 0;            // no two points of one colour are coupled (false for a periodic
                                     // direction whose size is not a multiple of 3): colour sweeps valid
     int mc_block_ok = 0;            // point-block pattern (k_mcb_* applicable)
+    int mc_ready = 0;               // the coloured ordering is complete
     int32_t *mc_pcoord = nullptr;   // per permuted point: owned-grid (i, j, k) packed 10 bits each
     int *zpiv = nullptr;            // first row with a zero pivot of the last own factorisation
     double *part = nullptr;         // partial dot products, (m+2) x ceil(n / DOT_CHUNK)
@@ -720,6 +721,14 @@ static inline dim3 grid1(int64_t n)
         rocsparse_status s_ = (x);                                                   \
         if (s_ != rocsparse_status_success) return fail(IGAP_ERR_HIP, "%s: rocsparse status %d", #x, (int)s_); \
     } while (0)
+// device allocation through the library's allocator; failure is IGAP_ERR_NOMEM
+#define DALLOC(ptr, bytes)                                                                            \
+    do {                                                                                              \
+        hipError_t e_ = igap::dmalloc((ptr), (bytes));                                                \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(IGAP_ERR_NOMEM, "device allocation of %zu bytes: %s", (size_t)(bytes),       \
+                        hipGetErrorString(e_));                                                       \
+    } while (0)
 #define BLCHK(x)                                                                     \
     do {                                                                             \
         rocblas_status s_ = (x);                                                     \
@@ -731,6 +740,42 @@ static inline dim3 grid1(int64_t n)
 using namespace igap;
 
 // ---------------------------------------------------------------- set-up
+// a new solver's buffers and library handles (the caller frees s on failure)
+static int solver_init(igap_gpu_ctx *c, Solver *s, int m)
+{
+    s->m = m;
+    s->n = c->ndof;
+    const int64_t n = c->ndof;
+    if (c->nnz > 0x7fffffffLL || n > 0x7fffffffLL)
+        return fail(IGAP_ERR_ARG, "the solver's 32-bit sparse indices need nnz < 2^31 (nnz = %lld)", (long long)c->nnz);
+    DALLOC(&s->V, sizeof(double) * (size_t)(m + 1) * n);
+    double **vecs[] = {&s->w, &s->z, &s->t, &s->t2, &s->d, &s->x, &s->xt, &s->F, &s->Ft, &s->diag};
+    for (double **v : vecs) DALLOC(v, sizeof(double) * n);
+    DALLOC(&s->h, sizeof(double) * 3 * (m + 2));
+    DALLOC(&s->part, sizeof(double) * (size_t)(m + 2) * (size_t)((n + DOT_CHUNK - 1) / DOT_CHUNK));
+    DALLOC(&s->fa, sizeof(float) * n);
+    DALLOC(&s->fb, sizeof(float) * n);
+    DALLOC(&s->rowptr32, sizeof(int32_t) * (n + 1));
+    hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, n + 1, c->d_rowptr,
+                       s->rowptr32);
+    HIPCHK(hipGetLastError());
+    SPCHK(rocsparse_create_handle(&s->sp));
+    SPCHK(rocsparse_set_stream(s->sp, c->stream));
+    SPCHK(rocsparse_create_mat_descr(&s->dA));
+    SPCHK(rocsparse_create_mat_descr(&s->dL));
+    SPCHK(rocsparse_set_mat_fill_mode(s->dL, rocsparse_fill_mode_lower));
+    SPCHK(rocsparse_set_mat_diag_type(s->dL, rocsparse_diag_type_unit));
+    SPCHK(rocsparse_create_mat_descr(&s->dU));
+    SPCHK(rocsparse_set_mat_fill_mode(s->dU, rocsparse_fill_mode_upper));
+    SPCHK(rocsparse_set_mat_diag_type(s->dU, rocsparse_diag_type_non_unit));
+    BLCHK(rocblas_create_handle(&s->bl));
+    BLCHK(rocblas_set_stream(s->bl, c->stream));
+    BLCHK(rocblas_set_pointer_mode(s->bl, rocblas_pointer_mode_host));
+    return IGAP_OK;
+}
+
+// the context's solver with GMRES restart m, built on first use (or when m
+// changes); published only once complete, so a failed call leaves none
 static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
 {
     Solver *s = c->solver;
@@ -740,44 +785,40 @@ static int solver_get(igap_gpu_ctx *c, int m, Solver **out)
     }
     if (!s) {
         s = new Solver();
+        const int rc = solver_init(c, s, m);
+        if (rc) {
+            solver_destroy(s);
+            return rc;
+        }
         c->solver = s;
-        s->m = m;
-        s->n = c->ndof;
-        int64_t n = c->ndof;
-        if (c->nnz > 0x7fffffffLL || n > 0x7fffffffLL)
-            return fail(IGAP_ERR_ARG, "ILU(0) through rocSPARSE needs nnz < 2^31 (nnz = %lld)", (long long)c->nnz);
-        HIPCHK(hipMalloc(&s->V, sizeof(double) * (size_t)(m + 1) * n));
-        double **vecs[] = {&s->w, &s->z, &s->t, &s->t2, &s->d, &s->x, &s->xt, &s->F, &s->Ft, &s->diag};
-        for (double **v : vecs) HIPCHK(hipMalloc(v, sizeof(double) * n));
-        HIPCHK(hipMalloc(&s->h, sizeof(double) * 3 * (m + 2)));
-        HIPCHK(hipMalloc(&s->part, sizeof(double) * (size_t)(m + 2) * (size_t)((n + DOT_CHUNK - 1) / DOT_CHUNK)));
-        HIPCHK(hipMalloc(&s->fa, sizeof(float) * n));
-        HIPCHK(hipMalloc(&s->fb, sizeof(float) * n));
-        HIPCHK(hipMalloc(&s->rowptr32, sizeof(int32_t) * (n + 1)));
-        hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c->stream, n + 1, c->d_rowptr,
-                           s->rowptr32);
-        HIPCHK(hipGetLastError());
-        SPCHK(rocsparse_create_handle(&s->sp));
-        SPCHK(rocsparse_set_stream(s->sp, c->stream));
-        SPCHK(rocsparse_create_mat_descr(&s->dA));
-        SPCHK(rocsparse_create_mat_descr(&s->dL));
-        SPCHK(rocsparse_set_mat_fill_mode(s->dL, rocsparse_fill_mode_lower));
-        SPCHK(rocsparse_set_mat_diag_type(s->dL, rocsparse_diag_type_unit));
-        SPCHK(rocsparse_create_mat_descr(&s->dU));
-        SPCHK(rocsparse_set_mat_fill_mode(s->dU, rocsparse_fill_mode_upper));
-        SPCHK(rocsparse_set_mat_diag_type(s->dU, rocsparse_diag_type_non_unit));
-        BLCHK(rocblas_create_handle(&s->bl));
-        BLCHK(rocblas_set_stream(s->bl, c->stream));
-        BLCHK(rocblas_set_pointer_mode(s->bl, rocblas_pointer_mode_host));
     }
     *out = s;
     return IGAP_OK;
 }
 
 // the coloured ordering and the permuted CSR pattern (host, once per context)
+static int mc_build_body(igap_gpu_ctx *c, Solver *s);
+
+// the coloured ordering, built once; a failed build frees what it allocated
+// so the next call starts over
 static int mc_build(igap_gpu_ctx *c, Solver *s)
 {
-    if (s->mc_perm) return IGAP_OK;
+    if (s->mc_ready) return IGAP_OK;
+    const int rc = mc_build_body(c, s);
+    if (rc) {
+        int32_t **bufs[] = {&s->mc_pcoord, &s->mc_dpos, &s->mc_perm, &s->mc_rowptr, &s->mc_col, &s->mc_src};
+        for (int32_t **b : bufs) {
+            if (*b) (void)hipFree(*b);
+            *b = nullptr;
+        }
+        return rc;
+    }
+    s->mc_ready = 1;
+    return IGAP_OK;
+}
+
+static int mc_build_body(igap_gpu_ctx *c, Solver *s)
+{
     const int64_t n = c->ndof, nnz = c->nnz, ncp = n / 3;
     const int nb1 = c->nb[1], nb2 = c->nb[2];
     std::vector<int64_t> rp(n + 1);
@@ -863,16 +904,16 @@ static int mc_build(igap_gpu_ctx *c, Solver *s)
             const int64_t i = a / ((int64_t)nb1 * nb2), j = (a / nb2) % nb1, k = a % nb2;
             pc[q] = (int32_t)(i | (j << 10) | (k << 20));
         }
-        HIPCHK(hipMalloc(&s->mc_pcoord, sizeof(int32_t) * ncp));
+        DALLOC(&s->mc_pcoord, sizeof(int32_t) * ncp);
         HIPCHK(hipMemcpyAsync(s->mc_pcoord, pc.data(), sizeof(int32_t) * ncp, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    HIPCHK(hipMalloc(&s->mc_dpos, sizeof(int32_t) * n));
+    DALLOC(&s->mc_dpos, sizeof(int32_t) * n);
     HIPCHK(hipMemcpyAsync(s->mc_dpos, pdpos.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMalloc(&s->mc_perm, sizeof(int32_t) * n));
-    HIPCHK(hipMalloc(&s->mc_rowptr, sizeof(int32_t) * (n + 1)));
-    HIPCHK(hipMalloc(&s->mc_col, sizeof(int32_t) * nnz));
-    HIPCHK(hipMalloc(&s->mc_src, sizeof(int32_t) * nnz));
+    DALLOC(&s->mc_perm, sizeof(int32_t) * n);
+    DALLOC(&s->mc_rowptr, sizeof(int32_t) * (n + 1));
+    DALLOC(&s->mc_col, sizeof(int32_t) * nnz);
+    DALLOC(&s->mc_src, sizeof(int32_t) * nnz);
     HIPCHK(hipMemcpyAsync(s->mc_perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(s->mc_rowptr, prp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(s->mc_col, pcol.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
@@ -885,7 +926,7 @@ static int mc_build(igap_gpu_ctx *c, Solver *s)
 // own factorisation (k_mcb_ilu), colour by colour, of the values in mc_ilu[p]
 static int mc_factor_own(igap_gpu_ctx *c, Solver *s, int p)
 {
-    if (!s->zpiv) HIPCHK(hipMalloc(&s->zpiv, sizeof(int)));
+    if (!s->zpiv) DALLOC(&s->zpiv, sizeof(int));
     hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, c->stream, s->zpiv, 0x7fffffff);
     for (int cc = 0; cc < 27; cc++) {
         const int r0 = s->mc_cstart[cc], np = (s->mc_cstart[cc + 1] - r0) / 3;
@@ -914,7 +955,7 @@ static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p, int impl = 0)
     int rc = mc_build(c, s);
     if (rc) return rc;
     const int n = (int)c->ndof, nnz = (int)c->nnz;
-    if (!s->mc_ilu[p]) HIPCHK(hipMalloc(&s->mc_ilu[p], (p == IGAP_F64 ? sizeof(double) : sizeof(float)) * (size_t)nnz));
+    if (!s->mc_ilu[p]) DALLOC(&s->mc_ilu[p], (p == IGAP_F64 ? sizeof(double) : sizeof(float)) * (size_t)nnz);
     if (p == IGAP_F64)
         hipLaunchKernelGGL((k_gather<double, double>), grid1(nnz), dim3(256), 0, c->stream, (int64_t)nnz, s->mc_src, c->d_val64,
                            (double *)s->mc_ilu[p]);
@@ -924,6 +965,9 @@ static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p, int impl = 0)
     HIPCHK(hipGetLastError());
     if (impl == 0 && s->mc_block_ok) return mc_factor_own(c, s, p);
     if (!s->mc_analysed[p]) {
+        // leftovers of an analysis that failed part way
+        if (s->mc_info[p]) { rocsparse_destroy_mat_info(s->mc_info[p]); s->mc_info[p] = nullptr; }
+        if (s->mc_buf[p]) { (void)hipFree(s->mc_buf[p]); s->mc_buf[p] = nullptr; }
         SPCHK(rocsparse_create_mat_info(&s->mc_info[p]));
         size_t b0 = 0, b1 = 0, b2 = 0;
         if (p == IGAP_F64) {
@@ -938,7 +982,7 @@ static int pc_ilu_setup_mc(igap_gpu_ctx *c, Solver *s, int p, int impl = 0)
             SPCHK(rocsparse_scsrsv_buffer_size(s->sp, rocsparse_operation_none, n, nnz, s->dU, v, s->mc_rowptr, s->mc_col, s->mc_info[p], &b2));
         }
         size_t bs = std::max(b0, std::max(b1, b2));
-        HIPCHK(hipMalloc(&s->mc_buf[p], bs > 0 ? bs : 1));
+        DALLOC(&s->mc_buf[p], bs > 0 ? bs : 1);
         if (p == IGAP_F64) {
             double *v = (double *)s->mc_ilu[p];
             SPCHK(rocsparse_dcsrilu0_analysis(s->sp, n, nnz, s->dA, v, s->mc_rowptr, s->mc_col, s->mc_info[p],
@@ -976,11 +1020,14 @@ static int pc_ilu_setup(igap_gpu_ctx *c, Solver *s, int p)
     int n = (int)c->ndof, nnz = (int)c->nnz;
     if (!s->ilu[p]) {
         size_t es = p == IGAP_F64 ? sizeof(double) : sizeof(float);
-        HIPCHK(hipMalloc(&s->ilu[p], es * (size_t)nnz));
+        DALLOC(&s->ilu[p], es * (size_t)nnz);
     }
     if (p == IGAP_F64) HIPCHK(hipMemcpyAsync(s->ilu[p], c->d_val64, sizeof(double) * nnz, hipMemcpyDeviceToDevice, c->stream));
     else HIPCHK(hipMemcpyAsync(s->ilu[p], c->d_val32, sizeof(float) * nnz, hipMemcpyDeviceToDevice, c->stream));
     if (!s->analysed[p]) {
+        // leftovers of an analysis that failed part way
+        if (s->info[p]) { rocsparse_destroy_mat_info(s->info[p]); s->info[p] = nullptr; }
+        if (s->buf[p]) { (void)hipFree(s->buf[p]); s->buf[p] = nullptr; }
         SPCHK(rocsparse_create_mat_info(&s->info[p]));
         size_t b0 = 0, b1 = 0, b2 = 0;
         if (p == IGAP_F64) {
@@ -996,7 +1043,7 @@ static int pc_ilu_setup(igap_gpu_ctx *c, Solver *s, int p)
         }
         size_t bs = b0 > b1 ? b0 : b1;
         bs = bs > b2 ? bs : b2;
-        HIPCHK(hipMalloc(&s->buf[p], bs > 0 ? bs : 1));
+        DALLOC(&s->buf[p], bs > 0 ? bs : 1);
         if (p == IGAP_F64) {
             double *v = (double *)s->ilu[p];
             SPCHK(rocsparse_dcsrilu0_analysis(s->sp, n, nnz, s->dA, v, s->rowptr32, c->d_col, s->info[p], rocsparse_analysis_policy_reuse,
@@ -1148,9 +1195,16 @@ static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y, int mode = 0
     const int64_t n = c->ndof, ncp = n / 3;
     if (!c->d_pcol) {
         if (c->nnz % 9) return fail(IGAP_ERR_STATE, "pattern is not point-blocked (nnz %lld)", (long long)c->nnz);
-        HIPCHK(hipMalloc(&c->d_pcol, sizeof(int32_t) * (size_t)(c->nnz / 9)));
+        int32_t *pcol = nullptr;
+        DALLOC(&pcol, sizeof(int32_t) * (size_t)(c->nnz / 9));
         hipLaunchKernelGGL(k_point_cols, grid1(ncp), dim3(256), 0, c->stream, ncp, (const int64_t *)c->d_rowptr,
-                           (const int32_t *)c->d_col, c->d_pcol);
+                           (const int32_t *)c->d_col, pcol);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            (void)hipFree(pcol);
+            return fail(IGAP_ERR_HIP, "k_point_cols: %s", hipGetErrorString(e));
+        }
+        c->d_pcol = pcol;
     }
     const dim3 g((unsigned)((ncp + 3) / 4)), bs(256);
     const int64_t *rp = (const int64_t *)c->d_rowptr;
@@ -1339,9 +1393,9 @@ static int op1_upload(const std::vector<double> &M, int rows, int cols, Op1 &o)
     }
     o.rows = rows;
     o.cols = cols;
-    HIPCHK(hipMalloc(&o.rp, sizeof(int) * (rows + 1)));
-    HIPCHK(hipMalloc(&o.ci, sizeof(int) * std::max<size_t>(1, ci.size())));
-    HIPCHK(hipMalloc(&o.w, sizeof(double) * std::max<size_t>(1, w.size())));
+    DALLOC(&o.rp, sizeof(int) * (rows + 1));
+    DALLOC(&o.ci, sizeof(int) * std::max<size_t>(1, ci.size()));
+    DALLOC(&o.w, sizeof(double) * std::max<size_t>(1, w.size()));
     HIPCHK(hipMemcpy(o.rp, rp.data(), sizeof(int) * (rows + 1), hipMemcpyHostToDevice));
     if (!ci.empty()) {
         HIPCHK(hipMemcpy(o.ci, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice));
@@ -1469,6 +1523,10 @@ static int mg_transfer(igap_gpu_ctx *c, MGHier *g, int l, int which, const doubl
     return IGAP_OK;
 }
 
+static int mg_build_levels(igap_gpu_ctx *c, Solver *s, MGHier *g, int maxlev);
+
+// the multigrid hierarchy, built on first use and published only once
+// complete: a failed build releases its coarse contexts and buffers
 static int mg_build(igap_gpu_ctx *c, Solver *s, int maxlev)
 {
     if (s->mg) return IGAP_OK;
@@ -1478,16 +1536,27 @@ static int mg_build(igap_gpu_ctx *c, Solver *s, int maxlev)
         if (wv != 1.0) return fail(IGAP_ERR_ARG, "IGAP_PC_MG needs unit weights (a B-spline patch)");
     if (maxlev <= 0 || maxlev > MG_MAXLEV) maxlev = MG_MAXLEV;
     MGHier *g = new MGHier();
+    const int rc = mg_build_levels(c, s, g, maxlev);
+    if (rc) {
+        mg_destroy(g);
+        return rc;
+    }
     s->mg = g;
+    return IGAP_OK;
+}
+
+static int mg_build_levels(igap_gpu_ctx *c, Solver *s, MGHier *g, int maxlev)
+{
     std::memcpy(g->bc_type, c->bc_h.bc_type, sizeof g->bc_type);
     g->lev[0].c = c;
     g->lev[0].s = s;
     g->nlev = 1;
     const size_t bytes0 = sizeof(double) * (size_t)c->ndof;
-    HIPCHK(hipMalloc(&g->tmp1, bytes0));
-    HIPCHK(hipMalloc(&g->tmp2, bytes0));
+    DALLOC(&g->tmp1, bytes0);
+    DALLOC(&g->tmp2, bytes0);
     while (g->nlev < maxlev) {
-        MGLevel &F = g->lev[g->nlev - 1];
+        const int lf = g->nlev - 1;
+        MGLevel &F = g->lev[lf];
         igap_gpu_ctx *cf = F.c;
         if (cf->ndof <= MG_COARSE_NDOF) break;
         int any = 0;
@@ -1525,9 +1594,10 @@ static int mg_build(igap_gpu_ctx *c, Solver *s, int maxlev)
         }
         md.w = wc.data();
         md.X = Xc.data();
-        MGLevel &C = g->lev[g->nlev];
+        MGLevel &C = g->lev[lf + 1];
         int rc = igap_gpu_create(&md, &c->phys_h, &c->bc_h, c->device, 0, &C.c);
         if (rc) return rc;
+        g->nlev = lf + 2;   // counted from here on, so a failure below releases it
         // every level works on the fine context's stream (its rocSPARSE and
         // rocBLAS handles included), so the cycle needs no cross-stream waits
         HIPCHK(hipStreamSynchronize(C.c->stream));
@@ -1545,21 +1615,20 @@ static int mg_build(igap_gpu_ctx *c, Solver *s, int maxlev)
             if ((rc = op1_upload(Rt, ncd[d], nfd[d], F.R[d]))) return rc;
             if ((rc = op1_upload(Td[d], ncd[d], nfd[d], F.T[d]))) return rc;
         }
-        g->nlev++;
     }
     for (int l = 0; l < g->nlev; l++) {
         MGLevel &L = g->lev[l];
         const size_t b = sizeof(double) * (size_t)L.c->ndof;
         double **v[] = {&L.r, &L.t, &L.b, &L.x};
-        for (double **q : v) HIPCHK(hipMalloc(q, b));
-        HIPCHK(hipMalloc(&L.Dinv, sizeof(double) * 3 * (size_t)L.c->ndof));
+        for (double **q : v) DALLOC(q, b);
+        DALLOC(&L.Dinv, sizeof(double) * 3 * (size_t)L.c->ndof);
     }
     MGLevel &Lc = g->lev[g->nlev - 1];
     if (g->nlev > 1 && Lc.c->ndof <= MG_DENSE_MAX) {
         const size_t nn = (size_t)Lc.c->ndof;
-        HIPCHK(hipMalloc(&Lc.dense, sizeof(double) * nn * nn));
-        HIPCHK(hipMalloc(&Lc.ipiv, sizeof(rocblas_int) * nn));
-        HIPCHK(hipMalloc(&Lc.info, sizeof(rocblas_int)));
+        DALLOC(&Lc.dense, sizeof(double) * nn * nn);
+        DALLOC(&Lc.ipiv, sizeof(rocblas_int) * nn);
+        DALLOC(&Lc.info, sizeof(rocblas_int));
     }
     return IGAP_OK;
 }
@@ -1713,8 +1782,10 @@ static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const d
 }
 
 // GMRES(m), right preconditioned: solve A d = b (d starts at 0)
-static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres)
+static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres,
+                 int *conv)
 {
+    *conv = 1;
     const int pp = o->pc_fp32 ? IGAP_F32 : p;   // preconditioner precision
     const int64_t n = c->ndof;
     const int m = s->m;
@@ -1803,6 +1874,8 @@ static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, c
     }
     *its = total;
     *relres = rnorm / bnorm;
+    // KSP_CONVERGED_RTOL / _ATOL, or KSP_DIVERGED_ITS after ksp_maxit
+    *conv = (rnorm <= o->ksp_rtol * bnorm || rnorm <= o->ksp_atol) ? 1 : 0;
     return IGAP_OK;
 }
 
@@ -1858,11 +1931,16 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     st->n_residual++;
     st->fnorm0 = fnorm;
     st->converged = 0;
+    double fprev_it = -1.0;   // |F| before the last Newton step
     for (int it = 0;; it++) {
         st->fnorm = fnorm;
         if (!std::isfinite(fnorm)) { st->converged = -4; break; }   // DIVERGED_FNORM_NAN
         if (fnorm <= o->snes_atol) { st->converged = 2; break; }    // CONVERGED_FNORM_ABS
         if (o->snes_rtol > 0 && fnorm <= o->snes_rtol * st->fnorm0) { st->converged = 3; break; }   // CONVERGED_FNORM_RELATIVE
+        // IGAPSolvSolve's stagnation test (src/interface.c:318): a Newton step
+        // that changes |F| by less than 1e-6 relative ends the solve, reported
+        // as SNES_DIVERGED_LINE_SEARCH
+        if (fprev_it > 0 && std::fabs(fprev_it - fnorm) / fprev_it < 1e-6) { st->converged = -6; break; }
         if (it >= o->snes_maxit) { st->converged = -5; break; }     // DIVERGED_MAX_IT
         // tangent + preconditioner
         t = clk::now();
@@ -1895,18 +1973,22 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         // solve K d = -F
         t = clk::now();
         hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, -1.0, s->F, 0.0, s->Ft);
-        int kits = 0;
+        int kits = 0, kconv = 1;
         double relres = 0;
-        rc = gmres(c, s, o, p, s->Ft, s->d, &kits, &relres);
+        rc = gmres(c, s, o, p, s->Ft, s->d, &kits, &relres, &kconv);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->stream));
         st->t_ksp_ms += ms(t, clk::now());
         st->ksp_its += kits;
         st->last_ksp_relres = relres;
+        // SNES_DIVERGED_LINEAR_SOLVE: PETSc gives up on the first failed
+        // linear solve (maxLinearSolveFailures 1) and keeps the iterate
+        if (!kconv) { st->converged = -3; break; }
         // line search
         t = clk::now();
         double lambda = 1.0, f0 = 0.5 * fnorm * fnorm, slope = -fnorm * fnorm;
         double lprev = 0, fprev = 0, ftrial = 0, fn_trial = 0;
+        bool ls_ok = true;
         for (int ls = 0;; ls++) {
             hipLaunchKernelGGL(k_waxpy, grid1(n), dim3(256), 0, c->stream, n, lambda, s->d, s->x, s->xt);
             rc = igap_gpu_residual(c, s->xt, s->Ft);
@@ -1915,7 +1997,8 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
             rc = nrm2(c, s, s->Ft, &fn_trial);
             if (rc) return rc;
             ftrial = 0.5 * fn_trial * fn_trial;
-            if (!o->linesearch || (std::isfinite(ftrial) && ftrial <= f0 + 1e-4 * lambda * slope) || lambda < 1e-12 || ls >= 40) break;
+            if (!o->linesearch || (std::isfinite(ftrial) && ftrial <= f0 + 1e-4 * lambda * slope)) break;
+            if (lambda < 1e-12 || ls >= 40) { ls_ok = false; break; }
             double lnew;
             if (ls == 0 || !std::isfinite(ftrial)) {
                 lnew = std::isfinite(ftrial) ? -slope / (2.0 * (ftrial - f0 - slope)) : 0.1;
@@ -1937,10 +2020,14 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
             lambda = lnew;
             st->ls_backtracks++;
         }
+        st->t_residual_ms += ms(t, clk::now());
+        // SNES_DIVERGED_LINE_SEARCH: no sufficient decrease down to the
+        // smallest step; the iterate stays where it was
+        if (!ls_ok) { st->converged = -6; break; }
         std::swap(s->x, s->xt);
         std::swap(s->F, s->Ft);
+        fprev_it = fnorm;
         fnorm = fn_trial;
-        st->t_residual_ms += ms(t, clk::now());
         st->newton_its++;
         if (o->monitor) {
             std::fprintf(stderr, "  newton %d: |F| %.3e  ksp %d (relres %.1e)  lambda %.3g  %.0f ms\n", st->newton_its, fnorm,

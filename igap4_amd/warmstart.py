@@ -138,15 +138,19 @@ def features(dU1, dU2, dG):
     return x.astype(np.float32), s1, s2
 
 
-def samples_from_trajectory(U, G, mask):
+def samples_from_trajectory(U, G, mask, filt=None):
     """Training pairs from solutions U[0..K] and Dirichlet data G[0..K]:
-    (x_k, target_k) for k = 3..K, target = third difference / s2."""
+    (x_k, target_k) for k = 3..K, target = third difference / s2 (projected
+    by ``filt`` when given, so that the network fits what the filter keeps)."""
     out = []
     keep = (mask.reshape(-1) == 0)
     for k in range(3, len(U)):
         dU1, dU2 = U[k - 1] - U[k - 2], U[k - 2] - U[k - 3]
         x, s1, s2 = features(dU1, dU2, G[k] - G[k - 1])
-        t = ((U[k] - 3 * U[k - 1] + 3 * U[k - 2] - U[k - 3]) * keep) / s2
+        t3 = U[k] - 3 * U[k - 1] + 3 * U[k - 2] - U[k - 3]
+        if filt is not None:
+            t3 = filt(t3)
+        t = (t3 * keep) / s2
         out.append((x, t.reshape(-1, 3).astype(np.float32)))
     return out
 
@@ -228,10 +232,11 @@ def _bs2_rows(kv, nfun, x):
     return out
 
 
-def coarse_ops_1d(kf, nel):
+def coarse_ops_1d(kf, nel, with_knots=False):
     """Prolongation P (nel+2 x nel/2+2, exact knot insertion) and the
     interpolation T at the coarse Greville abscissae (nel/2+2 x nel+2) for an
-    open quadratic knot vector with an even number nel >= 4 of elements."""
+    open quadratic knot vector with an even number nel >= 4 of elements
+    (and the coarse knot vector with ``with_knots``)."""
     kf = np.asarray(kf, dtype=np.float64)
     nc_el = nel // 2
     kc = np.concatenate([[kf[2]] * 3, kf[2 + 2 * np.arange(1, nc_el)], [kf[nel + 2]] * 3])
@@ -240,24 +245,34 @@ def coarse_ops_1d(kf, nel):
     gc = 0.5 * (kc[1:nc + 1] + kc[2:nc + 2])
     P = np.linalg.solve(_bs2_rows(kf, nf, gf), _bs2_rows(kc, nc, gf))
     T = np.linalg.solve(_bs2_rows(kc, nc, gc), _bs2_rows(kf, nf, gc))
-    return P, T
+    return (P, T, kc) if with_knots else (P, T)
 
 
 class CoarseFilter:
-    """c -> P (T c): the projection of a control-point field onto the
-    next-coarser spline space (every even direction halved), which keeps the
-    smooth part of a correction and drops what the stiffness would amplify.
-    Non-periodic patches only (identity otherwise)."""
+    """c -> P (T c): the projection of a control-point field onto a coarser
+    spline space (``levels`` halvings of every even direction, the multigrid's
+    hierarchy), which keeps the smooth part of a correction and drops what the
+    stiffness would amplify (rough error of amplitude e on a mesh of size h
+    raises the residual like e / h^4 for this strain-gradient operator, so each
+    level cuts that amplification about 16-fold).  Non-periodic directions
+    only (identity otherwise)."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, levels=1):
         self.shape = tuple(mesh.nb)
+        self.levels = int(levels)
         self.ops = []
         for d in range(3):
-            n = mesh.nelem[d]
-            if mesh.periodic[d] or n % 2 or n < 4:
-                self.ops.append(None)
-            else:
-                self.ops.append(coarse_ops_1d(mesh.knots[d], n))
+            n, kf = mesh.nelem[d], np.asarray(mesh.knots[d], dtype=np.float64)
+            P = T = None
+            if not mesh.periodic[d]:
+                for _ in range(self.levels):
+                    if n % 2 or n < 4:
+                        break
+                    p, t, kf = coarse_ops_1d(kf, n, with_knots=True)
+                    P = p if P is None else P @ p
+                    T = t if T is None else t @ T
+                    n //= 2
+            self.ops.append(None if P is None else (P, T))
 
     def __call__(self, c):
         x = np.asarray(c, dtype=np.float64).reshape(self.shape + (3,))
@@ -321,7 +336,8 @@ class LearnedGuess:
 # ------------------------------------------------------------------ driver
 def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
         seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
-        arms=("plain", "linear", "quadratic", "learned"), pc="ilu0", coarse_filter=True):
+        arms=("plain", "linear", "quadratic", "learned"), pc="ilu0", coarse_filter=1, filter_targets=False,
+        neval=None):
     """Load-step comparison: steps 1..ntrain generate the training trajectory
     (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
     differences the network learns are not dominated by solver tolerance),
@@ -340,7 +356,11 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     Every step of every arm is solved to ||R|| <= rtol ||R0||, R0 the residual
     of the previous solution under the step's load (the first residual of the
     reference's iteration), so a better initial iterate is not asked for more;
-    the per-step wall time includes building the guess."""
+    the per-step wall time includes building the guess.
+
+    coarse_filter: levels of the coarse-space projection of the learned term
+    (0: none); filter_targets: train on targets projected the same way;
+    neval: evaluate only the first neval steps after ntrain."""
     from . import assm as am
     from .loadstep import run_load_steps, tip_bc
 
@@ -373,7 +393,8 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         A.close()
         raise RuntimeError(f"training trajectory stopped at load step {len(gen)} of {ntrain} (not converged)")
     mask = A.dirichlet()[0]
-    samples = samples_from_trajectory(U, G, mask)
+    filt = CoarseFilter(mesh, levels=int(coarse_filter)) if coarse_filter else None
+    samples = samples_from_trajectory(U, G, mask, filt=filt if filter_targets else None)
     net = WarmStartNet(grid, seed=seed, device=device)
     t0 = time.perf_counter()
     tr = train_selected(net, samples, epochs=epochs, lr=lr, seed=seed,
@@ -385,7 +406,6 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     # the acceptance metric on the held-out steps: residual norm of the
     # quadratic extrapolation with and without the learned correction
     keepm = (mask.reshape(-1) == 0)
-    filt = CoarseFilter(mesh) if coarse_filter else None
     ratios_val = []
     if tr["correction_enabled"]:
         for i in range(len(samples) - tr["val_samples"], len(samples)):
@@ -406,6 +426,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     t_train = time.perf_counter() - t0
 
     logs, finals = {}, {}
+    last = nsteps if neval is None else min(nsteps, ntrain + int(neval))
     for name in arms:
         A.set_bc(tip_bc(tip_total, ntrain / nsteps))
         if name == "plain":
@@ -415,7 +436,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
             gs.set_previous_load(G[-1])
             opts, log, kf = optsx, gs.log, ksp_factor
         st = run_load_steps(A, nsteps, tip_total=tip_total, guess=gs, opts=opts, start=ntrain + 1, prev=U[-3:],
-                            start_rtol=rtol, ksp_atol_factor=kf,
+                            start_rtol=rtol, ksp_atol_factor=kf, max_steps=last,
                             callback=lambda k, u, x, nm=name: say(f"  {nm} step {k}: newton {x['newton_its']} "
                                                                   f"ksp {x['ksp_its']} {x['wall_ms']:.0f} ms"))
         if name == "plain":
@@ -433,7 +454,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         out = {"steps": len(log), "newton_its": int(sum(r["newton_its"] for r in log)),
                "ksp_its": int(sum(r["ksp_its"] for r in log)),
                "wall_ms_per_step": float(np.mean([r["wall_ms"] for r in log])),
-               "all_converged": bool(all(r["converged"] > 0 for r in log) and len(log) == nsteps - ntrain)}
+               "all_converged": bool(all(r["converged"] > 0 for r in log) and len(log) == last - ntrain)}
         ratios = [r["r_cand"] / r["r_ext"] for r in log if "r_cand" in r and r["r_ext"] > 0]
         if ratios:
             out["accepted_fraction"] = float(np.mean([r.get("accepted", False) for r in log]))
@@ -444,14 +465,14 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     scale = max(1e-300, float(np.max(np.abs(ref))))
     summary = {
         "workload": f"static SVK + gradient beam {nelem[0]}x{nelem[1]}x{nelem[2]}, tip {tip_total} in {nsteps} load "
-                    f"steps, train on 1..{ntrain}, evaluate {ntrain + 1}..{nsteps}",
+                    f"steps, train on 1..{ntrain}, evaluate {ntrain + 1}..{last}",
         "criterion": f"||R|| <= {rtol:g} ||R(U_prev; load_k)|| per step; preconditioner {pc}; arms other than "
                      f"plain: {prec} tangent and preconditioner, Krylov to {ksp_factor:g} x the Newton threshold",
         "grid": list(grid), "train_samples": tr["train_samples"], "val_samples": tr["val_samples"], "epochs": epochs,
         "loss_first": hist[0], "loss_last": hist[-1], "val_loss_best": tr["val_loss_best"],
         "val_loss_zero_correction": tr["val_loss_zero_correction"], "correction_enabled": tr["correction_enabled"],
         "val_residual_ratio_learned_over_quadratic": float(np.median(ratios_val)) if ratios_val else None,
-        "correction_used": use_corr, "coarse_filter": bool(coarse_filter),
+        "correction_used": use_corr, "coarse_filter_levels": int(coarse_filter), "filter_targets": bool(filter_targets),
         "t_generate_s": t_gen, "t_train_s": t_train,
         "generate": {"newton_its": int(sum(x["newton_its"] for x in gen))},
     }
@@ -516,7 +537,7 @@ def run_transient(nelem=(128, 32, 32), nsteps=200, ntrain=100, dt=1e-3, epochs=6
                         log=lambda ep, l, v: say(f"  epoch {ep}: loss {l:.3e} held-out {v:.3e}")
                         if ep % 10 == 0 else None)
     keepm = (mask.reshape(-1) == 0)
-    filt = CoarseFilter(mesh) if coarse_filter else None
+    filt = CoarseFilter(mesh, levels=int(coarse_filter)) if coarse_filter else None
     ratios_val = []
     if tr["correction_enabled"]:
         for i in range(len(samples) - tr["val_samples"], len(samples)):
@@ -612,7 +633,9 @@ def main():
     ap.add_argument("--ksp-factor", type=float, default=0.1)
     ap.add_argument("--arms", default="plain,linear,quadratic,learned")
     ap.add_argument("--pc", default="ilu0", choices=["ilu0", "mg", "jacobi"])
-    ap.add_argument("--coarse-filter", type=int, default=1)
+    ap.add_argument("--coarse-filter", type=int, default=1, help="levels of the coarse-space projection (0: none)")
+    ap.add_argument("--filter-targets", type=int, default=0)
+    ap.add_argument("--neval", type=int, default=0, help="evaluate only this many steps after training (0: all)")
     ap.add_argument("--transient", action="store_true", help="time steps of gradelasttime instead of load steps")
     ap.add_argument("--dt", type=float, default=1e-3)
     a = ap.parse_args()
@@ -620,11 +643,12 @@ def main():
     if a.transient:
         s = run_transient(nel, a.steps, a.train, a.dt, a.epochs, a.lr, a.prec if a.prec != "mixed" else "f64",
                           a.pc if a.pc != "ilu0" else "jacobi", rtol=a.rtol, gen_rtol=a.gen_rtol,
-                          ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), coarse_filter=bool(a.coarse_filter))
+                          ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), coarse_filter=a.coarse_filter)
         print(json.dumps(s), flush=True)
         return
     s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
-            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc, coarse_filter=bool(a.coarse_filter))
+            ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc, coarse_filter=a.coarse_filter,
+            filter_targets=bool(a.filter_targets), neval=a.neval or None)
     print(json.dumps(s), flush=True)
 
 

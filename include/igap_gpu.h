@@ -162,6 +162,24 @@ double *igap_gpu_rvec(igap_gpu_ctx *ctx);
  * vals may be NULL for the context's own arrays. */
 int igap_gpu_tangent(igap_gpu_ctx *ctx, const double *U, void *vals, int prec);
 
+/* Pointwise physics on the device with the context's physics: the Phys
+ * callbacks residual / tangent / density (typedefs residual_ptr, tangent_ptr,
+ * density_ptr, include/physimpl.h:6-9; set to gradelast_* by PhysGradElast,
+ * phys/physgradelast.c:25-28, and to gradelasttime_* by PhysGradElastTime,
+ * phys/physgradelasttime.c:25-28), for npts jets in one launch, by the device
+ * code the assembly kernels use.
+ *   u : npts x 30*(torder+1) doubles, u[30 h + 10 k + a] with h = 0 the
+ *       current level, 1 the previous and 2 the one before (the reference's
+ *       u, u1 = u + 30, u0 = u + 60; jet slot a: value, 3 first and 6 second
+ *       derivatives 11,12,13,22,23,33)
+ *   r : npts x 30, as residual_ptr writes it (NULL: not evaluated)
+ *   C : npts x 900 row-major, index (10 k + a)*30 + (10 l + b), as tangent_ptr
+ *       writes it (NULL: not evaluated)
+ *   hh: npts x 13 (gradelast) or npts x 2 (gradelasttime), as density_ptr
+ *       writes it (NULL: not evaluated)
+ * Device pointers on the context's device; stream-ordered on its stream. */
+int igap_gpu_phys_eval(igap_gpu_ctx *ctx, int64_t npts, const double *u, double *r, double *C, double *hh);
+
 /* ---- Newton-Krylov solve (IGAPSolvInit / IGAPSolvSolve, src/interface.c:
  * 207-250, 302-328: SNES NEWTONLS with BT line search, KSP GMRES right
  * preconditioned, PC ILU(0)) with every vector in HBM. ---------------------- */
@@ -199,7 +217,11 @@ typedef struct {
 
 typedef struct {
     int    newton_its, ksp_its, ls_backtracks, n_residual;
-    int    converged;     /* > 0 converged (2 abs, 3 rel), < 0 diverged (-4 NaN, -5 max it) */
+    int    converged;     /* > 0 converged (2 abs, 3 rel); < 0 diverged, PETSc's codes:      */
+                          /* -3 linear solve not converged within ksp_maxit, -4 NaN,        */
+                          /* -5 snes_maxit reached, -6 line search failed or the stagnation */
+                          /* stop of IGAPSolvSolve (|F| changed < 1e-6 relative in a step,  */
+                          /* src/interface.c:318)                                           */
     double fnorm0, fnorm, last_ksp_relres;
     double t_residual_ms, t_tangent_ms, t_pc_ms, t_ksp_ms, t_total_ms;
 } igap_solver_stats;
@@ -228,6 +250,13 @@ int igap_gpu_memcpy_dtod(igap_gpu_ctx *ctx, void *dst, const void *src, int64_t 
 int igap_gpu_timing_enable(igap_gpu_ctx *ctx, int on);
 int igap_gpu_timing_get(igap_gpu_ctx *ctx, int which, double *ms_total, int64_t *count);
 int igap_gpu_timing_reset(igap_gpu_ctx *ctx);
+
+/* (No reference counterpart.)  Test hook: the n-th device allocation the
+ * library makes from now on (n = 0: the next one) fails as if the device were
+ * out of memory, once; n < 0 disarms it.  For tests that check that a call
+ * failing part way leaves every object usable (the next call rebuilds what
+ * the failed one did not finish). */
+int igap_gpu_debug_fail_alloc(int n);
 
 /* (No reference counterpart.)  Self test of the matrix-core operand layout
  * the tangent kernels assume:

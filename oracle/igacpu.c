@@ -565,11 +565,20 @@ int64_t oc_csr_nnz(const oc_mesh *m)
     return 9 * tot[0] * tot[1] * tot[2];
 }
 
-/* the first nrows rows of the pattern (nrows < 0: all of them) */
-int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t nrows)
+/* rows [row0, row0 + nrows) of the pattern (nrows < 0: to the last row);
+ * rowptr[0] = 0 and positions count from row0 */
+static void row_window(const oc_mesh *m, int64_t *row0, int64_t *nrows)
 {
-    int64_t nd = oc_ndof(m), tot = 0, r = 0;
-    if (nrows < 0 || nrows > nd) nrows = nd;
+    int64_t nd = oc_ndof(m);
+    if (*row0 < 0) *row0 = 0;
+    if (*row0 > nd) *row0 = nd;
+    if (*nrows < 0 || *row0 + *nrows > nd) *nrows = nd - *row0;
+}
+
+int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t row0, int64_t nrows)
+{
+    row_window(m, &row0, &nrows);
+    int64_t tot = 0, r = 0;
     int s[5];
     for (int ax = 0; ax < m->nb[0]; ax++) {
         int nx = stencil1(m, 0, ax, s);
@@ -578,8 +587,8 @@ int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t nrows)
             for (int az = 0; az < m->nb[2]; az++) {
                 int nz = stencil1(m, 2, az, s);
                 for (int i = 0; i < 3; i++, r++) {
-                    if (r >= nrows) return tot;
-                    tot += 3 * nx * ny * nz;
+                    if (r >= row0 + nrows) return tot;
+                    if (r >= row0) tot += 3 * nx * ny * nz;
                 }
             }
         }
@@ -587,10 +596,10 @@ int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t nrows)
     return tot;
 }
 
-void oc_csr_pattern_rows(const oc_mesh *m, int64_t nrows, int64_t *rowptr, int32_t *col)
+void oc_csr_pattern_rows(const oc_mesh *m, int64_t row0, int64_t nrows, int64_t *rowptr, int32_t *col)
 {
-    int64_t nd = oc_ndof(m), r = 0, pos = 0;
-    if (nrows < 0 || nrows > nd) nrows = nd;
+    row_window(m, &row0, &nrows);
+    int64_t r = 0, pos = 0;
     int sx[5], sy[5], sz[5];
     rowptr[0] = 0;
     for (int ax = 0; ax < m->nb[0]; ax++) {
@@ -599,21 +608,22 @@ void oc_csr_pattern_rows(const oc_mesh *m, int64_t nrows, int64_t *rowptr, int32
             int ny = stencil1(m, 1, ay, sy);
             for (int az = 0; az < m->nb[2]; az++) {
                 int nz = stencil1(m, 2, az, sz);
-                for (int i = 0; i < 3; i++) {
-                    if (r >= nrows) return;
+                for (int i = 0; i < 3; i++, r++) {
+                    if (r >= row0 + nrows) return;
+                    if (r < row0) continue;
                     for (int x = 0; x < nx; x++)
                         for (int y = 0; y < ny; y++)
                             for (int z = 0; z < nz; z++)
                                 for (int j = 0; j < 3; j++)
                                     col[pos++] = (int32_t)dof_of_active(m, sx[x], sy[y], sz[z], j);
-                    rowptr[++r] = pos;
+                    rowptr[r - row0 + 1] = pos;
                 }
             }
         }
     }
 }
 
-void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col) { oc_csr_pattern_rows(m, -1, rowptr, col); }
+void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col) { oc_csr_pattern_rows(m, 0, -1, rowptr, col); }
 
 void oc_setnz(const oc_mesh *m, int32_t *d_nz)
 {
@@ -751,6 +761,9 @@ void oc_neumann(const oc_mesh *m, const oc_bc *bc, double *F)
             for (int idof = 0; idof < 3; idof++) {
                 if (bc->bc_type[6 * f + 3 * order + idof] != 1) continue;
                 double t = bc->neumann[6 * f + 3 * order + idof];
+                /* a zero traction adds only -0.0 terms, which leave every sum
+                 * as it is (the reference still integrates them) */
+                if (t == 0.0) continue;
                 for (int ex = 0; ex < m->nelem[d0]; ex++)
                     for (int ey = 0; ey < m->nelem[d1]; ey++) {
                         double X0 = k0[ex + p], X1 = k0[ex + p + 1], Y0 = k1[ey + p], Y1 = k1[ey + p + 1];
@@ -1046,10 +1059,10 @@ static int64_t find_col(const int64_t *rowptr, const int32_t *col, int64_t row, 
 
 void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist,
                 const int64_t *rowptr, const int32_t *col, double *val,
-                const int *box, int apply_bc, int nthreads, int64_t nrows)
+                const int *box, int apply_bc, int nthreads, int64_t row0, int64_t nrows)
 {
     int64_t nd = oc_ndof(m);
-    if (nrows < 0 || nrows > nd) nrows = nd;
+    row_window(m, &row0, &nrows);
     set_threads(nthreads);
     for (int64_t i = 0; i < rowptr[nrows]; i++) val[i] = 0.0;   /* MatZeroEntries :594 */
     int ncx = ncolour1(m, 0), ncy = ncolour1(m, 1), ncz = ncolour1(m, 2);
@@ -1072,9 +1085,9 @@ void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const doub
                             idx[3 * (9 * bx + 3 * by + bz) + k] = dof_of_active(m, ex + bx, ey + by, ez + bz, k);
             /* MatSetValues(...,ADD_VALUES): per row, search each column */
             for (int r = 0; r < 81; r++) {
-                if (idx[r] >= nrows) continue;   /* rows outside the stored prefix */
+                if (idx[r] < row0 || idx[r] >= row0 + nrows) continue;   /* rows outside the stored window */
                 for (int c = 0; c < 81; c++) {
-                    int64_t pos = find_col(rowptr, col, idx[r], (int32_t)idx[c]);
+                    int64_t pos = find_col(rowptr, col, idx[r] - row0, (int32_t)idx[c]);
                     if (pos >= 0) val[pos] += ke[81 * r + c];
                 }
             }
@@ -1089,7 +1102,7 @@ void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const doub
     oc_dirichlet(m, bc, mask, g);
     for (int64_t r = 0; r < nrows; r++)
         for (int64_t p = rowptr[r]; p < rowptr[r + 1]; p++)
-            if (mask[r] || mask[col[p]]) val[p] = (col[p] == r) ? 1.0 : 0.0;
+            if (mask[row0 + r] || mask[col[p]]) val[p] = (col[p] == row0 + r) ? 1.0 : 0.0;
     free(mask);
     free(g);
 }

@@ -90,9 +90,10 @@ int64_t oc_ndof(const oc_mesh *m);
 /* CSR pattern of the AIJ matrix: rows are owned dofs, columns sorted */
 int64_t oc_csr_nnz(const oc_mesh *m);
 void oc_csr_pattern(const oc_mesh *m, int64_t *rowptr, int32_t *col);
-/* the first nrows rows of that pattern (nrows < 0: all) */
-int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t nrows);
-void oc_csr_pattern_rows(const oc_mesh *m, int64_t nrows, int64_t *rowptr, int32_t *col);
+/* rows [row0, row0 + nrows) of that pattern (nrows < 0: to the last row),
+ * rowptr[0] = 0 */
+int64_t oc_csr_nnz_rows(const oc_mesh *m, int64_t row0, int64_t nrows);
+void oc_csr_pattern_rows(const oc_mesh *m, int64_t row0, int64_t nrows, int64_t *rowptr, int32_t *col);
 void oc_setnz(const oc_mesh *m, int32_t *d_nz);   /* assmcore_iga.c:1054-1087, one rank */
 
 /* Dirichlet rows: mask[ndof] (1 if constrained) and g[ndof] (value inserted,
@@ -107,11 +108,11 @@ void oc_neumann(const oc_mesh *m, const oc_bc *bc, double *F);
 void oc_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist,
                  double *R, const int *box, int apply_bc, int nthreads);
 /* Tangent_iga (assmcore_iga.c:507-858) into the CSR pattern of oc_csr_pattern,
- * or of its first nrows rows (oc_csr_pattern_rows; nrows < 0: all rows):
- * contributions to later rows are dropped. */
+ * or of its rows [row0, row0 + nrows) (oc_csr_pattern_rows; nrows < 0: to
+ * the last row): contributions to other rows are dropped. */
 void oc_tangent(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc, const double *U_hist,
                 const int64_t *rowptr, const int32_t *col, double *val,
-                const int *box, int apply_bc, int nthreads, int64_t nrows);
+                const int *box, int apply_bc, int nthreads, int64_t row0, int64_t nrows);
 
 /* per-element kernels (one element, reference local ordering 3*ibpe+idof) */
 void oc_element_residual(const oc_mesh *m, const oc_phys *ph, const oc_bc *bc,

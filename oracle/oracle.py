@@ -201,14 +201,16 @@ def clamped_tip_bc(tip=0.0, tip_dof=2):
 
 
 # ---------------------------------------------------------------- assembly
-def csr_pattern(mesh, nrows=None):
-    """CSR pattern of the tangent (rowptr, col); ``nrows`` keeps only the first
-    rows (a row-prefix pattern for tangents of elements near x = 0)."""
-    nr = mesh.ndof if nrows is None else min(int(nrows), mesh.ndof)
-    nnz = lib().oc_csr_nnz_rows(C.byref(mesh.c), C.c_int64(nr))
+def csr_pattern(mesh, nrows=None, row0=0):
+    """CSR pattern of the tangent (rowptr, col) for the rows [row0, row0 +
+    nrows) (default: every row); a row window serves tangents restricted to
+    the elements near one end of the patch."""
+    r0 = max(0, min(int(row0), mesh.ndof))
+    nr = mesh.ndof - r0 if nrows is None else max(0, min(int(nrows), mesh.ndof - r0))
+    nnz = lib().oc_csr_nnz_rows(C.byref(mesh.c), C.c_int64(r0), C.c_int64(nr))
     rowptr = np.zeros(nr + 1, dtype=np.int64)
     col = np.zeros(nnz, dtype=np.int32)
-    lib().oc_csr_pattern_rows(C.byref(mesh.c), C.c_int64(nr), _p(rowptr, c_i64_p), _p(col, c_i32_p))
+    lib().oc_csr_pattern_rows(C.byref(mesh.c), C.c_int64(r0), C.c_int64(nr), _p(rowptr, c_i64_p), _p(col, c_i32_p))
     return rowptr, col
 
 
@@ -247,16 +249,17 @@ def residual(mesh, ph, bc, U_hist, box=None, apply_bc=True, nthreads=0):
     return R
 
 
-def tangent(mesh, ph, bc, U_hist, pattern=None, box=None, apply_bc=True, nthreads=0):
-    """Tangent_iga into ``pattern`` (default: the full pattern); a row-prefix
-    pattern from csr_pattern(mesh, nrows) keeps only those rows."""
+def tangent(mesh, ph, bc, U_hist, pattern=None, box=None, apply_bc=True, nthreads=0, row0=0):
+    """Tangent_iga into ``pattern`` (default: the full pattern); a row-window
+    pattern from csr_pattern(mesh, nrows, row0) keeps only those rows (pass
+    the same row0)."""
     U = np.ascontiguousarray(U_hist, dtype=np.float64)
     rowptr, col = csr_pattern(mesh) if pattern is None else pattern
     val = np.zeros(len(col))
     b = _box(box)
     lib().oc_tangent(C.byref(mesh.c), C.byref(ph), C.byref(bc), _p(U), _p(rowptr, c_i64_p),
                      _p(col, c_i32_p), _p(val), _p(b, c_int_p) if b is not None else None,
-                     int(apply_bc), int(nthreads), C.c_int64(len(rowptr) - 1))
+                     int(apply_bc), int(nthreads), C.c_int64(int(row0)), C.c_int64(len(rowptr) - 1))
     return rowptr, col, val
 
 

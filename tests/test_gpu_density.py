@@ -33,10 +33,11 @@ def test_intdensity_matches_oracle(igap, oracle, case):
 
 
 def test_volume_kat_unit_cube(igap, oracle):
-    """ex1's KAT (unit cube, periodic, gradelasttime): the second density
-    integrates to the volume 1 (the GPU path needs >= 5 elements per
-    periodic direction, so 5x5x6 stands in for ex1's 3x2x1)."""
-    m = igap.MeshIGABeam(L=(1, 1, 1), nelem=(5, 5, 6), periodic=(1, 1, 1))
+    """ex1's KAT: MeshIGABeam(1,1,1, mref = 3, porder 2, periodic in all three
+    directions) (example/ex1/main.c:23; mesh/igabeam.c:30 gives 2^mref = 8
+    elements per direction) with gradelasttime and the IGAPSolvRand state:
+    the second density integrates to the volume 1."""
+    m = igap.MeshIGABeam(L=(1, 1, 1), nelem=(8, 8, 8), periodic=(1, 1, 1))
     with igap.Assm(m, igap.PhysGradElastTime(), igap.BCDefault()) as A:
         A.set_uhist(oracle.glibc_rand_state(3 * A.ndof))
         HH = A.intdensity()
