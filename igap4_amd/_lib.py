@@ -109,6 +109,7 @@ _SIGS = {
     "igap_gpu_timing_reset": (C.c_int, [C.c_void_p]),
     "igap_gpu_selftest_mfma": (C.c_int, [c_double_p, c_float_p]),
     "igap_gpu_debug_fail_alloc": (C.c_int, [C.c_int]),
+    "igap_gpu_residual_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "igap_gpu_phys_eval": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "igap_gpu_selftest_ilu": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_double_p]),
     "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),

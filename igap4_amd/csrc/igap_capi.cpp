@@ -428,6 +428,8 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     c->ndof = 3 * c->ncp;
     hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete c; return fail(IGAP_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)); }
+    // the no-allocation residual's device array on this device (once per device)
+    HIPTRY(scratch_setup(c->device));
     for (int i = 0; i < 4; i++) {
         he = hipEventCreate(&c->ev[i]);
         if (he != hipSuccess) { igap_gpu_destroy(c); return fail(IGAP_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(he)); }
@@ -696,6 +698,23 @@ extern "C" int igap_gpu_residual(igap_gpu_ctx *c, const double *U, double *R)
         c->t_ms[3] += ms;
         c->t_n[3] += 1;
     }
+    return IGAP_OK;
+}
+
+extern "C" int igap_gpu_residual_noalloc(igap_gpu_ctx *c, const double *U, double *R)
+{
+    if (!c) return fail(IGAP_ERR_ARG, "null context");
+    const int64_t cap = scratch_capacity_doubles();
+    if (81 * c->nel > cap)
+        return fail(IGAP_ERR_ARG, "igap_gpu_residual_noalloc: %lld elements, its device array holds %lld", (long long)c->nel,
+                    (long long)(cap / 81));
+    HIPCHK(hipSetDevice(c->device));
+    double *scr = scratch_device_ptr(c->device);
+    if (!scr) return fail(IGAP_ERR_STATE, "no device array on device %d (contexts set it up)", c->device);
+    if (!R) R = c->d_R;
+    if (U && U != c->d_Uh) HIPCHK(launch_copy_f64(c->d_Uh, U, c->ndof, c->stream));   // U_hist[0] = U (:68)
+    HIPCHK(launch_residual_elem(c->M, c->P, c->d_Uh, c->ndof, scr, (int)c->nel, c->stream));
+    HIPCHK(launch_residual_gather(c->M, scr, c->d_Uh, c->d_F, c->d_mask, c->d_g, R, c->ncp, c->stream));
     return IGAP_OK;
 }
 

@@ -14,6 +14,8 @@
 // Element numbering e = (ex*ny + ey)*nz + ez (the reference's x->y->z loop
 // order); local basis b = 9*bx + 3*by + bz; local dof 3*b + i.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <stdint.h>
 
 #include "igap_device.h"
@@ -1355,6 +1357,43 @@ hipError_t launch_copy_f64(double *dst, const double *src, int64_t n, hipStream_
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- scratch
+// The element vectors of igap_gpu_residual_noalloc live in this file-scope
+// device array: the code object carries it, so every GPU the library runs on
+// holds its own copy (256 MiB) until the process ends, and using it needs no
+// allocation.  Host code must not pass the array's name to a kernel (that is
+// a host-side stand-in): scratch_setup looks up each device's address once,
+// with that device current, and the kernels take that address.
+constexpr int64_t SCRATCH_DOUBLES = (int64_t)32 << 20;   // 81 per element: 414,252 elements
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+hipError_t scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (g_scratch_dev[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;   // the caller makes the device current
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<double *>(ptr);
+    return hipSuccess;
+}
+
+double *scratch_device_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < SCRATCH_MAX_DEVICES) ? g_scratch_dev[device] : nullptr;
+}
+
+int64_t scratch_capacity_doubles() { return SCRATCH_DOUBLES; }
 
 hipError_t launch_phys_eval(const PhysK &P, int64_t npts, const double *u, double *r, double *C, double *hh, hipStream_t s)
 {

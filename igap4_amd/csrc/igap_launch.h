@@ -22,5 +22,10 @@ hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t s);
 hipError_t launch_copy_f64(double *dst, const double *src, int64_t n, hipStream_t s);
 hipError_t launch_selftest_mfma(double *o64, float *o32, hipStream_t s);
 hipError_t launch_phys_eval(const PhysK &P, int64_t npts, const double *u, double *r, double *C, double *hh, hipStream_t s);
+// the no-allocation residual's device array: once per device (current),
+// its address, and its size in doubles
+hipError_t scratch_setup(int device);
+double *scratch_device_ptr(int device);
+int64_t scratch_capacity_doubles();
 
 }  // namespace igap

@@ -153,6 +153,15 @@ const double *igap_gpu_dirichlet_values(const igap_gpu_ctx *ctx);
  * R = volume integral - Neumann load, Dirichlet rows R_d = U_d - g_d.
  * R may be NULL to write the context's own R buffer (igap_gpu_rvec). */
 int igap_gpu_residual(igap_gpu_ctx *ctx, const double *U, double *R);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): igap_gpu_residual with the element vectors in a device
+ * array the library's code object carries (256 MiB on every GPU it runs on,
+ * kept until the process ends), so the call allocates nothing.  At most
+ * 414,252 elements (81 doubles each); a larger patch is IGAP_ERR_ARG.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race and must not be issued. */
+int igap_gpu_residual_noalloc(igap_gpu_ctx *ctx, const double *U, double *R);
 /* the context's residual vector (assm->RR, src/interface.c:187-198) */
 double *igap_gpu_rvec(igap_gpu_ctx *ctx);
 /* Tangent_iga: copies U into U_hist level 0 (U may be NULL to use U_hist as
