@@ -56,7 +56,7 @@ def parse():
                     help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling run for rocprofv3 --pmc: one 256 MiB device copy (the HBM "
-                         "counter calibration) then N residual+tangent calls; prints no JSON")
+                         "counter calibration) then N x (residual + FP64 and FP32 tangent); prints no JSON")
     ap.add_argument("--pmc-json", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="turn the counter_collection.csv of a FETCH_SIZE and a WRITE_SIZE "
                          "--probe run into profiles/tangent_elem_<nelem>_pmc.json")
@@ -72,21 +72,29 @@ def say(msg):
 
 
 def probe(nel, calls):
-    import torch
+    """Counter-collection run: one 256 MiB device-to-device copy (the HBM
+    counter calibration), then ``calls`` x (residual + FP64 tangent + FP32
+    tangent) on the workload."""
+    import ctypes as C
     import igap4_amd as ig
-    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda:0")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    del src, dst
+    from igap4_amd import _lib
     mesh = ig.MeshIGABeam(L=(4.0, 1.0, 1.0), nelem=nel)
     with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.05)) as A:
+        L = _lib.lib()
+        src, dst = C.c_void_p(), C.c_void_p()
+        _lib.check(L.igap_gpu_malloc(A.ctx, C.byref(src), COPY_BYTES))
+        _lib.check(L.igap_gpu_malloc(A.ctx, C.byref(dst), COPY_BYTES))
+        _lib.check(L.igap_gpu_memcpy_dtod(A.ctx, dst, src, COPY_BYTES))
+        A.synchronize()
+        _lib.check(L.igap_gpu_free(A.ctx, src))
+        _lib.check(L.igap_gpu_free(A.ctx, dst))
         A.set_uhist(ig.smooth_state(mesh, amp=0.01))
         for _ in range(calls):
             A.residual_dev(None, None)
             A.tangent_dev(None, None, 0)
+            A.tangent_dev(None, None, 1)
         A.synchronize()
-    print(f"probe done: copy {COPY_BYTES} B, {calls} x (residual + tangent) on {nel}", flush=True)
+    say(f"probe done: copy {COPY_BYTES} B, {calls} x (residual + FP64 and FP32 tangents) on {nel}")
 
 
 def pmc_json(nel, fetch_csv, write_csv):
@@ -363,6 +371,22 @@ def main():
     }
     A.close()
     say(f"bench: R+K {line['value']:.4g} elements/s, {ms64:.1f} ms per step")
+    # the other scatter (element tangents added straight into the CSR values,
+    # no element-matrix buffers): its whole tangent call, for comparison
+    from igap4_amd import _lib
+    with ig.Assm(mesh, phys, bc, device=0, flags=_lib.IGAP_TANGENT_DIRECT) as D:
+        D.set_uhist(U)
+        for prec, key in ((0, "tangent_call_direct_f64"), (1, "tangent_call_direct_f32")):
+            D.tangent_dev(None, None, prec)
+            D.synchronize()
+            D.timing(True)
+            D.timing_reset()
+            for _ in range(3):
+                D.tangent_dev(None, None, prec)
+            ms, n = D.timing_get(4)
+            D.timing(False)
+            line["kernels_ms"][key] = ms / max(1, n)
+    say(f"bench: direct-scatter tangent {line['kernels_ms']['tangent_call_direct_f64']:.1f} ms (FP64)")
     if a.newton_steps > 0:
         line["newton_load_step"] = newton_load_steps(nel, a.newton_steps, nsteps=a.ramp_steps, tip=a.ramp_tip)
     if a.cpu_baseline:

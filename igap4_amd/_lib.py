@@ -21,6 +21,8 @@ IGAP_PHYS_GRADELAST = 0
 IGAP_PHYS_GRADELASTTIME = 2
 IGAP_F64 = 0
 IGAP_F32 = 1
+IGAP_TANGENT_ELEMENT_MATRICES = 1
+IGAP_TANGENT_DIRECT = 2
 
 c_double_p = C.POINTER(C.c_double)
 c_float_p = C.POINTER(C.c_float)

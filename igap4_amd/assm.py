@@ -140,13 +140,15 @@ def BCClampedTip(tip=0.0, tip_dof=2):
 class Assm:
     """IGAPAssmInit + the Residual/Tangent callbacks for one patch on one GPU."""
 
-    def __init__(self, mesh: MeshIGABeam, phys: PhysDesc, bc: BCDesc, device: int = -1):
+    def __init__(self, mesh: MeshIGABeam, phys: PhysDesc, bc: BCDesc, device: int = -1, flags: int = 0):
+        """``flags``: 0 (automatic), _lib.IGAP_TANGENT_ELEMENT_MATRICES or
+        _lib.IGAP_TANGENT_DIRECT (how the tangent reaches the CSR values)."""
         self.mesh = mesh
         self.phys = phys
         self.bc = bc
         ctx = C.c_void_p()
         md = mesh.desc()
-        check(lib().igap_gpu_create(C.byref(md), C.byref(phys), C.byref(bc), device, 0, C.byref(ctx)))
+        check(lib().igap_gpu_create(C.byref(md), C.byref(phys), C.byref(bc), device, int(flags), C.byref(ctx)))
         self.ctx = ctx
         ndof, nnz, nel, tor = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
         check(lib().igap_gpu_sizes(ctx, C.byref(ndof), C.byref(nnz), C.byref(nel), C.byref(tor)))

@@ -396,8 +396,10 @@ static int dalloc(T **p, int64_t n)
 extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc *phys, const igap_bc_desc *bc, int device, int flags,
                                igap_gpu_ctx **out)
 {
-    (void)flags;
     if (!mesh || !phys || !bc || !out) return fail(IGAP_ERR_ARG, "null argument");
+    if (flags & ~(IGAP_TANGENT_ELEMENT_MATRICES | IGAP_TANGENT_DIRECT)) return fail(IGAP_ERR_ARG, "unknown flags 0x%x", flags);
+    if ((flags & IGAP_TANGENT_ELEMENT_MATRICES) && (flags & IGAP_TANGENT_DIRECT))
+        return fail(IGAP_ERR_ARG, "IGAP_TANGENT_ELEMENT_MATRICES and IGAP_TANGENT_DIRECT exclude each other");
     *out = nullptr;
     if (mesh->porder != 2) return fail(IGAP_ERR_ARG, "porder = %d: only quadratic patches (porder 2) are supported", mesh->porder);
     if (phys->kind != IGAP_PHYS_GRADELAST && phys->kind != IGAP_PHYS_GRADELASTTIME)
@@ -423,6 +425,17 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     }
     c->nel = (int64_t)c->nelem[0] * c->nelem[1] * c->nelem[2];
     if (c->nel > 0x7fffffff) { delete c; return fail(IGAP_ERR_ARG, "too many elements"); }
+    {
+        const bool periodic = c->periodic[0] || c->periodic[1] || c->periodic[2];
+        if ((flags & IGAP_TANGENT_DIRECT) && periodic) {
+            delete c;
+            return fail(IGAP_ERR_ARG, "IGAP_TANGENT_DIRECT needs a non-periodic patch");
+        }
+        // 0 automatic (decided at the first tangent call), 1 element
+        // matrices, 2 direct
+        c->tangent_mode = (flags & IGAP_TANGENT_DIRECT) ? 2 : (flags & IGAP_TANGENT_ELEMENT_MATRICES) ? 1 : 0;
+        c->tangent_periodic = periodic ? 1 : 0;
+    }
     c->ncp_active = (int64_t)c->na[0] * c->na[1] * c->na[2];
     c->ncp = (int64_t)c->nb[0] * c->nb[1] * c->nb[2];
     c->ndof = 3 * c->ncp;
@@ -726,6 +739,36 @@ extern "C" int igap_gpu_tangent(igap_gpu_ctx *c, const double *U, void *vals, in
     if (!vals) {
         vals = (prec == IGAP_F64) ? (void *)igap_gpu_csr_val64(c) : (void *)igap_gpu_csr_val32(c);
         if (!vals) return IGAP_ERR_NOMEM;
+    }
+    // automatic choice: element matrices (measured faster: the element
+    // kernel's stores stream out and a separate gather sums them) while their
+    // buffer fits in free device memory with room to spare, direct scatter
+    // into the CSR values otherwise (non-periodic patches)
+    bool direct = c->tangent_mode == 2;
+    if (c->tangent_mode == 0 && !c->d_Ke[prec] && !c->tangent_periodic) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t need = (prec == IGAP_F64 ? sizeof(double) : sizeof(float)) * 6561 * (size_t)c->nel;
+        direct = need + tot / 10 > fr;
+    }
+    if (direct) {
+        // element matrices added straight into the CSR values, colour by colour
+        if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));
+        if (U && U != c->d_Uh) HIPCHK(launch_copy_f64(c->d_Uh, U, c->ndof, c->stream));   // U_hist[0] = U (:557)
+        int rc = t_begin(c, 0);
+        if (rc) return rc;
+        HIPCHK(launch_tangent_direct(c->M, c->P, c->d_Uh, c->ndof, c->d_rowptr, c->d_mask, vals, prec, c->nnz, c->ncp, c->stream));
+        rc = t_end(c, 0, 1);
+        if (rc) return rc;
+        if (c->timing) {
+            HIPCHK(hipEventRecord(c->ev[3], c->stream));
+            HIPCHK(hipEventSynchronize(c->ev[3]));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+            c->t_ms[4] += ms;
+            c->t_n[4] += 1;
+        }
+        return IGAP_OK;
     }
     if (!c->d_Ke[prec]) {
         size_t es = prec == IGAP_F64 ? sizeof(double) : sizeof(float);

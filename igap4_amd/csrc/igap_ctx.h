@@ -60,6 +60,8 @@ struct igap_gpu_ctx {
     hipEvent_t ev[4];
     double t_ms[5] = {0, 0, 0, 0, 0};
     int64_t t_n[5] = {0, 0, 0, 0, 0};
+    int tangent_mode = 0;           // 0 automatic, 1 element matrices + gather, 2 direct into the CSR
+    int tangent_periodic = 0;       // a periodic direction (direct scatter unavailable)
     // Newton-Krylov solver state (created on first use)
     igap::Solver *solver = nullptr;
 };

@@ -15,6 +15,7 @@
 // order); local basis b = 9*bx + 3*by + bz; local dof 3*b + i.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <stdint.h>
 
@@ -500,9 +501,42 @@ __constant__ const int kBlkI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kBlkJ[6] = {0, 1, 2, 1, 2, 2};
 
 
-template <typename T, int NH>
+// Where k_tangent_elem<..., DIRECT = true> adds its element matrix: the
+// elements of one of the 27 colours (ex, ey, ez) = (cx, cy, cz) + 3 (i, j, k)
+// of a non-periodic patch -- no two of them share a control point, so none
+// share a CSR row -- straight into the CSR values, skipping the entries of
+// Dirichlet rows and columns (MatZeroRowsColumns, whose unit diagonal
+// k_dirichlet_diag writes afterwards).
+struct Scatter {
+    const int64_t *rowptr;
+    const int32_t *mask;
+    int c[3];    // colour
+    int nc[3];   // elements of the colour per direction
+};
+
+// CSR position of column 3C + j in the row of control point B that starts at
+// row_start, for control points B, C within +-2 of each other on a
+// non-periodic patch: the column block's slot in the row's sorted stencil
+__device__ __forceinline__ int64_t stencil_pos(const MeshK &M, int64_t row_start, int Bx, int By, int Bz, int Cx, int Cy, int Cz,
+                                               int j)
+{
+    const int lx = Bx - 2 < 0 ? 0 : Bx - 2, ly = By - 2 < 0 ? 0 : By - 2, lz = Bz - 2 < 0 ? 0 : Bz - 2;
+    const int hy = By + 2 > M.nb[1] - 1 ? M.nb[1] - 1 : By + 2, hz = Bz + 2 > M.nb[2] - 1 ? M.nb[2] - 1 : Bz + 2;
+    const int nsy = hy - ly + 1, nsz = hz - lz + 1;
+    return row_start + 3 * (((Cx - lx) * nsy + (Cy - ly)) * nsz + (Cz - lz)) + j;
+}
+
+// the same position for row 3B + i looked up in rowptr
+__device__ __forceinline__ int64_t csr_pos(const MeshK &M, const int64_t *__restrict__ rowptr, int Bx, int By, int Bz, int i,
+                                           int Cx, int Cy, int Cz, int j)
+{
+    const int64_t row = 3 * (((int64_t)M.nb[1] * M.nb[2]) * Bx + (int64_t)M.nb[2] * By + Bz) + i;
+    return stencil_pos(M, rowptr[row], Bx, By, Bz, Cx, Cy, Cz, j);
+}
+
+template <typename T, int NH, bool DIRECT>
 __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
-                                                         T *__restrict__ Ke, int nel)
+                                                         T *__restrict__ Ke, int nel, Scatter S)
 {
     // field levels held per control point: static U; transient the two
     // combinations the rule evaluates, Um = (U^1 + U^0)/2 and Ud = (U - U^0)/2
@@ -542,8 +576,19 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int e = blockIdx.x;
-    const int nz = M.nelem[2], ny = M.nelem[1];
-    const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
+    int ex, ey, ez;
+    if (DIRECT) {
+        // element (i, j, k) of the colour's lattice
+        const int k = e % S.nc[2], j = (e / S.nc[2]) % S.nc[1], i = e / (S.nc[1] * S.nc[2]);
+        ex = S.c[0] + 3 * i;
+        ey = S.c[1] + 3 * j;
+        ez = S.c[2] + 3 * k;
+    } else {
+        const int nz = M.nelem[2], ny = M.nelem[1];
+        ex = e / (ny * nz);
+        ey = (e / nz) % ny;
+        ez = e % nz;
+    }
     const int ed[3] = {ex, ey, ez};
 
     // ---- element data, 1-D tables
@@ -891,9 +936,58 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
         __syncthreads();
     }
 
+    const int c = 16 * ctile + m;
+    if (DIRECT) {
+        // ---- add (row 3b+i, column 3c+j) and the mirrored entry into the CSR
+        //      values, Dirichlet rows and columns left at zero.  The elements
+        //      of one colour share no row, so no other workgroup of the launch
+        //      touches these entries.  The row starts and masks of the
+        //      element's 81 rows are staged in LDS (the geometry buffer is
+        //      free after the last chunk); then every load of the lane is
+        //      issued before its first store, so the epilogue waits for one
+        //      round trip.
+        int64_t *sRow = reinterpret_cast<int64_t *>(sGeo);
+        int32_t *sMsk = reinterpret_cast<int32_t *>(sGeo + 81);
+        if (tid < 81) {
+            const int b = tid / 3, i = tid - 3 * (tid / 3);
+            const int64_t cp = ((int64_t)M.nb[1] * M.nb[2]) * (ex + b / 9) + (int64_t)M.nb[2] * (ey + (b / 3) % 3) + (ez + b % 3);
+            sRow[tid] = S.rowptr[3 * cp + i];
+            sMsk[tid] = S.mask[3 * cp + i];
+        }
+        __syncthreads();
+        if (c < 27) {
+            const int Cx = ex + c / 9, Cy = ey + (c / 3) % 3, Cz = ez + c % 3;
+            T vp[3][8], vq[3][8];
+#pragma unroll
+            for (int t = 0; t < 3; t++)
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int b = 16 * (k >> 2) + Mfma<T>::row(kr, k & 3);
+                    vp[t][k] = vq[t][k] = (T)0;
+                    if (b < 27 && sMsk[3 * c + bj[t]] == 0 && sMsk[3 * b + bi[t]] == 0) {
+                        const int Bx = ex + b / 9, By = ey + (b / 3) % 3, Bz = ez + b % 3;
+                        vp[t][k] = Ke[stencil_pos(M, sRow[3 * b + bi[t]], Bx, By, Bz, Cx, Cy, Cz, bj[t])];
+                        if (bi[t] != bj[t]) vq[t][k] = Ke[stencil_pos(M, sRow[3 * c + bj[t]], Cx, Cy, Cz, Bx, By, Bz, bi[t])];
+                    }
+                }
+#pragma unroll
+            for (int t = 0; t < 3; t++)
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int b = 16 * (k >> 2) + Mfma<T>::row(kr, k & 3);
+                    if (b < 27 && sMsk[3 * c + bj[t]] == 0 && sMsk[3 * b + bi[t]] == 0) {
+                        const int Bx = ex + b / 9, By = ey + (b / 3) % 3, Bz = ez + b % 3;
+                        const T v = acc[t][k >> 2][k & 3];
+                        Ke[stencil_pos(M, sRow[3 * b + bi[t]], Bx, By, Bz, Cx, Cy, Cz, bj[t])] = vp[t][k] + v;
+                        if (bi[t] != bj[t])
+                            Ke[stencil_pos(M, sRow[3 * c + bj[t]], Cx, Cy, Cz, Bx, By, Bz, bi[t])] = vq[t][k] + v;
+                    }
+                }
+        }
+        return;
+    }
     // ---- write K_e (row 3b+i, column 3c+j) and the mirrored block (3c+j, 3b+i)
     T *out = Ke + (int64_t)e * 6561;
-    const int c = 16 * ctile + m;
     if (c < 27) {
 #pragma unroll
         for (int t = 0; t < 3; t++)
@@ -909,6 +1003,21 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
                     }
                 }
     }
+}
+
+// the unit diagonal MatZeroRowsColumns(diag = 1) leaves on the Dirichlet rows
+// of a non-periodic patch (their other entries stayed zero)
+template <typename TV>
+__global__ __launch_bounds__(256) void k_dirichlet_diag(MeshK M, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ mask,
+                                                        TV *__restrict__ vals, int64_t ncp)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= ncp) return;
+    const int nb1 = M.nb[1], nb2 = M.nb[2];
+    const int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if (mask[3 * a + i]) vals[csr_pos(M, rowptr, ax, ay, az, i, ax, ay, az, i)] = (TV)1.0;
 }
 
 // sorted owned control points within +-2 of a (clipped, or wrapped when periodic)
@@ -1312,13 +1421,51 @@ hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh,
                                hipStream_t s)
 {
     dim3 grid(nel), block(256);
+    const Scatter S{};
     if (prec == 0) {
-        if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel);
-        else hipLaunchKernelGGL((k_tangent_elem<double, 3>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel);
+        if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
+        else hipLaunchKernelGGL((k_tangent_elem<double, 3, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
     } else {
-        if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<float, 1>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel);
-        else hipLaunchKernelGGL((k_tangent_elem<float, 3>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel);
+        if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<float, 1, false>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel, S);
+        else hipLaunchKernelGGL((k_tangent_elem<float, 3, false>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel, S);
     }
+    return hipGetLastError();
+}
+
+// Tangent straight into the CSR values of a non-periodic patch: zero the
+// values, one element launch per colour (colours in a fixed order, so every
+// entry sums its contributions in the same order on every call), then the
+// Dirichlet diagonal.
+hipError_t launch_tangent_direct(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, const int64_t *rowptr,
+                                 const int32_t *mask, void *vals, int prec, int64_t nnz, int64_t ncp, hipStream_t s)
+{
+    for (int d = 0; d < 3; d++)
+        if (M.periodic[d]) return hipErrorInvalidValue;
+    const int64_t fb = std::min<int64_t>((nnz + 255) / 256, 65536);
+    if (prec == 0) hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)fb), dim3(256), 0, s, (double *)vals, 0.0, nnz);
+    else hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)fb), dim3(256), 0, s, (float *)vals, 0.0f, nnz);
+    for (int cx = 0; cx < 3; cx++)
+        for (int cy = 0; cy < 3; cy++)
+            for (int cz = 0; cz < 3; cz++) {
+                Scatter S;
+                S.rowptr = rowptr;
+                S.mask = mask;
+                S.c[0] = cx; S.c[1] = cy; S.c[2] = cz;
+                for (int d = 0; d < 3; d++) S.nc[d] = (M.nelem[d] - S.c[d] + 2) / 3;
+                const int64_t n = (int64_t)S.nc[0] * S.nc[1] * S.nc[2];
+                if (n <= 0) continue;
+                dim3 grid((unsigned)n), block(256);
+                if (prec == 0) {
+                    if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
+                    else hipLaunchKernelGGL((k_tangent_elem<double, 3, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
+                } else {
+                    if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<float, 1, true>), grid, block, 0, s, M, P, Uh, ndof, (float *)vals, (int)n, S);
+                    else hipLaunchKernelGGL((k_tangent_elem<float, 3, true>), grid, block, 0, s, M, P, Uh, ndof, (float *)vals, (int)n, S);
+                }
+            }
+    const dim3 g((unsigned)((ncp + 255) / 256)), b(256);
+    if (prec == 0) hipLaunchKernelGGL(k_dirichlet_diag<double>, g, b, 0, s, M, rowptr, mask, (double *)vals, ncp);
+    else hipLaunchKernelGGL(k_dirichlet_diag<float>, g, b, 0, s, M, rowptr, mask, (float *)vals, ncp);
     return hipGetLastError();
 }
 

@@ -15,6 +15,8 @@ hipError_t launch_residual_gather(const MeshK &M, const double *Re, const double
                                   const double *g, double *R, int64_t ncp, hipStream_t s);
 hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, void *Ke, int prec, int nel,
                                hipStream_t s);
+hipError_t launch_tangent_direct(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, const int64_t *rowptr,
+                                 const int32_t *mask, void *vals, int prec, int64_t nnz, int64_t ncp, hipStream_t s);
 hipError_t launch_tangent_gather(const MeshK &M, const void *Ke, int prec, const int64_t *rowptr, const int32_t *mask, void *vals,
                                  int64_t ncp, hipStream_t s);
 hipError_t launch_csr_cols(const MeshK &M, const int64_t *rowptr, int32_t *col, int64_t ncp, hipStream_t s);

@@ -94,10 +94,24 @@ void igap_bc_default(igap_bc_desc *bc);
 int igap_mesh_igabeam(const double L[3], const int nelem[3], const int periodic[3], int perturbed,
                       double *knot0, double *knot1, double *knot2, double *w, double *X);
 
+/* How igap_gpu_tangent turns element tangents into CSR values (flags of
+ * igap_gpu_create; 0 chooses at the first tangent call of each precision:
+ * element matrices, the faster path, while their buffer fits in free device
+ * memory with a tenth of the device to spare, direct scatter otherwise on
+ * non-periodic patches).  Both sum every entry in a fixed order (bitwise
+ * repeatable) and give the same values up to the order of summation.
+ *   IGAP_TANGENT_ELEMENT_MATRICES  the element matrices are written to HBM
+ *       (6561 values per element, allocated on the first tangent call) and a
+ *       gather sums each entry's contributions in ascending element order
+ *   IGAP_TANGENT_DIRECT            the element kernel adds its matrix into the
+ *       CSR values, one launch per class of the 27-colouring (the elements of
+ *       one class share no control point); non-periodic patches only */
+enum { IGAP_TANGENT_ELEMENT_MATRICES = 1, IGAP_TANGENT_DIRECT = 2 };
+
 /* IGAPAssmInit (src/interface.c:131-203) for one GPU: uploads the patch,
  * builds the CSR pattern, the Dirichlet row set and the Neumann load, and
  * allocates U_hist, R and the matrix values.  device < 0 keeps the current
- * device.  flags: reserved, pass 0. */
+ * device.  flags: 0 or one of the IGAP_TANGENT_* choices above. */
 int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc *phys, const igap_bc_desc *bc,
                     int device, int flags, igap_gpu_ctx **ctx);
 /* IGAPFinalize (src/interface.c:372-435) for the context: frees everything

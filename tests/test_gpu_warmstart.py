@@ -1,8 +1,8 @@
 """Learned warm start on the GPU (include/igap_ws.h): the BF16 matrix-core
 layout, forward values and gradients against the CPU oracle with the same
 BF16 rounding (oracle/convnet.py), Adam against the oracle's Adam, training
-that lowers the loss, and the load-step ablation: the learned guess is used
-only when its residual beats extrapolation, and on a nonlinear ramp it does."""
+that lowers the loss, and the load-step ablation of the learned term against
+the quadratic extrapolation it corrects."""
 import numpy as np
 import pytest
 
@@ -57,7 +57,9 @@ def test_forward_matches_oracle(ws):
     y_ref, _ = cn.forward(p, x, GRID, CIN, COUT, DIL, emulate=True)
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
     print(f"  forward rel err {err:.2e}")
-    assert err < 5e-3
+    # the same BF16 roundings on both sides: only the FP32 summation order
+    # differs (measured ~1e-7); a dropped tap or a wrong dilation is O(1)
+    assert err < 1e-5
 
 
 def test_gradients_match_oracle(ws):
@@ -77,7 +79,7 @@ def test_gradients_match_oracle(ws):
             a, b = g[sl], g_ref[sl]
             err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
             print(f"  layer {l} d{name} rel err {err:.2e}")
-            assert err < 2e-2
+            assert err < 1e-5
 
 
 def test_adam_step_matches_oracle(ws):
@@ -122,15 +124,32 @@ def test_warm_start_ablation_small_beam(ws, igap, solver):
     the same state with the previous solution (plain FP64), linear and
     quadratic extrapolation and the learned guess (FP32 tangent with ILU(0),
     or FP64 tangent with the FP32 multigrid preconditioner).  Every arm
-    reaches the plain FP64 answer to 1e-8; the learned candidate is accepted
-    against linear extrapolation on most steps, with a smaller residual."""
+    reaches the plain FP64 answer to 1e-8.
+
+    Learned against quadratic (the same candidate without the network): the
+    learned term is used only if it lowered the residual of the quadratic
+    extrapolation on the held-out training steps.  When it is used, the
+    learned arm must need no more Krylov iterations than the quadratic arm
+    on the evaluated steps (what the network gains; the candidates' residual
+    ratios are printed beside); when the held-out test switches it off, the
+    learned arm must be the quadratic arm exactly (same iterates, same
+    counts)."""
     pc, prec = solver.split("-")
-    s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=80, lr=2e-3, verbose=False, pc=pc, prec=prec)
+    s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=80, lr=2e-3, verbose=False, pc=pc,
+               prec=prec, coarse_filter=2)
     for arm in ("plain", "linear", "quadratic", "learned"):
         print(arm, s[arm])
         assert s[arm]["all_converged"]
         assert s[arm]["final_rel_diff_vs_plain"] < 1e-8
-    print({k: s[k] for k in ("loss_first", "loss_last")})
+    print({k: s[k] for k in ("loss_first", "loss_last", "val_loss_best", "val_loss_zero_correction",
+                             "val_residual_ratio_learned_over_quadratic", "correction_used")})
     assert s["loss_last"] < s["loss_first"]
-    assert s["learned"]["accepted_fraction"] > 0.5
-    assert s["learned"]["median_residual_ratio_candidate_over_linear"] < 1.0
+    q, l = s["quadratic"], s["learned"]
+    rq, rl = q["median_residual_ratio_candidate_over_linear"], l["median_residual_ratio_candidate_over_linear"]
+    print(f"learned vs quadratic on steps 31..60: candidate residual / linear {rl:.3g} vs {rq:.3g}; "
+          f"newton {l['newton_its']} vs {q['newton_its']}, ksp {l['ksp_its']} vs {q['ksp_its']}")
+    if s["correction_used"]:
+        assert l["ksp_its"] <= q["ksp_its"]
+    else:
+        assert rl == rq and l["newton_its"] == q["newton_its"] and l["ksp_its"] == q["ksp_its"]
+        assert l["final_rel_diff_vs_plain"] == q["final_rel_diff_vs_plain"]
