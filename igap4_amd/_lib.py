@@ -23,6 +23,8 @@ IGAP_F64 = 0
 IGAP_F32 = 1
 IGAP_TANGENT_ELEMENT_MATRICES = 1
 IGAP_TANGENT_DIRECT = 2
+IGAP_PC_APPLY_FULL = 0
+IGAP_PC_APPLY_SMOOTHER = 1
 
 c_double_p = C.POINTER(C.c_double)
 c_float_p = C.POINTER(C.c_float)
@@ -116,6 +118,7 @@ _SIGS = {
     "igap_gpu_selftest_ilu": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), c_double_p]),
     "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),
     "igap_gpu_snes_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
+    "igap_gpu_pc_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     # include/igap_ws.h
     "igap_ws_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_uint64,
                                  C.POINTER(C.c_void_p)]),

@@ -303,6 +303,26 @@ class Assm:
         check(lib().igap_gpu_snes_solve(self.ctx, C.byref(opts), C.byref(st)))
         return st.as_dict()
 
+    def pc_apply(self, b, which="full"):
+        """x = M^-1 b with the preconditioner of the last solve (PCApply);
+        ``which`` "smoother": the multigrid's finest-level smoother only.
+        Host vectors in and out."""
+        w = _lib.IGAP_PC_APPLY_FULL if which == "full" else _lib.IGAP_PC_APPLY_SMOOTHER
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        ptrs = []
+        try:
+            for _ in range(2):
+                p = C.c_void_p()
+                check(lib().igap_gpu_malloc(self.ctx, C.byref(p), 8 * self.ndof))
+                ptrs.append(p)
+            self.htod(ptrs[0], b)
+            check(lib().igap_gpu_pc_apply(self.ctx, w, ptrs[0], ptrs[1]))
+            self.synchronize()
+            return self.dtoh(ptrs[1], self.ndof)
+        finally:
+            for p in ptrs:
+                lib().igap_gpu_free(self.ctx, p)
+
     # ------------------------------------------------------------ timing
     def timing(self, on=True):
         check(lib().igap_gpu_timing_enable(self.ctx, 1 if on else 0))

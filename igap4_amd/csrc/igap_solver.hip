@@ -72,6 +72,8 @@ struct Solver {
                                     // direction whose size is not a multiple of 3): colour sweeps valid
     int mc_block_ok = 0;            // point-block pattern (k_mcb_* applicable)
     int mc_ready = 0;               // the coloured ordering is complete
+    // the preconditioner of the last refresh (igap_gpu_pc_apply)
+    int pc_ready = 0, pc_kind = 0, pc_prec = 0, pc_ord = 0;
     int32_t *mc_pcoord = nullptr;   // per permuted point: owned-grid (i, j, k) packed 10 bits each
     int *zpiv = nullptr;            // first row with a zero pivot of the last own factorisation
     double *part = nullptr;         // partial dot products, (m+2) x ceil(n / DOT_CHUNK)
@@ -1970,6 +1972,12 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         }
         HIPCHK(hipStreamSynchronize(c->stream));
         st->t_pc_ms += ms(t, clk::now());
+        if (refactor) {
+            s->pc_ready = 1;
+            s->pc_kind = o->pc;
+            s->pc_prec = pp;
+            s->pc_ord = o->pc_order;
+        }
         // solve K d = -F
         t = clk::now();
         hipLaunchKernelGGL(k_axpby, grid1(n), dim3(256), 0, c->stream, n, -1.0, s->F, 0.0, s->Ft);
@@ -2040,6 +2048,24 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     HIPCHK(hipStreamSynchronize(c->stream));
     st->t_total_ms = ms(T0, clk::now());
     return IGAP_OK;
+}
+
+// PCApply (and, for the multigrid, KSPSolve on its finest level's smoother)
+// with the preconditioner the last Newton solve set up
+extern "C" int igap_gpu_pc_apply(igap_gpu_ctx *c, int which, const double *b, double *x)
+{
+    if (!c || !b || !x) return fail(IGAP_ERR_ARG, "null argument");
+    if (which != IGAP_PC_APPLY_FULL && which != IGAP_PC_APPLY_SMOOTHER) return fail(IGAP_ERR_ARG, "which = %d", which);
+    if (b == x) return fail(IGAP_ERR_ARG, "b and x must differ");
+    HIPCHK(hipSetDevice(c->device));
+    Solver *s = c->solver;
+    if (!s || !s->pc_ready) return fail(IGAP_ERR_STATE, "no preconditioner set up yet (igap_gpu_snes_solve builds it)");
+    if (which == IGAP_PC_APPLY_SMOOTHER) {
+        if (s->pc_kind != IGAP_PC_MG || !s->mg) return fail(IGAP_ERR_STATE, "the smoother belongs to IGAP_PC_MG");
+        // the finest level always smooths (only a coarser one is ever dense)
+        return mg_cheb(c, s->mg, 0, s->pc_prec, b, x, 1, MG_CHEB_DEG);
+    }
+    return pc_apply(c, s, s->pc_kind, s->pc_prec, s->pc_ord, b, x);
 }
 
 // Self test of the point-block ILU(0): factor the context's current tangent

@@ -285,13 +285,23 @@ class CoarseFilter:
         return x.reshape(-1)
 
 
+def smooth_candidate(A, c, H):
+    """c - S R(H): one application of the multigrid's finest-level smoother
+    (igap_gpu_pc_apply, the lagged preconditioner of the last solve) to the
+    residual of the candidate state H.  It damps the rough part of a
+    candidate's error, which the stiffness amplifies in the residual, and
+    leaves the smooth part, where extrapolation and the network differ."""
+    d = A.pc_apply(A.residual(H), which="smoother")
+    return c - d
+
+
 class LearnedGuess:
     """``guess(k, prev)`` for run_load_steps.  mode "learned": quadratic
     extrapolation + network, used when its residual norm is below the linear
     extrapolation's; "quadratic": the same candidate without the network,
     same acceptance; "linear": the linear extrapolation always."""
 
-    def __init__(self, A, net=None, mode="learned", use_correction=True, filt=None, levels=None):
+    def __init__(self, A, net=None, mode="learned", use_correction=True, filt=None, levels=None, smooth=False):
         if mode not in ("learned", "quadratic", "linear"):
             raise ValueError(f"unknown mode {mode!r}")
         if mode == "learned" and net is None:
@@ -299,6 +309,7 @@ class LearnedGuess:
         self.A, self.net, self.mode = A, net, mode
         self.use_correction = use_correction
         self.filt = filt
+        self.smooth = smooth
         # U_hist of a candidate state (transient: [c, U_n, U_{n-1}])
         self.levels = levels or (lambda c, prev: c)
         self.mask = A.dirichlet()[0].reshape(-1)
@@ -313,7 +324,7 @@ class LearnedGuess:
         g = self.A.dirichlet()[1]
         rec = {"step": k}
         ext = 2.0 * prev[-1] - prev[-2] if len(prev) >= 2 else prev[-1].copy()
-        guess = ext
+        cand = None
         if self.mode != "linear" and len(prev) >= 3 and self.g_prev is not None:
             dU1, dU2 = prev[-1] - prev[-2], prev[-2] - prev[-3]
             cand = ext + (dU1 - dU2) * self.keep
@@ -323,6 +334,12 @@ class LearnedGuess:
                 if self.filt is not None:
                     corr = self.filt(corr)
                 cand = cand + corr * s2 * self.keep
+        if self.smooth:
+            ext = smooth_candidate(self.A, ext, self.levels(ext, prev))
+            if cand is not None:
+                cand = smooth_candidate(self.A, cand, self.levels(cand, prev))
+        guess = ext
+        if cand is not None:
             r_ext = float(np.linalg.norm(self.A.residual(self.levels(ext, prev))))
             r_c = float(np.linalg.norm(self.A.residual(self.levels(cand, prev))))
             rec.update(r_ext=r_ext, r_cand=r_c, accepted=bool(r_c < r_ext))
@@ -337,7 +354,7 @@ class LearnedGuess:
 def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
         seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
         arms=("plain", "linear", "quadratic", "learned"), pc="ilu0", coarse_filter=1, filter_targets=False,
-        neval=None):
+        neval=None, smooth=False):
     """Load-step comparison: steps 1..ntrain generate the training trajectory
     (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
     differences the network learns are not dominated by solver tolerance),
@@ -360,7 +377,12 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
 
     coarse_filter: levels of the coarse-space projection of the learned term
     (0: none); filter_targets: train on targets projected the same way;
-    neval: evaluate only the first neval steps after ntrain."""
+    neval: evaluate only the first neval steps after ntrain; smooth: every
+    candidate of the extrapolating arms (and of the held-out test) is
+    smoothed once by the multigrid's finest-level smoother before the
+    residual comparison (pc "mg" only)."""
+    if smooth and pc != "mg":
+        raise ValueError("smooth needs pc='mg' (the multigrid's smoother)")
     from . import assm as am
     from .loadstep import run_load_steps, tip_bc
 
@@ -418,6 +440,9 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
             if filt is not None:
                 corr = filt(corr)
             cl = cq + corr * s2 * keepm
+            if smooth:
+                cq = smooth_candidate(A, cq, cq)
+                cl = smooth_candidate(A, cl, cl)
             rq = float(np.linalg.norm(A.residual(cq)))
             ratios_val.append(float(np.linalg.norm(A.residual(cl))) / rq if rq > 0 else 1.0)
     use_corr = bool(tr["correction_enabled"] and ratios_val and np.median(ratios_val) < 1.0)
@@ -432,7 +457,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         if name == "plain":
             gs, opts, log, kf = "previous", opts64, [], None
         else:
-            gs = LearnedGuess(A, net, mode=name, use_correction=use_corr, filt=filt)
+            gs = LearnedGuess(A, net, mode=name, use_correction=use_corr, filt=filt, smooth=smooth)
             gs.set_previous_load(G[-1])
             opts, log, kf = optsx, gs.log, ksp_factor
         st = run_load_steps(A, nsteps, tip_total=tip_total, guess=gs, opts=opts, start=ntrain + 1, prev=U[-3:],
@@ -473,6 +498,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         "val_loss_zero_correction": tr["val_loss_zero_correction"], "correction_enabled": tr["correction_enabled"],
         "val_residual_ratio_learned_over_quadratic": float(np.median(ratios_val)) if ratios_val else None,
         "correction_used": use_corr, "coarse_filter_levels": int(coarse_filter), "filter_targets": bool(filter_targets),
+        "smooth": bool(smooth),
         "t_generate_s": t_gen, "t_train_s": t_train,
         "generate": {"newton_its": int(sum(x["newton_its"] for x in gen))},
     }
@@ -636,6 +662,7 @@ def main():
     ap.add_argument("--coarse-filter", type=int, default=1, help="levels of the coarse-space projection (0: none)")
     ap.add_argument("--filter-targets", type=int, default=0)
     ap.add_argument("--neval", type=int, default=0, help="evaluate only this many steps after training (0: all)")
+    ap.add_argument("--smooth", type=int, default=0, help="smooth every candidate once before the residual test")
     ap.add_argument("--transient", action="store_true", help="time steps of gradelasttime instead of load steps")
     ap.add_argument("--dt", type=float, default=1e-3)
     a = ap.parse_args()
@@ -648,7 +675,7 @@ def main():
         return
     s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
             ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc, coarse_filter=a.coarse_filter,
-            filter_targets=bool(a.filter_targets), neval=a.neval or None)
+            filter_targets=bool(a.filter_targets), neval=a.neval or None, smooth=bool(a.smooth))
     print(json.dumps(s), flush=True)
 
 

@@ -255,6 +255,17 @@ int igap_gpu_solver_default_opts(igap_solver_opts *opts);
  * the last iterate and the context's R buffer its residual.  Synchronous. */
 int igap_gpu_snes_solve(igap_gpu_ctx *ctx, const igap_solver_opts *opts, igap_solver_stats *stats);
 
+/* PCApply (PETSc's, on the PC of IGAP4's KSP, src/interface.c:224-239) with
+ * the preconditioner the last igap_gpu_snes_solve set up (lagged: the
+ * tangent of its last refresh): x = M^-1 b.  IGAP_PC_APPLY_SMOOTHER applies
+ * only the multigrid's finest-level smoother from x = 0 (KSPSolve on the
+ * level-0 smoother of a PCMG): degree-3 Chebyshev with 3x3 point-block
+ * Jacobi, which damps the components of b above 2% of the spectrum and
+ * leaves the smooth ones.  Device vectors of ndof doubles, b != x;
+ * stream-ordered.  IGAP_ERR_STATE before the first solve. */
+enum { IGAP_PC_APPLY_FULL = 0, IGAP_PC_APPLY_SMOOTHER = 1 };
+int igap_gpu_pc_apply(igap_gpu_ctx *ctx, int which, const double *b, double *x);
+
 /* ---- helpers (no reference counterpart: device memory for callers that
  * hold no HIP runtime of their own; all on the context's device and stream,
  * the copies synchronous) -------------------------------------------------- */
