@@ -744,13 +744,17 @@ extern "C" int igap_gpu_tangent(igap_gpu_ctx *c, const double *U, void *vals, in
     // kernel's stores stream out and a separate gather sums them) while their
     // buffer fits in free device memory with room to spare, direct scatter
     // into the CSR values otherwise (non-periodic patches)
-    bool direct = c->tangent_mode == 2;
-    if (c->tangent_mode == 0 && !c->d_Ke[prec] && !c->tangent_periodic) {
-        size_t fr = 0, tot = 0;
-        HIPCHK(hipMemGetInfo(&fr, &tot));
-        const size_t need = (prec == IGAP_F64 ? sizeof(double) : sizeof(float)) * 6561 * (size_t)c->nel;
-        direct = need + tot / 10 > fr;
+    if (c->tangent_mode == 0 && c->tangent_choice[prec] == 0) {   // decided once per precision
+        int choice = 1;
+        if (!c->tangent_periodic) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            const size_t need = (prec == IGAP_F64 ? sizeof(double) : sizeof(float)) * 6561 * (size_t)c->nel;
+            if (need + tot / 10 > fr) choice = 2;
+        }
+        c->tangent_choice[prec] = choice;
     }
+    const bool direct = c->tangent_mode == 2 || (c->tangent_mode == 0 && c->tangent_choice[prec] == 2);
     if (direct) {
         // element matrices added straight into the CSR values, colour by colour
         if (c->timing) HIPCHK(hipEventRecord(c->ev[2], c->stream));

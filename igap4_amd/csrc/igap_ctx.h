@@ -62,6 +62,7 @@ struct igap_gpu_ctx {
     int64_t t_n[5] = {0, 0, 0, 0, 0};
     int tangent_mode = 0;           // 0 automatic, 1 element matrices + gather, 2 direct into the CSR
     int tangent_periodic = 0;       // a periodic direction (direct scatter unavailable)
+    int tangent_choice[2] = {0, 0}; // automatic mode, per precision: 0 undecided, 1 element matrices, 2 direct
     // Newton-Krylov solver state (created on first use)
     igap::Solver *solver = nullptr;
 };
