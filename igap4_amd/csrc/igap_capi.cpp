@@ -383,6 +383,23 @@ static int dalloc(T **p, int64_t n)
             return r_;               \
         }                            \
     } while (0)
+// A separable patch: every control coordinate x_d depends on the control
+// point's (active) index along d alone, and all weights are equal (bitwise).
+// Then x_d = f_d(xi_d) on every element, G is diagonal, the rational basis is
+// the B-spline basis and k_tangent_sep applies.
+static int separable_patch(const igap_gpu_ctx *c)
+{
+    const int64_t n1 = c->na[1], n2 = c->na[2];
+    for (int64_t id = 0; id < c->ncp_active; id++) {
+        if (c->w_h[id] != c->w_h[0]) return 0;
+        const int64_t a0 = id / (n1 * n2), a1 = (id / n2) % n1, a2 = id % n2;
+        const int64_t base[3] = {a0 * n1 * n2, a1 * n2, a2};   // the point with the other two indices 0
+        for (int d = 0; d < 3; d++)
+            if (c->X_h[3 * id + d] != c->X_h[3 * base[d] + d]) return 0;
+    }
+    return 1;
+}
+
 // a HIP call inside igap_gpu_create: release the partly built context c on failure
 #define HIPTRY(x)                                                                                    \
     do {                                                                                             \
@@ -397,7 +414,8 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
                                igap_gpu_ctx **out)
 {
     if (!mesh || !phys || !bc || !out) return fail(IGAP_ERR_ARG, "null argument");
-    if (flags & ~(IGAP_TANGENT_ELEMENT_MATRICES | IGAP_TANGENT_DIRECT)) return fail(IGAP_ERR_ARG, "unknown flags 0x%x", flags);
+    if (flags & ~(IGAP_TANGENT_ELEMENT_MATRICES | IGAP_TANGENT_DIRECT | IGAP_TANGENT_GENERAL_GEOMETRY))
+        return fail(IGAP_ERR_ARG, "unknown flags 0x%x", flags);
     if ((flags & IGAP_TANGENT_ELEMENT_MATRICES) && (flags & IGAP_TANGENT_DIRECT))
         return fail(IGAP_ERR_ARG, "IGAP_TANGENT_ELEMENT_MATRICES and IGAP_TANGENT_DIRECT exclude each other");
     *out = nullptr;
@@ -493,6 +511,7 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     M.w = c->d_w;
     M.X = c->d_X;
     gauss4(M.gq, M.gw);
+    M.separable = (flags & IGAP_TANGENT_GENERAL_GEOMETRY) ? 0 : separable_patch(c);
 
     // CSR pattern: rows 3*cp + i, row length 3 * prod_d stencil_d
     std::vector<int64_t> rowptr(c->ndof + 1);

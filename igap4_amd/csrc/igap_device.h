@@ -30,6 +30,7 @@ struct MeshK {
     const double *X;  // 3 * active
     double par_periodic[9];
     double gq[4], gw[4];   // Gauss-Legendre nodes/weights on [0,1]
+    int separable;    // x_d depends on xi_d alone and the weights are equal (k_tangent_sep applies)
 };
 
 struct PhysK {

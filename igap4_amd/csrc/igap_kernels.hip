@@ -501,6 +501,136 @@ __constant__ const int kBlkI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kBlkJ[6] = {0, 1, 2, 1, 2, 2};
 
 
+// Stage A of the element tangent (256 threads): the Gauss-point fields by
+// sum factorisation from the control points in sCP and the 1-D tables in sT.
+// Static physics: the strain-measure state of all 64 points -> sSt[q][STW];
+// transient: the physical u-jets of Um and Ud -> sJet[q][h][k][a].  Per point
+// sGeo[q][GEOK] = wgt (= Gauss weight x span lengths x det G), then, when
+// GEOK allows, G^-1 (9), d2x/dxi2 (18) and the weight jet W (10).  sWork
+// holds the field buffers.  Ends with a block barrier.
+template <int NH, int GEOK>
+__device__ __forceinline__ void tangent_fields(const MeshK &M, const PhysK &P, const int *ed, double (*sT)[4][3][3],
+                                               const double *sCP, double *sWork, double *sGeo, double *sJet, double *sSt)
+{
+    constexpr int NL = (NH == 1) ? 1 : 2;
+    constexpr int STR = 4 + 3 * NL;
+    constexpr int NF = 4 + 3 * NL;             // fields w, w X (3), w U (3 per held level)
+    constexpr int NFP = (NH == 1) ? NF : 6;    // fields per sum-factorisation pass
+    const int tid = threadIdx.x;
+    // ---- stage A: fields at the 64 Gauss points by sum factorisation.
+    //      The fields P_f = sum_b c_{b,f} N_b with c = w, w X (3), w U (3 per
+    //      level) are contracted one direction at a time (z, then y, then x:
+    //      27 -> 108 -> 288 -> 640 values per field); the rational jets follow
+    //      by the quotient rule on P_f / W and the physical jets by the chain
+    //      rule, both linear in the basis, so the result is the reference's
+    //      sum over b of c_b R_b in another order of summation.
+    {
+        double *sA = sWork, *sB = sA + NFP * 108;
+        sf_zy<(NH == 1 ? NF : 4), STR>(sT, sCP, sA, sB);
+
+        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
+        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+        const double *tx = &sT[0][qd[0]][0][0];
+        const int boff = (4 * qd[1] + qd[2]) * 6;
+        double W[10], Jg[3][10];
+        sf_jet(sB + boff, tx, W);
+        // part 0: the three geometry fields; part 1 + k: displacement
+        // component k of every held level (transient: second pass below)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int f = (part == 0) ? 1 + j : 4 + 3 * j + part - 1;
+            if (part == 0 || (NH == 1 && j < NL)) {
+                double Pj[10];
+                sf_jet(sB + 288 * f + boff, tx, Pj);
+                nurbs_param(Pj, 1.0, W, Jg[j]);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 10; a++) Jg[j][a] = (a >= 1 && a <= 3 && a - 1 == j) ? 1.0 : 0.0;
+            }
+        }
+        double G[9], Gi[9], X2[18];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) G[3 * i + j] = Jg[i][1 + j];
+        const double detG = inv3(G, Gi);
+#pragma unroll
+        for (int i = 0; i < 9; i++) Gi[i] = __shfl(Gi[i], lq, 64);
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = __shfl(Jg[d][4 + s2], lq, 64);
+        double xq[3], dX[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double *kV = M.knot[d] + ed[d];
+            const double c = M.gq[qd[d]];
+            xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
+            dX[d] = kV[3] - kV[2];
+        }
+        if (NH != 1) {
+            // second pass: the fields w Um (3) and w Ud (3)
+            __syncthreads();
+            sf_zy<6, STR, 4>(sT, sCP, sA, sB);
+            if (part > 0) {
+#pragma unroll
+                for (int h = 0; h < NL; h++) {
+                    double Pj[10];
+                    sf_jet(sB + 288 * (3 * h + part - 1) + boff, tx, Pj);
+                    nurbs_param(Pj, 1.0, W, Jg[h]);
+                }
+            }
+        }
+        if (part > 0) {
+            const int k = part - 1;
+#pragma unroll
+            for (int h = 0; h < NL; h++) {
+                phys_jet(Jg[h], Gi, X2);
+                // the periodic macro-field shifts every level alike: Um takes
+                // it, the difference Ud does not
+                if (h == 0) add_periodic1(M, xq, k, Jg[h]);
+                if (NH != 1) {
+#pragma unroll
+                    for (int a = 0; a < 10; a++) sJet[q * 30 * NL + 30 * h + 10 * k + a] = Jg[h][a];
+                }
+            }
+        }
+        if (NH == 1) {
+            double u[30];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int a = 0; a < 10; a++) u[10 * k + a] = __shfl(Jg[0][a], lq + 1 + k, 64);
+            if (part == 0) {
+                State st;
+                phys_state(u, P, st);
+                double *sp = sSt + q * STW;
+#pragma unroll
+                for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+#pragma unroll
+                for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+#pragma unroll
+                for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
+            }
+        }
+        if (part == 0) {
+            double *g = sGeo + q * GEOK;
+            g[0] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
+            if (GEOK > 1) {
+#pragma unroll
+                for (int i = 0; i < 9; i++) g[1 + i] = Gi[i];
+#pragma unroll
+                for (int i = 0; i < 18; i++) g[10 + i] = X2[i];
+            }
+            if (GEOK > 28) {
+#pragma unroll
+                for (int i = 0; i < 10; i++) g[28 + i] = W[i];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Where k_tangent_elem<..., DIRECT = true> adds its element matrix: the
 // elements of one of the 27 colours (ex, ey, ez) = (cx, cy, cz) + 3 (i, j, k)
 // of a non-periodic patch -- no two of them share a control point, so none
@@ -622,116 +752,7 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
     }
     __syncthreads();
 
-    // ---- stage A: fields at the 64 Gauss points by sum factorisation.
-    //      The fields P_f = sum_b c_{b,f} N_b with c = w, w X (3), w U (3 per
-    //      level) are contracted one direction at a time (z, then y, then x:
-    //      27 -> 108 -> 288 -> 640 values per field); the rational jets follow
-    //      by the quotient rule on P_f / W and the physical jets by the chain
-    //      rule, both linear in the basis, so the result is the reference's
-    //      sum over b of c_b R_b in another order of summation.
-    {
-        double *sA = reinterpret_cast<double *>(sWork), *sB = sA + NFP * 108;
-        sf_zy<(NH == 1 ? NF : 4), STR>(sT, sCP, sA, sB);
-
-        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
-        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
-        const double *tx = &sT[0][qd[0]][0][0];
-        const int boff = (4 * qd[1] + qd[2]) * 6;
-        double W[10], Jg[3][10];
-        sf_jet(sB + boff, tx, W);
-        // part 0: the three geometry fields; part 1 + k: displacement
-        // component k of every held level (transient: second pass below)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int f = (part == 0) ? 1 + j : 4 + 3 * j + part - 1;
-            if (part == 0 || (NH == 1 && j < NL)) {
-                double Pj[10];
-                sf_jet(sB + 288 * f + boff, tx, Pj);
-                nurbs_param(Pj, 1.0, W, Jg[j]);
-            } else {
-#pragma unroll
-                for (int a = 0; a < 10; a++) Jg[j][a] = (a >= 1 && a <= 3 && a - 1 == j) ? 1.0 : 0.0;
-            }
-        }
-        double G[9], Gi[9], X2[18];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) G[3 * i + j] = Jg[i][1 + j];
-        const double detG = inv3(G, Gi);
-#pragma unroll
-        for (int i = 0; i < 9; i++) Gi[i] = __shfl(Gi[i], lq, 64);
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int s2 = 0; s2 < 6; s2++) X2[6 * d + s2] = __shfl(Jg[d][4 + s2], lq, 64);
-        double xq[3], dX[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double *kV = M.knot[d] + ed[d];
-            const double c = M.gq[qd[d]];
-            xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
-            dX[d] = kV[3] - kV[2];
-        }
-        if (NH != 1) {
-            // second pass: the fields w Um (3) and w Ud (3)
-            __syncthreads();
-            sf_zy<6, STR, 4>(sT, sCP, sA, sB);
-            if (part > 0) {
-#pragma unroll
-                for (int h = 0; h < NL; h++) {
-                    double Pj[10];
-                    sf_jet(sB + 288 * (3 * h + part - 1) + boff, tx, Pj);
-                    nurbs_param(Pj, 1.0, W, Jg[h]);
-                }
-            }
-        }
-        if (part > 0) {
-            const int k = part - 1;
-#pragma unroll
-            for (int h = 0; h < NL; h++) {
-                phys_jet(Jg[h], Gi, X2);
-                // the periodic macro-field shifts every level alike: Um takes
-                // it, the difference Ud does not
-                if (h == 0) add_periodic1(M, xq, k, Jg[h]);
-                if (NH != 1) {
-#pragma unroll
-                    for (int a = 0; a < 10; a++) sJet[q * 30 * NL + 30 * h + 10 * k + a] = Jg[h][a];
-                }
-            }
-        }
-        if (NH == 1) {
-            double u[30];
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-#pragma unroll
-                for (int a = 0; a < 10; a++) u[10 * k + a] = __shfl(Jg[0][a], lq + 1 + k, 64);
-            if (part == 0) {
-                State st;
-                phys_state(u, P, st);
-                double *sp = sSt + q * STW;
-#pragma unroll
-                for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
-#pragma unroll
-                for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
-#pragma unroll
-                for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
-            }
-        }
-        if (part == 0) {
-            double *g = sGeo + q * GEOK;
-            g[0] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * detG;
-#pragma unroll
-            for (int i = 0; i < 9; i++) g[1 + i] = Gi[i];
-#pragma unroll
-            for (int i = 0; i < 18; i++) g[10 + i] = X2[i];
-            if (NH == 1) {
-#pragma unroll
-                for (int i = 0; i < 10; i++) g[28 + i] = W[i];
-            }
-        }
-    }
-    __syncthreads();
+    tangent_fields<NH, GEOK>(M, P, ed, sT, sCP, reinterpret_cast<double *>(sWork), sGeo, sJet, sSt);
 
     typedef typename Mfma<T>::acc_t acc_t;
     acc_t acc[3][2];
@@ -1001,6 +1022,314 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
                         out[(3 * b + bi[t]) * 81 + 3 * c + bj[t]] = v;
                         if (bi[t] != bj[t]) out[(3 * c + bj[t]) * 81 + 3 * b + bi[t]] = v;
                     }
+                }
+    }
+}
+
+// ============================================================================
+// Tangent on a separable patch (k_tangent_sep, static physics): every
+// coordinate of the geometry depends on its own parametric direction only,
+// x_d = f_d(xi_d), and all control weights are equal, as on the igabeam
+// patches (mesh/igabeam.c).  Then G = diag(f_x', f_y', f_z'), the rational
+// basis is the B-spline basis, and the physical jet of basis b at Gauss point
+// q is a product of three 1-D tables of physical derivatives,
+//   R_q[a][b] = Px[qx][dx(a)][bx] Py[qy][dy(a)][by] Pz[qz][dz(a)][bz],
+//   P_d[q][0] = N,  P_d[q][1] = N' / g,  P_d[q][2] = (N'' - (g'/g) N') / g^2
+// (g = f_d', g' = f_d'' at the point; the chain rule of src/assmcore_iga.c:
+// 179-208 with a diagonal G).  The contraction of Tangent_iga
+// (src/assmcore_iga.c:723-749),
+//   K^ij[b][c] = sum_q sum_{a,a'} R_q[a][b] D_q[(i,a),(j,a')] R_q[a'][c],
+// then factors direction by direction (sum factorisation), with s = (dx, dy)
+// the x-y part of a jet slot:
+//   F_q[a][(s',cz)]   = sum_dz' D_q[a][(s',dz')] Pz[qz][dz'][cz]         per point, a lane per row a
+//   E1[(s,bz),(s',cz)] = sum_qz sum_dz Pz[qz][dz][bz] F_q[(s,dz)][(s',cz)]  per (qx, qy) pencil
+//   E2[(dx,by,bz),(dx',cy,cz)] += sum_{dy,dy'} Py[qy][dy][by] Py[qy][dy'][cy] E1[((dx,dy),bz),((dx',dy'),cz)]
+//   K[(bx,by,bz),(cx,cy,cz)]   += sum_{dx,dx'} Px[qx][dx][bx] Px[qx][dx'][cx] E2   after the fourth qy
+// ~0.7 MFLOP per element on the vector ALUs instead of the 27x27 product per
+// point and jet slot.  The displacement jets come from the same tables by sum
+// factorisation (no quotient or chain rule left to apply), the pointwise
+// tangent rows D_q = w_q C_q are those of k_tangent_elem, and the element
+// matrix is written in its layout (row 3b+i, column 3c+j).
+// ============================================================================
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                        T *__restrict__ Ke, int nel)
+{
+    constexpr int NA = 9;                       // jet slots 1..9 (no value slot in the static tangent)
+    constexpr int FR = 18;                      // (s', cz) columns of F and E1
+    constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
+    constexpr int WORK_F = 4 * 6 * NA * FR * (int)sizeof(T);       // F of the pencil's four points
+    constexpr int WORK_E = 6 * 18 * FR * (int)sizeof(T);           // E1, in the same memory after F is read
+    static_assert(WORK_E <= WORK_F, "E1 fits where F was");
+    __shared__ double sPd[3][4][3][3];          // 1-D tables of physical derivatives [d][q_d][order][i]
+    __shared__ T sP[3][4][3][3];                // the same in the contraction's precision
+    __shared__ double sG[3][4];                 // dx_d / dxi_d at the 1-D Gauss points
+    __shared__ double sU[27 * 3];
+    __shared__ double sWgt[64];
+    __shared__ double sSt[64 * STW];
+    __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
+    T *sF = reinterpret_cast<T *>(sWork);
+    T *sE = reinterpret_cast<T *>(sWork);
+    // jet slot a -> order in z, x-y part s; x-y part s -> (dx, dy)
+    constexpr int SDZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
+    constexpr int SXY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
+    constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int e = blockIdx.x;
+    const int nz = M.nelem[2], ny = M.nelem[1];
+    const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
+    const int ed[3] = {ex, ey, ez};
+
+    // ---- element data: displacements, and per direction and 1-D Gauss
+    //      point the B-spline values and derivatives turned into physical
+    //      ones with g = dx_d/dxi_d and g' from the element's three control
+    //      coordinates along d (x_d depends on the index along d alone)
+    if (tid < 27) {
+        const int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
+        const int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ex + bx, M.nb[0], M.periodic[0])
+                         + (int64_t)M.nb[2] * own1(ey + by, M.nb[1], M.periodic[1]) + own1(ez + bz, M.nb[2], M.periodic[2]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) sU[3 * tid + k] = Uh[3 * oc + k];
+    } else if (tid >= 64 && tid < 64 + 12) {
+        const int d = (tid - 64) >> 2, q = (tid - 64) & 3;
+        const double *kV = M.knot[d] + ed[d];
+        const double c = M.gq[q];
+        const double x = (-c + 1.0) * kV[2] + c * kV[3];
+        double t[9];
+        bspline1d(kV, x, t, t + 3, t + 6);
+        double xc[3];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const int ax = ex + (d == 0 ? b : 0), ay = ey + (d == 1 ? b : 0), az = ez + (d == 2 ? b : 0);
+            xc[b] = M.X[3 * (((int64_t)M.na[1] * M.na[2]) * ax + (int64_t)M.na[2] * ay + az) + d];
+        }
+        const double g = t[3] * xc[0] + t[4] * xc[1] + t[5] * xc[2];
+        const double gp = t[6] * xc[0] + t[7] * xc[1] + t[8] * xc[2];
+        const double ig = 1.0 / g, r = gp * ig;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double p1 = t[3 + b] * ig, p2 = (t[6 + b] - r * t[3 + b]) * (ig * ig);
+            sPd[d][q][0][b] = t[b];
+            sPd[d][q][1][b] = p1;
+            sPd[d][q][2][b] = p2;
+            sP[d][q][0][b] = (T)t[b];
+            sP[d][q][1][b] = (T)p1;
+            sP[d][q][2][b] = (T)p2;
+        }
+        sG[d][q] = g;
+    }
+    __syncthreads();
+    // ---- stage A: the physical displacement jets at the 64 points by sum
+    //      factorisation over the physical tables (z, y, then x), the
+    //      strain-measure states and the point weights
+    {
+        double *sA = reinterpret_cast<double *>(sWork), *sB = sA + 3 * 108;
+        // z: sA[k][bx*3+by][qz][dz] = sum_bz U_k(b) Pz[qz][dz][bz]
+        if (tid < 27) {
+            const int k = tid / 9, bxy = tid - 9 * k;
+            const double c0 = sU[3 * (3 * bxy) + k], c1 = sU[3 * (3 * bxy + 1) + k], c2 = sU[3 * (3 * bxy + 2) + k];
+            double *o = sA + 108 * k + 12 * bxy;
+#pragma unroll
+            for (int qz = 0; qz < 4; qz++)
+#pragma unroll
+                for (int dz = 0; dz < 3; dz++) {
+                    const double *tz = &sPd[2][qz][dz][0];
+                    o[3 * qz + dz] = c0 * tz[0] + c1 * tz[1] + c2 * tz[2];
+                }
+        }
+        __syncthreads();
+        // y: sB[k][bx][qy][qz][s6] = sum_by sA[k][bx*3+by][qz][dz(s6)] Py[qy][dy(s6)][by],
+        //    s6 = (dy, dz) in the order (0,0) (1,0) (0,1) (2,0) (1,1) (0,2)
+        if (tid < 36) {
+            const int k = tid / 12, r = tid - 12 * k, bx = r >> 2, qz = r & 3;
+            const double *a = sA + 108 * k + 36 * bx + 3 * qz;   // a[12 by + dz]
+            double av[3][3];
+#pragma unroll
+            for (int by = 0; by < 3; by++)
+#pragma unroll
+                for (int dz = 0; dz < 3; dz++) av[by][dz] = a[12 * by + dz];
+            double *o = sB + 288 * k + 96 * bx + 6 * qz;
+#pragma unroll
+            for (int qy = 0; qy < 4; qy++)
+#pragma unroll
+                for (int s6 = 0; s6 < 6; s6++) {
+                    const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
+                    const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
+                    const double *ty = &sPd[1][qy][dy][0];
+                    o[24 * qy + s6] = av[0][dz] * ty[0] + av[1][dz] * ty[1] + av[2][dz] * ty[2];
+                }
+        }
+        __syncthreads();
+        // x: lane 4q + 1 + k the jet of component k at point q (plus the
+        //    periodic macro-field); lane 4q the state and the weight
+        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
+        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+        double J[10];
+        if (part > 0) {
+            const int k = part - 1;
+            sf_jet(sB + 288 * k + (4 * qd[1] + qd[2]) * 6, &sPd[0][qd[0]][0][0], J);
+            double xq[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double *kV = M.knot[d] + ed[d];
+                const double c = M.gq[qd[d]];
+                xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
+            }
+            add_periodic1(M, xq, k, J);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 10; a++) J[a] = 0.0;
+        }
+        double u[30];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int a = 0; a < 10; a++) u[10 * k + a] = __shfl(J[a], lq + 1 + k, 64);
+        if (part == 0) {
+            State st;
+            phys_state(u, P, st);
+            double *sp = sSt + q * STW;
+#pragma unroll
+            for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+#pragma unroll
+            for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+#pragma unroll
+            for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
+            double dX[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) dX[d] = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
+            sWgt[q] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2]
+                    * (sG[0][qd[0]] * sG[1][qd[1]] * sG[2][qd[2]]);
+        }
+    }
+    __syncthreads();
+
+    // the z and y/x passes run on the first two and three waves of a rotated
+    // numbering, so that the workgroups sharing a CU spread them over its SIMDs
+    const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
+    const int zp = rt / 18, zr = rt - 18 * zp;                   // z pass, rt < 108: block, 3 s' + cz
+    const int yp = rt / 27, yu = rt - 27 * yp;                   // y/x pass, rt < 162: block, bz, cz, by
+    const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
+    T E2[27], K[27];                                             // [dx][dx'][cy], [bx][cx][cy]
+#pragma unroll
+    for (int i = 0; i < 27; i++) { E2[i] = (T)0; K[i] = (T)0; }
+
+    for (int chunk = 0; chunk < 16; chunk++) {
+        const int qx = chunk >> 2, qy = chunk & 3;
+        // ---- rows of D_q and their z contraction F_q: wave wv takes point
+        //      qz = wv of the pencil, lane 9p + r row r + 1 of block p
+        {
+            const int qz = wv, q = 4 * chunk + qz;
+            if (lane < 6 * NA) {
+                const int p = lane / NA, ar = lane - p * NA;
+                double row[NA];
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
+                c_row<1>(sSt + q * STW, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
+                const double wgt = sWgt[q];
+                const T *pz = &sP[2][qz][0][0];
+                T f[FR];
+#pragma unroll
+                for (int i = 0; i < FR; i++) f[i] = (T)0;
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) {
+                    const int a = bb + 1;
+                    const T v = (T)(wgt * row[bb]);
+#pragma unroll
+                    for (int cz = 0; cz < 3; cz++) f[3 * SXY[a] + cz] += v * pz[3 * SDZ[a] + cz];
+                }
+                T *o = sF + ((qz * 6 + p) * NA + ar) * FR;
+#pragma unroll
+                for (int i = 0; i < FR; i++) o[i] = f[i];
+            }
+        }
+        __syncthreads();
+        // ---- z pass over the pencil's four points
+        T e1[18];   // [s][bz]
+        if (rt < 108) {
+#pragma unroll
+            for (int i = 0; i < 18; i++) e1[i] = (T)0;
+#pragma unroll
+            for (int qz = 0; qz < 4; qz++) {
+                const T *Fp = sF + (qz * 6 + zp) * NA * FR + zr;
+                const T *pz = &sP[2][qz][0][0];
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) {
+                    const int a = bb + 1;
+                    const T fv = Fp[FR * bb];
+#pragma unroll
+                    for (int bz = 0; bz < 3; bz++) e1[3 * SXY[a] + bz] += pz[3 * SDZ[a] + bz] * fv;
+                }
+            }
+        }
+        __syncthreads();   // F is read; E1 takes its place
+        if (rt < 108) {
+            T *o = sE + zp * 18 * FR + zr;
+#pragma unroll
+            for (int i = 0; i < 18; i++) o[FR * i] = e1[i];
+        }
+        __syncthreads();
+        // ---- y pass, and the x pass after the pencil row's fourth qy
+        if (rt < 162) {
+            const T *E = sE + yp * 18 * FR;
+            T pyb[3], pyc[9];
+#pragma unroll
+            for (int d = 0; d < 3; d++) pyb[d] = sP[1][qy][d][yby];
+#pragma unroll
+            for (int i = 0; i < 9; i++) pyc[i] = sP[1][qy][i / 3][i % 3];   // [dy'][cy]
+#pragma unroll
+            for (int s1 = 0; s1 < 6; s1++)
+#pragma unroll
+                for (int s2 = 0; s2 < 6; s2++) {
+                    const T t1 = pyb[XYY[s1]] * E[(3 * s1 + ybz) * FR + 3 * s2 + ycz];
+#pragma unroll
+                    for (int cy = 0; cy < 3; cy++) E2[(3 * XYX[s1] + XYX[s2]) * 3 + cy] += t1 * pyc[3 * XYY[s2] + cy];
+                }
+            if (qy == 3) {
+                T px[9], G[27];   // px[dx][b]; G[dx][cx][cy]
+#pragma unroll
+                for (int i = 0; i < 9; i++) px[i] = sP[0][qx][i / 3][i % 3];
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                    for (int cx = 0; cx < 3; cx++)
+#pragma unroll
+                        for (int cy = 0; cy < 3; cy++) {
+                            T acc = px[cx] * E2[(3 * dx + 0) * 3 + cy];
+                            acc += px[3 + cx] * E2[(3 * dx + 1) * 3 + cy];
+                            acc += px[6 + cx] * E2[(3 * dx + 2) * 3 + cy];
+                            G[(3 * dx + cx) * 3 + cy] = acc;
+                        }
+#pragma unroll
+                for (int bx = 0; bx < 3; bx++)
+#pragma unroll
+                    for (int cx = 0; cx < 3; cx++)
+#pragma unroll
+                        for (int cy = 0; cy < 3; cy++)
+                            K[(3 * bx + cx) * 3 + cy] += px[bx] * G[(0 + cx) * 3 + cy] + px[3 + bx] * G[(3 + cx) * 3 + cy]
+                                                       + px[6 + bx] * G[(6 + cx) * 3 + cy];
+#pragma unroll
+                for (int i = 0; i < 27; i++) E2[i] = (T)0;
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- K_e: row 3b+i, column 3c+j, and the mirrored block (3c+j, 3b+i)
+    if (rt < 162) {
+        const int i = kBlkI[yp], j = kBlkJ[yp];
+        T *out = Ke + (int64_t)e * 6561;
+#pragma unroll
+        for (int bx = 0; bx < 3; bx++)
+#pragma unroll
+            for (int cx = 0; cx < 3; cx++)
+#pragma unroll
+                for (int cy = 0; cy < 3; cy++) {
+                    const int b = 9 * bx + 3 * yby + ybz, c = 9 * cx + 3 * cy + ycz;
+                    const T v = K[(3 * bx + cx) * 3 + cy];
+                    out[(3 * b + i) * 81 + 3 * c + j] = v;
+                    if (i != j) out[(3 * c + j) * 81 + 3 * b + i] = v;
                 }
     }
 }
@@ -1422,6 +1751,11 @@ hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh,
 {
     dim3 grid(nel), block(256);
     const Scatter S{};
+    if (P.torder == 0 && M.separable) {
+        if (prec == 0) hipLaunchKernelGGL(k_tangent_sep<double>, grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel);
+        else hipLaunchKernelGGL(k_tangent_sep<float>, grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel);
+        return hipGetLastError();
+    }
     if (prec == 0) {
         if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
         else hipLaunchKernelGGL((k_tangent_elem<double, 3, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);

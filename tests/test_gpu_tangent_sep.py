@@ -1,0 +1,141 @@
+"""The sum-factorised tangent of separable patches (k_tangent_sep).
+
+A patch is separable when every control coordinate x_d depends on the control
+point's index along d alone and all weights are equal (the test of
+igap_gpu_create, restated in ``separable`` below).  There the element
+matrices come from 1-D tables of physical derivatives, and the static FP64
+tangent must match the oracle (the reference's loop nest,
+src/assmcore_iga.c:607-764) entry-wise within 1e-12 relative, and the
+general-geometry kernel (flag IGAP_TANGENT_GENERAL_GEOMETRY) within the same
+bound.  Cases: the igabeam beam (its end elements map quadratically, so the
+g' term of the second derivative is exercised), a graded patch (non-uniform
+knots, control coordinates a nonlinear function of the index, all weights
+2), a fully periodic patch with the par_periodic macro-field, and a truly
+distorted igabeamtest patch, which is not separable and takes the general
+kernel whatever the flag."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import NTHREADS, TOL32, TOL64, rowfloor_err
+
+pytestmark = pytest.mark.gpu
+
+
+def separable(mesh):
+    """x_d a function of the index along d alone, weights all equal (bitwise)."""
+    X = mesh.X.reshape(mesh.na[0], mesh.na[1], mesh.na[2], 3)
+    if not np.all(mesh.w == mesh.w[0]):
+        return False
+    return (np.array_equal(X[..., 0], np.broadcast_to(X[:, :1, :1, 0], X.shape[:3]))
+            and np.array_equal(X[..., 1], np.broadcast_to(X[:1, :, :1, 1], X.shape[:3]))
+            and np.array_equal(X[..., 2], np.broadcast_to(X[:1, :1, :, 2], X.shape[:3])))
+
+
+def grade(mesh):
+    """Non-uniform open knots t_i = (i/n)^1.3, control coordinates
+    L_d (greville + 0.02 sin(1.7 i)) along each direction, weights 2: still
+    separable, with a curved map inside every element."""
+    idx = np.indices(mesh.na).reshape(3, -1)
+    X = mesh.X.reshape(-1, 3)
+    for d in range(3):
+        n, k = mesh.nelem[d], mesh.knots[d]
+        k[2:n + 3] = (np.arange(n + 1) / n) ** 1.3
+        i = np.arange(mesh.na[d])
+        grev = 0.5 * (k[i + 1] + k[i + 2])
+        X[:, d] = mesh.L[d] * (grev + 0.02 * np.sin(1.7 * i))[idx[d]]
+    mesh.w[:] = 2.0
+
+
+def build(igap, oracle, name):
+    par_periodic = None
+    if name == "beam":
+        kw = dict(nelem=(16, 4, 4), L=(4, 1, 1))
+    elif name == "graded":
+        kw = dict(nelem=(6, 4, 5), L=(3, 1, 1.5))
+    elif name == "periodic":
+        kw = dict(nelem=(8, 8, 8), L=(1, 1, 1), periodic=(1, 1, 1))
+        par_periodic = [0.01, 0.0, 0.02, 0.0, -0.01, 0.0, 0.005, 0.0, 0.004]
+    else:   # distorted: nelem >= 4 in every direction, so the perturbation applies
+        kw = dict(nelem=(8, 5, 5), L=(4, 1, 1), perturbed=True)
+    mo = oracle.Mesh(kw["nelem"], L=kw["L"], periodic=kw.get("periodic", (0, 0, 0)), perturbed=kw.get("perturbed", False))
+    mg = igap.MeshIGABeam(**kw)
+    if name == "graded":
+        grade(mo)
+        grade(mg)
+    if par_periodic is not None:
+        bc_o, bc_g = oracle.make_bc(par_periodic=par_periodic), igap.BC(par_periodic=par_periodic)
+    elif any(kw.get("periodic", (0, 0, 0))):
+        bc_o, bc_g = oracle.make_bc(), igap.BCDefault()
+    else:
+        bc_o, bc_g = oracle.clamped_tip_bc(tip=0.02), igap.BCClampedTip(tip=0.02)
+    U = oracle.glibc_rand_state(mo.ndof) * 0.05 if name != "beam" else oracle.smooth_state(mo, amp=0.02)
+    return mo, mg, bc_o, bc_g, U
+
+
+CASES = ["beam", "graded", "periodic", "distorted"]
+
+
+@pytest.fixture(scope="module")
+def built(igap, oracle):
+    return {n: build(igap, oracle, n) for n in CASES}
+
+
+def test_separable_cases(built):
+    for n in CASES:
+        assert separable(built[n][1]) == (n != "distorted"), n
+        assert np.array_equal(built[n][0].X, built[n][1].X) and np.array_equal(built[n][0].w, built[n][1].w)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_tangent_vs_oracle(igap, oracle, built, case):
+    mo, mg, bc_o, bc_g, U = built[case]
+    ph_o, ph_g = oracle.make_phys(0), igap.PhysGradElast()
+    rp_o, col_o, K_o = oracle.tangent(mo, ph_o, bc_o, U, nthreads=NTHREADS)
+    with igap.Assm(mg, ph_g, bc_g) as A:
+        rp, col, K = A.tangent(U, prec="f64")
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o)
+    err = rowfloor_err(K, K_o, rp_o)
+    print(f"[{case}] separable={separable(mg)} max rel err vs oracle {err:.3e}")
+    assert err < TOL64
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_sum_factorised_vs_general_kernel(igap, built, case):
+    _, mg, _, bc_g, U = built[case]
+    from igap4_amd import _lib
+    ph = igap.PhysGradElast()
+    out = {}
+    for name, fl in (("default", 0), ("general", _lib.IGAP_TANGENT_GENERAL_GEOMETRY)):
+        with igap.Assm(mg, ph, bc_g, flags=fl) as A:
+            rp, _, k64 = A.tangent(U, prec="f64")
+            _, _, k32 = A.tangent(U, prec="f32")
+            out[name] = (k64, k32)
+    e64 = rowfloor_err(out["default"][0], out["general"][0], rp)
+    e32 = rowfloor_err(out["default"][1], out["general"][0], rp)
+    print(f"[{case}] default vs general FP64 {e64:.3e}; default FP32 vs general FP64 {e32:.3e}")
+    if case == "distorted":
+        # not separable: both flags take the general kernel
+        assert np.array_equal(out["default"][0], out["general"][0])
+        assert np.array_equal(out["default"][1], out["general"][1])
+    assert e64 < TOL64
+    assert e32 < TOL32
+
+
+def test_full_patch_and_repeatable(igap):
+    """64x16x16 beam: the two kernels agree on every entry; the
+    sum-factorised one is bitwise repeatable."""
+    from igap4_amd import _lib
+    mesh = igap.MeshIGABeam(L=(4, 1, 1), nelem=(64, 16, 16))
+    U = igap.smooth_state(mesh, amp=0.02)
+    ph, bc = igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCClampedTip(tip=0.05)
+    with igap.Assm(mesh, ph, bc) as A:
+        rp, _, k1 = A.tangent(U)
+        _, _, k2 = A.tangent(U)
+        _, _, f1 = A.tangent(U, prec="f32")
+        _, _, f2 = A.tangent(U, prec="f32")
+    with igap.Assm(mesh, ph, bc, flags=_lib.IGAP_TANGENT_GENERAL_GEOMETRY) as A:
+        _, _, kg = A.tangent(U)
+    assert np.array_equal(k1, k2) and np.array_equal(f1, f2)
+    e64, e32 = rowfloor_err(k1, kg, rp), rowfloor_err(f1, kg, rp)
+    print(f"[64x16x16] sum-factorised vs general FP64 {e64:.3e}, FP32 vs general FP64 {e32:.3e}")
+    assert e64 < TOL64 and e32 < TOL32
