@@ -408,15 +408,26 @@ __device__ __forceinline__ void phys_density_transient(const double *J, const Ph
 // ---------------------------------------------------------------- tangent entries
 // The state as stored in LDS: F[0..8], H[9..35], S[36..44], G[45..53], N[54..62]
 constexpr int ST_F = 0, ST_H = 9, ST_S = 36, ST_G = 45, ST_N = 54, STW = 63;
+// Compact layout (48 values): F[9], the symmetric H_k,IK (= u_k,IK), S and N
+// once per index pair (I <= K, in the slot order of sidx), G[9]
+constexpr int STC_H = 9, STC_S = 27, STC_G = 33, STC_N = 42, STWC = 48;
+template <bool CMP>
+__device__ __forceinline__ int st_h(int k, int I, int K) { return CMP ? STC_H + 6 * k + sidx(I, K) - 4 : ST_H + 9 * k + 3 * I + K; }
+template <bool CMP>
+__device__ __forceinline__ int st_s(int I, int J) { return CMP ? STC_S + sidx(I, J) - 4 : ST_S + 3 * I + J; }
+template <bool CMP>
+__device__ __forceinline__ int st_g(int I, int K) { return CMP ? STC_G + 3 * I + K : ST_G + 3 * I + K; }
+template <bool CMP>
+__device__ __forceinline__ int st_n(int I, int J) { return CMP ? STC_N + sidx(I, J) - 4 : ST_N + 3 * I + J; }
 
 // One row of the (k,l) block of C at one state, accumulated as row[b - A0] +=
 // s * C[(k,a),(l,b)] for the slots b = A0..9.  The row slot a (1..9) is the
 // lane's; all column slots are unrolled at compile time.  Value slots (a or
-// b = 0) carry no stored-energy term.
-template <int A0>
+// b = 0) carry no stored-energy term.  CMP: the state in the compact layout.
+template <int A0, bool CMP = false>
 __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, int a, int l, double s, double *row)
 {
-    const double *F = st + ST_F, *H = st + ST_H;
+    const double *F = st + ST_F;
     const double g2 = 2 * P.le * P.le;
     const double Fl0 = F[3 * l], Fl1 = F[3 * l + 1], Fl2 = F[3 * l + 2];
     const double Fl[3] = {Fl0, Fl1, Fl2};
@@ -424,18 +435,17 @@ __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, i
         const int L = a - 1;
         const double Fk0 = F[3 * k], Fk1 = F[3 * k + 1], Fk2 = F[3 * k + 2];
         const double FkL = F[3 * k + L], FlL = F[3 * l + L];
-        const double hk0 = H[9 * k + 3 * L], hk1 = H[9 * k + 3 * L + 1], hk2 = H[9 * k + 3 * L + 2];
+        const double hk0 = st[st_h<CMP>(k, L, 0)], hk1 = st[st_h<CMP>(k, L, 1)], hk2 = st[st_h<CMP>(k, L, 2)];
         const double fdot = Fk0 * Fl0 + Fk1 * Fl1 + Fk2 * Fl2;
-        const double *NL = st + ST_N + 3 * L, *SL = st + ST_S + 3 * L;
         const bool dia = (k == l);
 #pragma unroll
         for (int M = 0; M < 3; M++) {
             const double FkM = (M == 0) ? Fk0 : (M == 1 ? Fk1 : Fk2);
-            double v = NL[M] * FkL * Fl[M];
+            double v = st[st_n<CMP>(L, M)] * FkL * Fl[M];
             v += 0.5 * P.e345 * ((M == L) ? (fdot - FkL * FlL) : FkM * FlL);
-            const double hh = hk0 * H[9 * l + 3 * M] + hk1 * H[9 * l + 3 * M + 1] + hk2 * H[9 * l + 3 * M + 2];
+            const double hh = hk0 * st[st_h<CMP>(l, M, 0)] + hk1 * st[st_h<CMP>(l, M, 1)] + hk2 * st[st_h<CMP>(l, M, 2)];
             v += g2 * ((M == L) ? 2.0 / 3.0 : -1.0 / 3.0) * hh;
-            if (dia) v += SL[M];
+            if (dia) v += st[st_s<CMP>(L, M)];
             row[1 + M - A0] += s * v;
         }
         const double hk[3] = {hk0, hk1, hk2};
@@ -446,8 +456,8 @@ __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, i
             double v = g2 * ((L == I) ? 2.0 / 3.0 : -1.0 / 3.0) * hk[K] * Fl[I];
             if (I != K) v += g2 * ((L == K) ? 2.0 / 3.0 : -1.0 / 3.0) * hk[I] * Fl[K];
             if (dia) {
-                if (L == I) v += st[ST_G + 3 * I + K];
-                if (I != K && L == K) v += st[ST_G + 3 * K + I];
+                if (L == I) v += st[st_g<CMP>(I, K)];
+                if (I != K && L == K) v += st[st_g<CMP>(K, I)];
             }
             row[4 + pp - A0] += s * v;
         }
@@ -459,11 +469,11 @@ __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, i
         // first-gradient columns: C[(k,IK),(l,M)] = C[(l,M),(k,IK)]
 #pragma unroll
         for (int M = 0; M < 3; M++) {
-            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * H[9 * l + 3 * M + K] * FkI;
-            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * H[9 * l + 3 * M + I] * FkK;
+            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * st[st_h<CMP>(l, M, K)] * FkI;
+            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * st[st_h<CMP>(l, M, I)] * FkK;
             if (dia) {
-                if (M == I) v += st[ST_G + 3 * I + K];
-                if (off && M == K) v += st[ST_G + 3 * K + I];
+                if (M == I) v += st[st_g<CMP>(I, K)];
+                if (off && M == K) v += st[st_g<CMP>(K, I)];
             }
             row[1 + M - A0] += s * v;
         }

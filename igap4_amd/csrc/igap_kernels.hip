@@ -1051,6 +1051,121 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 // tangent rows D_q = w_q C_q are those of k_tangent_elem, and the element
 // matrix is written in its layout (row 3b+i, column 3c+j).
 // ============================================================================
+// ============================================================================
+// Tangent on a separable patch (k_tangent_sep, static physics): every
+// coordinate of the geometry depends on its own parametric direction only,
+// x_d = f_d(xi_d), and all control weights are equal, as on the igabeam
+// patches (mesh/igabeam.c).  Then G = diag(f_x', f_y', f_z'), the rational
+// basis is the B-spline basis, and the physical jet of basis b at Gauss point
+// q is a product of three 1-D tables of physical derivatives,
+//   R_q[a][b] = Px[qx][dx(a)][bx] Py[qy][dy(a)][by] Pz[qz][dz(a)][bz],
+//   P_d[q][0] = N,  P_d[q][1] = N' / g,  P_d[q][2] = (N'' - (g'/g) N') / g^2
+// (g = f_d', g' = f_d'' at the point; the chain rule of src/assmcore_iga.c:
+// 179-208 with a diagonal G).  The contraction of Tangent_iga
+// (src/assmcore_iga.c:723-749),
+//   K^ij[b][c] = sum_q sum_{a,a'} R_q[a][b] D_q[(i,a),(j,a')] R_q[a'][c],
+// then factors direction by direction (sum factorisation), with s = (dx, dy)
+// the x-y part of a jet slot:
+//   F_q[a][(s',cz)]   = sum_dz' D_q[a][(s',dz')] Pz[qz][dz'][cz]         per point, a lane per row a
+//   E1[(s,bz),(s',cz)] = sum_qz sum_dz Pz[qz][dz][bz] F_q[(s,dz)][(s',cz)]  per (qx, qy) pencil
+//   E2[(dx,by,bz),(dx',cy,cz)] += sum_{dy,dy'} Py[qy][dy][by] Py[qy][dy'][cy] E1[((dx,dy),bz),((dx',dy'),cz)]
+//   K[(bx,by,bz),(cx,cy,cz)]   += sum_{dx,dx'} Px[qx][dx][bx] Px[qx][dx'][cx] E2   after the fourth qy
+// ~0.7 MFLOP per element on the vector ALUs instead of the 27x27 product per
+// point and jet slot.  The displacement jets come from the same tables by sum
+// factorisation (no quotient or chain rule left to apply), the pointwise
+// tangent rows D_q = w_q C_q are those of k_tangent_elem, and the element
+// matrix is written in its layout (row 3b+i, column 3c+j).
+// ============================================================================
+// Work items of k_tangent_sep.  F[qz][p][r][(s',cz)] (r = jet slot - 1) and
+// E1[p][(s,bz)][(s',cz)] have rows of 18 values; sP[d][q][order][i] are the
+// 1-D tables of physical derivatives.
+// z item (p, g, s', cz): the rows of x-y parts g ((0,0), (1,0), (0,1): two
+// slots each, z orders dz0 and dz0 + 1) and g + 3 ((2,0), (1,1), (0,2): one
+// slot, z order 0) summed over the pencil's points against Pz.
+template <typename T>
+__device__ __forceinline__ void sep_zpass(const T *sF, T *sE, const T (*sP)[4][3][3], int iz)
+{
+    constexpr int NA = 9, FR = 18;
+    const int cz = iz % 3, s2 = (iz / 3) % 6, g = (iz / 18) % 3, p = iz / 54;
+    const int r0 = (g == 0) ? 2 : (g == 1 ? 0 : 1);   // slots 3, 1, 2
+    const int r1 = (g == 0) ? 8 : (g == 1 ? 5 : 7);   // slots 9, 6, 8
+    const int r2 = (g == 0) ? 3 : (g == 1 ? 4 : 6);   // slots 4, 5, 7
+    const int dz0 = (g == 0) ? 1 : 0;
+    T am[3] = {(T)0, (T)0, (T)0}, as[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+    for (int qz = 0; qz < 4; qz++) {
+        const T *Fp = sF + (qz * 6 + p) * NA * FR + 3 * s2 + cz;
+        const T f0 = Fp[FR * r0], f1 = Fp[FR * r1], f2 = Fp[FR * r2];
+        const T *pz = &sP[2][qz][0][0];
+#pragma unroll
+        for (int bz = 0; bz < 3; bz++) {
+            am[bz] += pz[3 * dz0 + bz] * f0 + pz[3 * dz0 + 3 + bz] * f1;
+            as[bz] += pz[bz] * f2;
+        }
+    }
+    T *o = sE + p * 18 * FR + 3 * s2 + cz;
+#pragma unroll
+    for (int bz = 0; bz < 3; bz++) {
+        o[FR * (3 * g + bz)] = am[bz];
+        o[FR * (3 * (g + 3) + bz)] = as[bz];
+    }
+}
+
+// y item (p, bz, cz, by, cy): E2[dx][dx'] += sum over x-y parts (s, s') of
+// Py[qy][dy][by] Py[qy][dy'][cy] E1[(s,bz)][(s',cz)]; after the fourth qy of
+// a qx, K[bx][cx] += sum_{dx,dx'} Px[qx][dx][bx] Px[qx][dx'][cx] E2[dx][dx']
+// and E2 restarts.
+template <typename T>
+__device__ __forceinline__ void sep_ypass(const T *sE, const T (*sP)[4][3][3], int iy, int qy, int qx, bool xstep, T (&E2)[9],
+                                          T (&Kc)[9])
+{
+    constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
+    const int cy = iy % 3, by = (iy / 3) % 3, cz = (iy / 9) % 3, bz = (iy / 27) % 3, p = iy / 81;
+    T c[9];   // [dy][dy']
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dy2 = 0; dy2 < 3; dy2++) c[3 * dy + dy2] = sP[1][qy][dy][by] * sP[1][qy][dy2][cy];
+    const T *E = sE + p * 18 * 18 + 18 * bz + cz;   // + 54 s + 3 s'
+#pragma unroll
+    for (int s1 = 0; s1 < 6; s1++)
+#pragma unroll
+        for (int s2 = 0; s2 < 6; s2++)
+            E2[3 * XYX[s1] + XYX[s2]] += c[3 * XYY[s1] + XYY[s2]] * E[54 * s1 + 3 * s2];
+    if (xstep) {
+        T px[9], G[9];   // px[dx][b]; G[dx][cx]
+#pragma unroll
+        for (int i = 0; i < 9; i++) px[i] = sP[0][qx][i / 3][i % 3];
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+            for (int cx = 0; cx < 3; cx++) G[3 * dx + cx] = px[cx] * E2[3 * dx] + px[3 + cx] * E2[3 * dx + 1] + px[6 + cx] * E2[3 * dx + 2];
+#pragma unroll
+        for (int bx = 0; bx < 3; bx++)
+#pragma unroll
+            for (int cx = 0; cx < 3; cx++) Kc[3 * bx + cx] += px[bx] * G[cx] + px[3 + bx] * G[3 + cx] + px[6 + bx] * G[6 + cx];
+#pragma unroll
+        for (int i = 0; i < 9; i++) E2[i] = (T)0;
+    }
+}
+
+// the y item's K[bx][cx] into the element matrix: row 3b+i, column 3c+j of
+// block p = (i, j), and the mirrored entry of the blocks i < j
+template <typename T>
+__device__ __forceinline__ void sep_write(T *out, int iy, const T (&Kc)[9])
+{
+    const int cy = iy % 3, by = (iy / 3) % 3, cz = (iy / 9) % 3, bz = (iy / 27) % 3, p = iy / 81;
+    const int i = kBlkI[p], j = kBlkJ[p];
+#pragma unroll
+    for (int bx = 0; bx < 3; bx++)
+#pragma unroll
+        for (int cx = 0; cx < 3; cx++) {
+            const int b = 9 * bx + 3 * by + bz, c = 9 * cx + 3 * cy + cz;
+            out[(3 * b + i) * 81 + 3 * c + j] = Kc[3 * bx + cx];
+            if (i != j) out[(3 * c + j) * 81 + 3 * b + i] = Kc[3 * bx + cx];
+        }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                         T *__restrict__ Ke, int nel)
@@ -1059,21 +1174,18 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
     constexpr int FR = 18;                      // (s', cz) columns of F and E1
     constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
     constexpr int WORK_F = 4 * 6 * NA * FR * (int)sizeof(T);       // F of the pencil's four points
-    constexpr int WORK_E = 6 * 18 * FR * (int)sizeof(T);           // E1, in the same memory after F is read
-    static_assert(WORK_E <= WORK_F, "E1 fits where F was");
     __shared__ double sPd[3][4][3][3];          // 1-D tables of physical derivatives [d][q_d][order][i]
     __shared__ T sP[3][4][3][3];                // the same in the contraction's precision
     __shared__ double sG[3][4];                 // dx_d / dxi_d at the 1-D Gauss points
     __shared__ double sU[27 * 3];
     __shared__ double sWgt[64];
-    __shared__ double sSt[64 * STW];
+    __shared__ double sSt[64 * STWC];          // strain-measure states, compact layout
+    __shared__ T sE[6 * 18 * FR];               // E1 of the current pencil
     __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
     T *sF = reinterpret_cast<T *>(sWork);
-    T *sE = reinterpret_cast<T *>(sWork);
-    // jet slot a -> order in z, x-y part s; x-y part s -> (dx, dy)
+    // jet slot a -> order in z, x-y part s
     constexpr int SDZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
     constexpr int SXY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
-    constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int e = blockIdx.x;
@@ -1189,13 +1301,16 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
         if (part == 0) {
             State st;
             phys_state(u, P, st);
-            double *sp = sSt + q * STW;
+            double *sp = sSt + q * STWC;
+            constexpr int PR[6] = {0, 1, 2, 4, 5, 8};   // index pairs I <= J in slot order
 #pragma unroll
-            for (int i = 0; i < 9; i++) sp[ST_F + i] = st.F[i];
+            for (int i = 0; i < 9; i++) { sp[i] = st.F[i]; sp[STC_G + i] = st.G[i]; }
 #pragma unroll
-            for (int i = 0; i < 27; i++) sp[ST_H + i] = st.H[i];
+            for (int k = 0; k < 3; k++)
 #pragma unroll
-            for (int i = 0; i < 9; i++) { sp[ST_S + i] = st.S[i]; sp[ST_G + i] = st.G[i]; sp[ST_N + i] = st.N[i]; }
+                for (int t = 0; t < 6; t++) sp[STC_H + 6 * k + t] = u[10 * k + 4 + t];
+#pragma unroll
+            for (int t = 0; t < 6; t++) { sp[STC_S + t] = st.S[PR[t]]; sp[STC_N + t] = st.N[PR[t]]; }
             double dX[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) dX[d] = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
@@ -1205,73 +1320,25 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
     }
     __syncthreads();
 
-    // the z and y/x passes run on the first two and three waves of a rotated
-    // numbering, so that the workgroups sharing a CU spread them over its SIMDs
+    // The z pass runs on 108 threads (block p, column (s', cz) of E1), the
+    // y/x pass on 162 (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]),
+    // numbered from a rotation of the thread index so that the workgroups
+    // sharing a CU put them on different SIMDs.  The y pass of pencil c runs
+    // beside the pointwise stage of pencil c + 1 (F and E1 have buffers of
+    // their own), so a pencil takes two barriers.
     const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
-    const int zp = rt / 18, zr = rt - 18 * zp;                   // z pass, rt < 108: block, 3 s' + cz
-    const int yp = rt / 27, yu = rt - 27 * yp;                   // y/x pass, rt < 162: block, bz, cz, by
+    const int zp = rt / 18, zr = rt - 18 * zp;
+    const int yp = rt / 27, yu = rt - 27 * yp;
     const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
-    T E2[27], K[27];                                             // [dx][dx'][cy], [bx][cx][cy]
+    constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
+    T E2[27], K[27];   // [dx][dx'][cy], [bx][cx][cy]
 #pragma unroll
     for (int i = 0; i < 27; i++) { E2[i] = (T)0; K[i] = (T)0; }
 
-    for (int chunk = 0; chunk < 16; chunk++) {
-        const int qx = chunk >> 2, qy = chunk & 3;
-        // ---- rows of D_q and their z contraction F_q: wave wv takes point
-        //      qz = wv of the pencil, lane 9p + r row r + 1 of block p
-        {
-            const int qz = wv, q = 4 * chunk + qz;
-            if (lane < 6 * NA) {
-                const int p = lane / NA, ar = lane - p * NA;
-                double row[NA];
-#pragma unroll
-                for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
-                c_row<1>(sSt + q * STW, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
-                const double wgt = sWgt[q];
-                const T *pz = &sP[2][qz][0][0];
-                T f[FR];
-#pragma unroll
-                for (int i = 0; i < FR; i++) f[i] = (T)0;
-#pragma unroll
-                for (int bb = 0; bb < NA; bb++) {
-                    const int a = bb + 1;
-                    const T v = (T)(wgt * row[bb]);
-#pragma unroll
-                    for (int cz = 0; cz < 3; cz++) f[3 * SXY[a] + cz] += v * pz[3 * SDZ[a] + cz];
-                }
-                T *o = sF + ((qz * 6 + p) * NA + ar) * FR;
-#pragma unroll
-                for (int i = 0; i < FR; i++) o[i] = f[i];
-            }
-        }
-        __syncthreads();
-        // ---- z pass over the pencil's four points
-        T e1[18];   // [s][bz]
-        if (rt < 108) {
-#pragma unroll
-            for (int i = 0; i < 18; i++) e1[i] = (T)0;
-#pragma unroll
-            for (int qz = 0; qz < 4; qz++) {
-                const T *Fp = sF + (qz * 6 + zp) * NA * FR + zr;
-                const T *pz = &sP[2][qz][0][0];
-#pragma unroll
-                for (int bb = 0; bb < NA; bb++) {
-                    const int a = bb + 1;
-                    const T fv = Fp[FR * bb];
-#pragma unroll
-                    for (int bz = 0; bz < 3; bz++) e1[3 * SXY[a] + bz] += pz[3 * SDZ[a] + bz] * fv;
-                }
-            }
-        }
-        __syncthreads();   // F is read; E1 takes its place
-        if (rt < 108) {
-            T *o = sE + zp * 18 * FR + zr;
-#pragma unroll
-            for (int i = 0; i < 18; i++) o[FR * i] = e1[i];
-        }
-        __syncthreads();
-        // ---- y pass, and the x pass after the pencil row's fourth qy
-        if (rt < 162) {
+    for (int chunk = 0; chunk <= 16; chunk++) {
+        if (chunk > 0 && rt < 162) {
+            // ---- y pass of the previous pencil, x pass after a qx's fourth qy
+            const int pc = chunk - 1, qx = pc >> 2, qy = pc & 3;
             const T *E = sE + yp * 18 * FR;
             T pyb[3], pyc[9];
 #pragma unroll
@@ -1295,23 +1362,70 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
 #pragma unroll
                     for (int cx = 0; cx < 3; cx++)
 #pragma unroll
-                        for (int cy = 0; cy < 3; cy++) {
-                            T acc = px[cx] * E2[(3 * dx + 0) * 3 + cy];
-                            acc += px[3 + cx] * E2[(3 * dx + 1) * 3 + cy];
-                            acc += px[6 + cx] * E2[(3 * dx + 2) * 3 + cy];
-                            G[(3 * dx + cx) * 3 + cy] = acc;
-                        }
+                        for (int cy = 0; cy < 3; cy++)
+                            G[(3 * dx + cx) * 3 + cy] = px[cx] * E2[(3 * dx) * 3 + cy] + px[3 + cx] * E2[(3 * dx + 1) * 3 + cy]
+                                                      + px[6 + cx] * E2[(3 * dx + 2) * 3 + cy];
 #pragma unroll
                 for (int bx = 0; bx < 3; bx++)
 #pragma unroll
                     for (int cx = 0; cx < 3; cx++)
 #pragma unroll
                         for (int cy = 0; cy < 3; cy++)
-                            K[(3 * bx + cx) * 3 + cy] += px[bx] * G[(0 + cx) * 3 + cy] + px[3 + bx] * G[(3 + cx) * 3 + cy]
+                            K[(3 * bx + cx) * 3 + cy] += px[bx] * G[cx * 3 + cy] + px[3 + bx] * G[(3 + cx) * 3 + cy]
                                                        + px[6 + bx] * G[(6 + cx) * 3 + cy];
 #pragma unroll
                 for (int i = 0; i < 27; i++) E2[i] = (T)0;
             }
+        }
+        if (chunk == 16) break;
+        // ---- rows of D_q and their z contraction F_q: wave wv takes point
+        //      qz = wv of the pencil, lane 9p + r row r + 1 of block p
+        {
+            const int qz = wv, q = 4 * chunk + qz;
+            if (lane < 6 * NA) {
+                const int p = lane / NA, ar = lane - p * NA;
+                double row[NA];
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
+                c_row<1, true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
+                const double wgt = sWgt[q];
+                const T *pz = &sP[2][qz][0][0];
+                T f[FR];
+#pragma unroll
+                for (int i = 0; i < FR; i++) f[i] = (T)0;
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) {
+                    const int a = bb + 1;
+                    const T v = (T)(wgt * row[bb]);
+#pragma unroll
+                    for (int cz = 0; cz < 3; cz++) f[3 * SXY[a] + cz] += v * pz[3 * SDZ[a] + cz];
+                }
+                T *o = sF + ((qz * 6 + p) * NA + ar) * FR;
+#pragma unroll
+                for (int i = 0; i < FR; i++) o[i] = f[i];
+            }
+        }
+        __syncthreads();
+        // ---- z pass over the pencil's four points -> E1[p][(s,bz)][(s',cz)]
+        if (rt < 108) {
+            T e1[18];   // [s][bz]
+#pragma unroll
+            for (int i = 0; i < 18; i++) e1[i] = (T)0;
+#pragma unroll
+            for (int qz = 0; qz < 4; qz++) {
+                const T *Fp = sF + (qz * 6 + zp) * NA * FR + zr;
+                const T *pz = &sP[2][qz][0][0];
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) {
+                    const int a = bb + 1;
+                    const T fv = Fp[FR * bb];
+#pragma unroll
+                    for (int bz = 0; bz < 3; bz++) e1[3 * SXY[a] + bz] += pz[3 * SDZ[a] + bz] * fv;
+                }
+            }
+            T *o = sE + zp * 18 * FR + zr;
+#pragma unroll
+            for (int i = 0; i < 18; i++) o[FR * i] = e1[i];
         }
         __syncthreads();
     }
