@@ -23,7 +23,7 @@ IGAP_F64 = 0
 IGAP_F32 = 1
 IGAP_TANGENT_ELEMENT_MATRICES = 1
 IGAP_TANGENT_DIRECT = 2
-IGAP_TANGENT_GENERAL_GEOMETRY = 4
+IGAP_GENERAL_GEOMETRY = 4
 IGAP_PC_APPLY_FULL = 0
 IGAP_PC_APPLY_SMOOTHER = 1
 

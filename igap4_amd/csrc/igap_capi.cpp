@@ -414,7 +414,7 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
                                igap_gpu_ctx **out)
 {
     if (!mesh || !phys || !bc || !out) return fail(IGAP_ERR_ARG, "null argument");
-    if (flags & ~(IGAP_TANGENT_ELEMENT_MATRICES | IGAP_TANGENT_DIRECT | IGAP_TANGENT_GENERAL_GEOMETRY))
+    if (flags & ~(IGAP_TANGENT_ELEMENT_MATRICES | IGAP_TANGENT_DIRECT | IGAP_GENERAL_GEOMETRY))
         return fail(IGAP_ERR_ARG, "unknown flags 0x%x", flags);
     if ((flags & IGAP_TANGENT_ELEMENT_MATRICES) && (flags & IGAP_TANGENT_DIRECT))
         return fail(IGAP_ERR_ARG, "IGAP_TANGENT_ELEMENT_MATRICES and IGAP_TANGENT_DIRECT exclude each other");
@@ -511,7 +511,7 @@ extern "C" int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc 
     M.w = c->d_w;
     M.X = c->d_X;
     gauss4(M.gq, M.gw);
-    M.separable = (flags & IGAP_TANGENT_GENERAL_GEOMETRY) ? 0 : separable_patch(c);
+    M.separable = (flags & IGAP_GENERAL_GEOMETRY) ? 0 : separable_patch(c);
 
     // CSR pattern: rows 3*cp + i, row length 3 * prod_d stencil_d
     std::vector<int64_t> rowptr(c->ndof + 1);

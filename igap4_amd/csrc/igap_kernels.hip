@@ -1051,119 +1051,130 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 // tangent rows D_q = w_q C_q are those of k_tangent_elem, and the element
 // matrix is written in its layout (row 3b+i, column 3c+j).
 // ============================================================================
-// ============================================================================
-// Tangent on a separable patch (k_tangent_sep, static physics): every
-// coordinate of the geometry depends on its own parametric direction only,
-// x_d = f_d(xi_d), and all control weights are equal, as on the igabeam
-// patches (mesh/igabeam.c).  Then G = diag(f_x', f_y', f_z'), the rational
-// basis is the B-spline basis, and the physical jet of basis b at Gauss point
-// q is a product of three 1-D tables of physical derivatives,
-//   R_q[a][b] = Px[qx][dx(a)][bx] Py[qy][dy(a)][by] Pz[qz][dz(a)][bz],
-//   P_d[q][0] = N,  P_d[q][1] = N' / g,  P_d[q][2] = (N'' - (g'/g) N') / g^2
-// (g = f_d', g' = f_d'' at the point; the chain rule of src/assmcore_iga.c:
-// 179-208 with a diagonal G).  The contraction of Tangent_iga
-// (src/assmcore_iga.c:723-749),
-//   K^ij[b][c] = sum_q sum_{a,a'} R_q[a][b] D_q[(i,a),(j,a')] R_q[a'][c],
-// then factors direction by direction (sum factorisation), with s = (dx, dy)
-// the x-y part of a jet slot:
-//   F_q[a][(s',cz)]   = sum_dz' D_q[a][(s',dz')] Pz[qz][dz'][cz]         per point, a lane per row a
-//   E1[(s,bz),(s',cz)] = sum_qz sum_dz Pz[qz][dz][bz] F_q[(s,dz)][(s',cz)]  per (qx, qy) pencil
-//   E2[(dx,by,bz),(dx',cy,cz)] += sum_{dy,dy'} Py[qy][dy][by] Py[qy][dy'][cy] E1[((dx,dy),bz),((dx',dy'),cz)]
-//   K[(bx,by,bz),(cx,cy,cz)]   += sum_{dx,dx'} Px[qx][dx][bx] Px[qx][dx'][cx] E2   after the fourth qy
-// ~0.7 MFLOP per element on the vector ALUs instead of the 27x27 product per
-// point and jet slot.  The displacement jets come from the same tables by sum
-// factorisation (no quotient or chain rule left to apply), the pointwise
-// tangent rows D_q = w_q C_q are those of k_tangent_elem, and the element
-// matrix is written in its layout (row 3b+i, column 3c+j).
-// ============================================================================
-// Work items of k_tangent_sep.  F[qz][p][r][(s',cz)] (r = jet slot - 1) and
-// E1[p][(s,bz)][(s',cz)] have rows of 18 values; sP[d][q][order][i] are the
-// 1-D tables of physical derivatives.
-// z item (p, g, s', cz): the rows of x-y parts g ((0,0), (1,0), (0,1): two
-// slots each, z orders dz0 and dz0 + 1) and g + 3 ((2,0), (1,1), (0,2): one
-// slot, z order 0) summed over the pencil's points against Pz.
+// Per-element set-up on a separable patch (256 threads, no barrier): the 27
+// control displacements -> sU[27][3]; per direction d and 1-D Gauss point q
+// the B-spline values and derivatives made physical with g = dx_d/dxi_d and
+// g' = d2x_d/dxi_d^2 from the element's three control coordinates along d
+// (x_d depends on the index along d alone):
+//   sPd[d][q][0][i] = N_i,  [1][i] = N_i' / g,  [2][i] = (N_i'' - (g'/g) N_i') / g^2
+// (the same in the precision T -> sP when given), and g -> sG[d][q].
 template <typename T>
-__device__ __forceinline__ void sep_zpass(const T *sF, T *sE, const T (*sP)[4][3][3], int iz)
+__device__ __forceinline__ void sep_element(const MeshK &M, const double *__restrict__ Uh, const int *ed, double *sU,
+                                            double (*sPd)[4][3][3], T (*sP)[4][3][3], double (*sG)[4])
 {
-    constexpr int NA = 9, FR = 18;
-    const int cz = iz % 3, s2 = (iz / 3) % 6, g = (iz / 18) % 3, p = iz / 54;
-    const int r0 = (g == 0) ? 2 : (g == 1 ? 0 : 1);   // slots 3, 1, 2
-    const int r1 = (g == 0) ? 8 : (g == 1 ? 5 : 7);   // slots 9, 6, 8
-    const int r2 = (g == 0) ? 3 : (g == 1 ? 4 : 6);   // slots 4, 5, 7
-    const int dz0 = (g == 0) ? 1 : 0;
-    T am[3] = {(T)0, (T)0, (T)0}, as[3] = {(T)0, (T)0, (T)0};
+    const int tid = threadIdx.x;
+    if (tid < 27) {
+        const int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
+        const int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ed[0] + bx, M.nb[0], M.periodic[0])
+                         + (int64_t)M.nb[2] * own1(ed[1] + by, M.nb[1], M.periodic[1]) + own1(ed[2] + bz, M.nb[2], M.periodic[2]);
 #pragma unroll
-    for (int qz = 0; qz < 4; qz++) {
-        const T *Fp = sF + (qz * 6 + p) * NA * FR + 3 * s2 + cz;
-        const T f0 = Fp[FR * r0], f1 = Fp[FR * r1], f2 = Fp[FR * r2];
-        const T *pz = &sP[2][qz][0][0];
+        for (int k = 0; k < 3; k++) sU[3 * tid + k] = Uh[3 * oc + k];
+    } else if (tid >= 64 && tid < 64 + 12) {
+        const int d = (tid - 64) >> 2, q = (tid - 64) & 3;
+        const double *kV = M.knot[d] + ed[d];
+        const double c = M.gq[q];
+        const double x = (-c + 1.0) * kV[2] + c * kV[3];
+        double t[9];
+        bspline1d(kV, x, t, t + 3, t + 6);
+        double xc[3];
 #pragma unroll
-        for (int bz = 0; bz < 3; bz++) {
-            am[bz] += pz[3 * dz0 + bz] * f0 + pz[3 * dz0 + 3 + bz] * f1;
-            as[bz] += pz[bz] * f2;
+        for (int b = 0; b < 3; b++) {
+            const int ax = ed[0] + (d == 0 ? b : 0), ay = ed[1] + (d == 1 ? b : 0), az = ed[2] + (d == 2 ? b : 0);
+            xc[b] = M.X[3 * (((int64_t)M.na[1] * M.na[2]) * ax + (int64_t)M.na[2] * ay + az) + d];
         }
-    }
-    T *o = sE + p * 18 * FR + 3 * s2 + cz;
+        const double g = t[3] * xc[0] + t[4] * xc[1] + t[5] * xc[2];
+        const double gp = t[6] * xc[0] + t[7] * xc[1] + t[8] * xc[2];
+        const double ig = 1.0 / g, r = gp * ig;
 #pragma unroll
-    for (int bz = 0; bz < 3; bz++) {
-        o[FR * (3 * g + bz)] = am[bz];
-        o[FR * (3 * (g + 3) + bz)] = as[bz];
+        for (int b = 0; b < 3; b++) {
+            const double p1 = t[3 + b] * ig, p2 = (t[6 + b] - r * t[3 + b]) * (ig * ig);
+            sPd[d][q][0][b] = t[b];
+            sPd[d][q][1][b] = p1;
+            sPd[d][q][2][b] = p2;
+            if (sP) {
+                sP[d][q][0][b] = (T)t[b];
+                sP[d][q][1][b] = (T)p1;
+                sP[d][q][2][b] = (T)p2;
+            }
+        }
+        sG[d][q] = g;
     }
 }
 
-// y item (p, bz, cz, by, cy): E2[dx][dx'] += sum over x-y parts (s, s') of
-// Py[qy][dy][by] Py[qy][dy'][cy] E1[(s,bz)][(s',cz)]; after the fourth qy of
-// a qx, K[bx][cx] += sum_{dx,dx'} Px[qx][dx][bx] Px[qx][dx'][cx] E2[dx][dx']
-// and E2 restarts.
-template <typename T>
-__device__ __forceinline__ void sep_ypass(const T *sE, const T (*sP)[4][3][3], int iy, int qy, int qx, bool xstep, T (&E2)[9],
-                                          T (&Kc)[9])
+// Physical displacement jets at the 64 Gauss points of a separable element by
+// sum factorisation over the physical tables (z, y, then x; sWork holds 1188
+// doubles), after sep_element and a barrier.  Thread 4q + 1 + k returns in J
+// the jet of component k at point q (par_periodic's macro-field added),
+// thread 4q zeros.
+__device__ __forceinline__ void sep_fields(const MeshK &M, const int *ed, const double *sU, const double (*sPd)[4][3][3],
+                                           double *sWork, double *J)
 {
-    constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
-    const int cy = iy % 3, by = (iy / 3) % 3, cz = (iy / 9) % 3, bz = (iy / 27) % 3, p = iy / 81;
-    T c[9];   // [dy][dy']
+    const int tid = threadIdx.x;
+    double *sA = sWork, *sB = sA + 3 * 108;
+    // z: sA[k][bx*3+by][qz][dz] = sum_bz U_k(b) Pz[qz][dz][bz]
+    if (tid < 27) {
+        const int k = tid / 9, bxy = tid - 9 * k;
+        const double c0 = sU[3 * (3 * bxy) + k], c1 = sU[3 * (3 * bxy + 1) + k], c2 = sU[3 * (3 * bxy + 2) + k];
+        double *o = sA + 108 * k + 12 * bxy;
 #pragma unroll
-    for (int dy = 0; dy < 3; dy++)
+        for (int qz = 0; qz < 4; qz++)
 #pragma unroll
-        for (int dy2 = 0; dy2 < 3; dy2++) c[3 * dy + dy2] = sP[1][qy][dy][by] * sP[1][qy][dy2][cy];
-    const T *E = sE + p * 18 * 18 + 18 * bz + cz;   // + 54 s + 3 s'
+            for (int dz = 0; dz < 3; dz++) {
+                const double *tz = &sPd[2][qz][dz][0];
+                o[3 * qz + dz] = c0 * tz[0] + c1 * tz[1] + c2 * tz[2];
+            }
+    }
+    __syncthreads();
+    // y: sB[k][bx][qy][qz][s6] = sum_by sA[k][bx*3+by][qz][dz(s6)] Py[qy][dy(s6)][by],
+    //    s6 = (dy, dz) in the order (0,0) (1,0) (0,1) (2,0) (1,1) (0,2)
+    if (tid < 36) {
+        const int k = tid / 12, r = tid - 12 * k, bx = r >> 2, qz = r & 3;
+        const double *a = sA + 108 * k + 36 * bx + 3 * qz;   // a[12 by + dz]
+        double av[3][3];
 #pragma unroll
-    for (int s1 = 0; s1 < 6; s1++)
+        for (int by = 0; by < 3; by++)
 #pragma unroll
-        for (int s2 = 0; s2 < 6; s2++)
-            E2[3 * XYX[s1] + XYX[s2]] += c[3 * XYY[s1] + XYY[s2]] * E[54 * s1 + 3 * s2];
-    if (xstep) {
-        T px[9], G[9];   // px[dx][b]; G[dx][cx]
+            for (int dz = 0; dz < 3; dz++) av[by][dz] = a[12 * by + dz];
+        double *o = sB + 288 * k + 96 * bx + 6 * qz;
 #pragma unroll
-        for (int i = 0; i < 9; i++) px[i] = sP[0][qx][i / 3][i % 3];
+        for (int qy = 0; qy < 4; qy++)
 #pragma unroll
-        for (int dx = 0; dx < 3; dx++)
+            for (int s6 = 0; s6 < 6; s6++) {
+                const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
+                const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
+                const double *ty = &sPd[1][qy][dy][0];
+                o[24 * qy + s6] = av[0][dz] * ty[0] + av[1][dz] * ty[1] + av[2][dz] * ty[2];
+            }
+    }
+    __syncthreads();
+    // x
+    const int q = tid >> 2, part = tid & 3;
+    const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+    if (part > 0) {
+        const int k = part - 1;
+        sf_jet(sB + 288 * k + (4 * qd[1] + qd[2]) * 6, &sPd[0][qd[0]][0][0], J);
+        double xq[3];
 #pragma unroll
-            for (int cx = 0; cx < 3; cx++) G[3 * dx + cx] = px[cx] * E2[3 * dx] + px[3 + cx] * E2[3 * dx + 1] + px[6 + cx] * E2[3 * dx + 2];
+        for (int d = 0; d < 3; d++) {
+            const double *kV = M.knot[d] + ed[d];
+            const double c = M.gq[qd[d]];
+            xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
+        }
+        add_periodic1(M, xq, k, J);
+    } else {
 #pragma unroll
-        for (int bx = 0; bx < 3; bx++)
-#pragma unroll
-            for (int cx = 0; cx < 3; cx++) Kc[3 * bx + cx] += px[bx] * G[cx] + px[3 + bx] * G[3 + cx] + px[6 + bx] * G[6 + cx];
-#pragma unroll
-        for (int i = 0; i < 9; i++) E2[i] = (T)0;
+        for (int a = 0; a < 10; a++) J[a] = 0.0;
     }
 }
 
-// the y item's K[bx][cx] into the element matrix: row 3b+i, column 3c+j of
-// block p = (i, j), and the mirrored entry of the blocks i < j
-template <typename T>
-__device__ __forceinline__ void sep_write(T *out, int iy, const T (&Kc)[9])
+// Gauss weight x span lengths x det G at point q of a separable element
+__device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int q, const double (*sG)[4])
 {
-    const int cy = iy % 3, by = (iy / 3) % 3, cz = (iy / 9) % 3, bz = (iy / 27) % 3, p = iy / 81;
-    const int i = kBlkI[p], j = kBlkJ[p];
+    const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
+    double dX[3];
 #pragma unroll
-    for (int bx = 0; bx < 3; bx++)
-#pragma unroll
-        for (int cx = 0; cx < 3; cx++) {
-            const int b = 9 * bx + 3 * by + bz, c = 9 * cx + 3 * cy + cz;
-            out[(3 * b + i) * 81 + 3 * c + j] = Kc[3 * bx + cx];
-            if (i != j) out[(3 * c + j) * 81 + 3 * b + i] = Kc[3 * bx + cx];
-        }
+    for (int d = 0; d < 3; d++) dX[d] = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
+    return M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * (sG[0][qd[0]] * sG[1][qd[1]] * sG[2][qd[2]]);
 }
 
 template <typename T>
@@ -1193,106 +1204,13 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
     const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
     const int ed[3] = {ex, ey, ez};
 
-    // ---- element data: displacements, and per direction and 1-D Gauss
-    //      point the B-spline values and derivatives turned into physical
-    //      ones with g = dx_d/dxi_d and g' from the element's three control
-    //      coordinates along d (x_d depends on the index along d alone)
-    if (tid < 27) {
-        const int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
-        const int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ex + bx, M.nb[0], M.periodic[0])
-                         + (int64_t)M.nb[2] * own1(ey + by, M.nb[1], M.periodic[1]) + own1(ez + bz, M.nb[2], M.periodic[2]);
-#pragma unroll
-        for (int k = 0; k < 3; k++) sU[3 * tid + k] = Uh[3 * oc + k];
-    } else if (tid >= 64 && tid < 64 + 12) {
-        const int d = (tid - 64) >> 2, q = (tid - 64) & 3;
-        const double *kV = M.knot[d] + ed[d];
-        const double c = M.gq[q];
-        const double x = (-c + 1.0) * kV[2] + c * kV[3];
-        double t[9];
-        bspline1d(kV, x, t, t + 3, t + 6);
-        double xc[3];
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            const int ax = ex + (d == 0 ? b : 0), ay = ey + (d == 1 ? b : 0), az = ez + (d == 2 ? b : 0);
-            xc[b] = M.X[3 * (((int64_t)M.na[1] * M.na[2]) * ax + (int64_t)M.na[2] * ay + az) + d];
-        }
-        const double g = t[3] * xc[0] + t[4] * xc[1] + t[5] * xc[2];
-        const double gp = t[6] * xc[0] + t[7] * xc[1] + t[8] * xc[2];
-        const double ig = 1.0 / g, r = gp * ig;
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            const double p1 = t[3 + b] * ig, p2 = (t[6 + b] - r * t[3 + b]) * (ig * ig);
-            sPd[d][q][0][b] = t[b];
-            sPd[d][q][1][b] = p1;
-            sPd[d][q][2][b] = p2;
-            sP[d][q][0][b] = (T)t[b];
-            sP[d][q][1][b] = (T)p1;
-            sP[d][q][2][b] = (T)p2;
-        }
-        sG[d][q] = g;
-    }
+    sep_element<T>(M, Uh, ed, sU, sPd, sP, sG);
     __syncthreads();
-    // ---- stage A: the physical displacement jets at the 64 points by sum
-    //      factorisation over the physical tables (z, y, then x), the
-    //      strain-measure states and the point weights
+    // ---- stage A: displacement jets, strain-measure states, point weights
     {
-        double *sA = reinterpret_cast<double *>(sWork), *sB = sA + 3 * 108;
-        // z: sA[k][bx*3+by][qz][dz] = sum_bz U_k(b) Pz[qz][dz][bz]
-        if (tid < 27) {
-            const int k = tid / 9, bxy = tid - 9 * k;
-            const double c0 = sU[3 * (3 * bxy) + k], c1 = sU[3 * (3 * bxy + 1) + k], c2 = sU[3 * (3 * bxy + 2) + k];
-            double *o = sA + 108 * k + 12 * bxy;
-#pragma unroll
-            for (int qz = 0; qz < 4; qz++)
-#pragma unroll
-                for (int dz = 0; dz < 3; dz++) {
-                    const double *tz = &sPd[2][qz][dz][0];
-                    o[3 * qz + dz] = c0 * tz[0] + c1 * tz[1] + c2 * tz[2];
-                }
-        }
-        __syncthreads();
-        // y: sB[k][bx][qy][qz][s6] = sum_by sA[k][bx*3+by][qz][dz(s6)] Py[qy][dy(s6)][by],
-        //    s6 = (dy, dz) in the order (0,0) (1,0) (0,1) (2,0) (1,1) (0,2)
-        if (tid < 36) {
-            const int k = tid / 12, r = tid - 12 * k, bx = r >> 2, qz = r & 3;
-            const double *a = sA + 108 * k + 36 * bx + 3 * qz;   // a[12 by + dz]
-            double av[3][3];
-#pragma unroll
-            for (int by = 0; by < 3; by++)
-#pragma unroll
-                for (int dz = 0; dz < 3; dz++) av[by][dz] = a[12 * by + dz];
-            double *o = sB + 288 * k + 96 * bx + 6 * qz;
-#pragma unroll
-            for (int qy = 0; qy < 4; qy++)
-#pragma unroll
-                for (int s6 = 0; s6 < 6; s6++) {
-                    const int dy = (s6 == 1 || s6 == 4) ? 1 : (s6 == 3 ? 2 : 0);
-                    const int dz = (s6 == 2 || s6 == 4) ? 1 : (s6 == 5 ? 2 : 0);
-                    const double *ty = &sPd[1][qy][dy][0];
-                    o[24 * qy + s6] = av[0][dz] * ty[0] + av[1][dz] * ty[1] + av[2][dz] * ty[2];
-                }
-        }
-        __syncthreads();
-        // x: lane 4q + 1 + k the jet of component k at point q (plus the
-        //    periodic macro-field); lane 4q the state and the weight
-        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
-        const int qd[3] = {q >> 4, (q >> 2) & 3, q & 3};
         double J[10];
-        if (part > 0) {
-            const int k = part - 1;
-            sf_jet(sB + 288 * k + (4 * qd[1] + qd[2]) * 6, &sPd[0][qd[0]][0][0], J);
-            double xq[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const double *kV = M.knot[d] + ed[d];
-                const double c = M.gq[qd[d]];
-                xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
-            }
-            add_periodic1(M, xq, k, J);
-        } else {
-#pragma unroll
-            for (int a = 0; a < 10; a++) J[a] = 0.0;
-        }
+        sep_fields(M, ed, sU, sPd, reinterpret_cast<double *>(sWork), J);
+        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
         double u[30];
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -1311,11 +1229,7 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
                 for (int t = 0; t < 6; t++) sp[STC_H + 6 * k + t] = u[10 * k + 4 + t];
 #pragma unroll
             for (int t = 0; t < 6; t++) { sp[STC_S + t] = st.S[PR[t]]; sp[STC_N + t] = st.N[PR[t]]; }
-            double dX[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) dX[d] = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
-            sWgt[q] = M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2]
-                    * (sG[0][qd[0]] * sG[1][qd[1]] * sG[2][qd[2]]);
+            sWgt[q] = sep_weight(M, ed, q, sG);
         }
     }
     __syncthreads();
@@ -1445,6 +1359,102 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
                     out[(3 * b + i) * 81 + 3 * c + j] = v;
                     if (i != j) out[(3 * c + j) * 81 + 3 * b + i] = v;
                 }
+    }
+}
+
+// ============================================================================
+// Residual on a separable patch (k_residual_sep, static physics; volume loop
+// of Residual_iga, src/assmcore_iga.c:118-259): the displacement jets of
+// sep_fields, the pointwise residual jets r_q (phys/gradelast.c) times the
+// point weight, and the projection
+//   R_e[b][k] = sum_q sum_a r_q[k][a] R_q[a][b]
+// with R_q the tensor product of the physical tables, as the transposed sum
+// factorisation x -> y -> z; no quotient or chain rule to undo.
+// ============================================================================
+__global__ __launch_bounds__(256, 4) void k_residual_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                         double *__restrict__ Re, int nel)
+{
+    __shared__ double sPd[3][4][3][3];
+    __shared__ double sG[3][4];
+    __shared__ double sU[27 * 3];
+    __shared__ double sWork[3 * (108 + 288)];   // field passes, then x^T (864) and y^T (324)
+    __shared__ double sJet[64 * 30];            // u-jets [q][k][a], then the weighted r_q
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x;
+    const int nz = M.nelem[2], ny = M.nelem[1];
+    const int ed[3] = {e / (ny * nz), (e / nz) % ny, e % nz};
+
+    sep_element<double>(M, Uh, ed, sU, sPd, nullptr, sG);
+    __syncthreads();
+    {
+        double J[10];
+        sep_fields(M, ed, sU, sPd, sWork, J);
+        const int q = tid >> 2, part = tid & 3;
+        if (part > 0) {
+#pragma unroll
+            for (int a = 0; a < 10; a++) sJet[q * 30 + 10 * (part - 1) + a] = J[a];
+        }
+    }
+    __syncthreads();
+    // ---- pointwise residual jets, weighted: one wave, lane = Gauss point
+    //      (the wave taking it rotates with the element)
+    if ((tid >> 6) == (e & 3)) {
+        const int q = tid & 63;
+        double u[30], r[30];
+#pragma unroll
+        for (int a = 0; a < 30; a++) u[a] = sJet[q * 30 + a];
+        phys_residual_static(P, u, r);
+        const double w = sep_weight(M, ed, q, sG);
+#pragma unroll
+        for (int a = 0; a < 30; a++) sJet[q * 30 + a] = w * r[a];
+    }
+    __syncthreads();
+    // ---- x^T: C1[k][bx][qy][qz][s6] = sum_qx sum_{a: S6(a) = s6} r[q][k][a] Px[qx][DX(a)][bx]
+    double *C1 = sWork, *C2 = sWork + 864;
+    for (int i = tid; i < 864; i += 256) {   // s6 slowest: the switch below is mostly wave-uniform
+        const int s6 = i / 144, t = i - 144 * s6, qz = t & 3, qy = (t >> 2) & 3, bx = (t >> 4) % 3, k = t / 48;
+        double acc = 0.0;
+#pragma unroll
+        for (int qx = 0; qx < 4; qx++) {
+            const double *rq = sJet + (16 * qx + 4 * qy + qz) * 30 + 10 * k;
+            const double *tx = &sPd[0][qx][0][0];   // tx[3 * dx + bx]
+            switch (s6) {
+                case 0: acc += rq[0] * tx[bx] + rq[1] * tx[3 + bx] + rq[4] * tx[6 + bx]; break;
+                case 1: acc += rq[2] * tx[bx] + rq[5] * tx[3 + bx]; break;
+                case 2: acc += rq[3] * tx[bx] + rq[6] * tx[3 + bx]; break;
+                case 3: acc += rq[7] * tx[bx]; break;
+                case 4: acc += rq[8] * tx[bx]; break;
+                default: acc += rq[9] * tx[bx]; break;
+            }
+        }
+        C1[6 * t + s6] = acc;   // [k][bx][qy][qz][s6]
+    }
+    __syncthreads();
+    // ---- y^T: C2[k][bx][by][qz][dz] = sum_qy sum_{s6: dz(s6) = dz} C1[..][s6] Py[qy][dy(s6)][by]
+    for (int i = tid; i < 324; i += 256) {   // dz slowest (mostly wave-uniform branches)
+        const int dz = i / 108, t = i - 108 * dz, qz = t & 3, u = t >> 2, by = u % 3, bx = (u / 3) % 3, k = u / 9;
+        double acc = 0.0;
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const double *c = C1 + (((k * 3 + bx) * 4 + qy) * 4 + qz) * 6;
+            const double *ty = &sPd[1][qy][0][0];   // ty[3 * dy + by]
+            if (dz == 0) acc += c[0] * ty[by] + c[1] * ty[3 + by] + c[3] * ty[6 + by];
+            else if (dz == 1) acc += c[2] * ty[by] + c[4] * ty[3 + by];
+            else acc += c[5] * ty[by];
+        }
+        C2[3 * t + dz] = acc;   // [k][bx][by][qz][dz]
+    }
+    __syncthreads();
+    // ---- z^T: R_e[3 b + k] = sum_qz sum_dz C2[k][bx][by][qz][dz] Pz[qz][dz][bz]
+    if (tid < 81) {
+        const int k = tid % 3, b = tid / 3, bx = b / 9, by = (b / 3) % 3, bz = b % 3;
+        const double *c = C2 + ((k * 3 + bx) * 3 + by) * 12;
+        double acc = 0.0;
+#pragma unroll
+        for (int qz = 0; qz < 4; qz++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) acc += c[3 * qz + dz] * sPd[2][qz][dz][bz];
+        Re[(int64_t)e * 81 + tid] = acc;
     }
 }
 
@@ -1847,7 +1857,8 @@ hipError_t launch_density(const MeshK &M, const PhysK &P, const double *Uh, int6
 hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh, int64_t ndof, double *Re, int nel, hipStream_t s)
 {
     dim3 grid(nel), block(256);
-    if (P.torder == 0) hipLaunchKernelGGL(k_residual_elem<1>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
+    if (P.torder == 0 && M.separable) hipLaunchKernelGGL(k_residual_sep, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
+    else if (P.torder == 0) hipLaunchKernelGGL(k_residual_elem<1>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     else hipLaunchKernelGGL(k_residual_elem<3>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     return hipGetLastError();
 }

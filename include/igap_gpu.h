@@ -106,20 +106,21 @@ int igap_mesh_igabeam(const double L[3], const int nelem[3], const int periodic[
  *   IGAP_TANGENT_DIRECT            the element kernel adds its matrix into the
  *       CSR values, one launch per class of the 27-colouring (the elements of
  *       one class share no control point); non-periodic patches only
- * Element kernel (static physics, element matrices): on a separable patch --
- * every control coordinate x_d depends on the control point's index along d
- * alone and all weights are equal, as on the igabeam patches -- the element
- * matrices come from a sum-factorised kernel (1-D tables of physical
- * derivatives); otherwise, and with the flag below, from the
- * general-geometry kernel.  Both give the same values up to rounding.
- *   IGAP_TANGENT_GENERAL_GEOMETRY  the general-geometry kernel on every patch */
-enum { IGAP_TANGENT_ELEMENT_MATRICES = 1, IGAP_TANGENT_DIRECT = 2, IGAP_TANGENT_GENERAL_GEOMETRY = 4 };
+ * Element kernels of the static physics: on a separable patch -- every
+ * control coordinate x_d depends on the control point's index along d alone
+ * and all weights are equal, as on the igabeam patches -- the residual and
+ * the element matrices of the tangent (element-matrix path) come from
+ * sum-factorised kernels over 1-D tables of physical derivatives; otherwise,
+ * and with the flag below, from the general-geometry kernels.  Both give the
+ * same values up to rounding.
+ *   IGAP_GENERAL_GEOMETRY          the general-geometry kernels on every patch */
+enum { IGAP_TANGENT_ELEMENT_MATRICES = 1, IGAP_TANGENT_DIRECT = 2, IGAP_GENERAL_GEOMETRY = 4 };
 
 /* IGAPAssmInit (src/interface.c:131-203) for one GPU: uploads the patch,
  * builds the CSR pattern, the Dirichlet row set and the Neumann load, and
  * allocates U_hist, R and the matrix values.  device < 0 keeps the current
  * device.  flags: 0, or one of IGAP_TANGENT_ELEMENT_MATRICES and
- * IGAP_TANGENT_DIRECT, optionally with IGAP_TANGENT_GENERAL_GEOMETRY. */
+ * IGAP_TANGENT_DIRECT, optionally with IGAP_GENERAL_GEOMETRY. */
 int igap_gpu_create(const igap_mesh_desc *mesh, const igap_phys_desc *phys, const igap_bc_desc *bc,
                     int device, int flags, igap_gpu_ctx **ctx);
 /* IGAPFinalize (src/interface.c:372-435) for the context: frees everything

@@ -1,12 +1,13 @@
-"""The sum-factorised tangent of separable patches (k_tangent_sep).
+"""The sum-factorised residual and tangent of separable patches
+(k_residual_sep, k_tangent_sep).
 
 A patch is separable when every control coordinate x_d depends on the control
 point's index along d alone and all weights are equal (the test of
-igap_gpu_create, restated in ``separable`` below).  There the element
-matrices come from 1-D tables of physical derivatives, and the static FP64
-tangent must match the oracle (the reference's loop nest,
-src/assmcore_iga.c:607-764) entry-wise within 1e-12 relative, and the
-general-geometry kernel (flag IGAP_TANGENT_GENERAL_GEOMETRY) within the same
+igap_gpu_create, restated in ``separable`` below).  There the element vectors
+and matrices come from 1-D tables of physical derivatives, and the static
+FP64 residual and tangent must match the oracle (the reference's loop nests,
+src/assmcore_iga.c:118-259 and 607-764) entry-wise within 1e-12 relative,
+and the general-geometry kernels (flag IGAP_GENERAL_GEOMETRY) within the same
 bound.  Cases: the igabeam beam (its end elements map quadratically, so the
 g' term of the second derivative is exercised), a graded patch (non-uniform
 knots, control coordinates a nonlinear function of the index, all weights
@@ -100,12 +101,31 @@ def test_tangent_vs_oracle(igap, oracle, built, case):
 
 
 @pytest.mark.parametrize("case", CASES)
+def test_residual_vs_oracle_and_general(igap, oracle, built, case):
+    from igap4_amd import _lib
+    mo, mg, bc_o, bc_g, U = built[case]
+    ph_o, ph_g = oracle.make_phys(0), igap.PhysGradElast()
+    R_o = oracle.residual(mo, ph_o, bc_o, U, nthreads=NTHREADS)
+    with igap.Assm(mg, ph_g, bc_g) as A:
+        R = A.residual(U)
+        R2 = A.residual(U)
+    with igap.Assm(mg, ph_g, bc_g, flags=_lib.IGAP_GENERAL_GEOMETRY) as A:
+        Rg = A.residual(U)
+    e_o, e_g = rowfloor_err(R, R_o), rowfloor_err(R, Rg)
+    print(f"[{case}] residual: max rel err vs oracle {e_o:.3e}, vs general kernel {e_g:.3e}")
+    assert np.array_equal(R, R2)
+    if case == "distorted":
+        assert np.array_equal(R, Rg)
+    assert e_o < TOL64 and e_g < TOL64
+
+
+@pytest.mark.parametrize("case", CASES)
 def test_sum_factorised_vs_general_kernel(igap, built, case):
     _, mg, _, bc_g, U = built[case]
     from igap4_amd import _lib
     ph = igap.PhysGradElast()
     out = {}
-    for name, fl in (("default", 0), ("general", _lib.IGAP_TANGENT_GENERAL_GEOMETRY)):
+    for name, fl in (("default", 0), ("general", _lib.IGAP_GENERAL_GEOMETRY)):
         with igap.Assm(mg, ph, bc_g, flags=fl) as A:
             rp, _, k64 = A.tangent(U, prec="f64")
             _, _, k32 = A.tangent(U, prec="f32")
@@ -133,7 +153,7 @@ def test_full_patch_and_repeatable(igap):
         _, _, k2 = A.tangent(U)
         _, _, f1 = A.tangent(U, prec="f32")
         _, _, f2 = A.tangent(U, prec="f32")
-    with igap.Assm(mesh, ph, bc, flags=_lib.IGAP_TANGENT_GENERAL_GEOMETRY) as A:
+    with igap.Assm(mesh, ph, bc, flags=_lib.IGAP_GENERAL_GEOMETRY) as A:
         _, _, kg = A.tangent(U)
     assert np.array_equal(k1, k2) and np.array_equal(f1, f2)
     e64, e32 = rowfloor_err(k1, kg, rp), rowfloor_err(f1, kg, rp)
