@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--ramp-tip", type=float, default=0.1, help="tip displacement at the end of the ramp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample")
+    ap.add_argument("--general-geometry", action="store_true",
+                    help="--probe: run the general-geometry kernels (context flag IGAP_GENERAL_GEOMETRY) instead "
+                         "of the sum-factorised ones the separable beam takes")
     ap.add_argument("--probe", type=int, default=0,
                     help="profiling run for rocprofv3 --pmc: one 256 MiB device copy (the HBM "
                          "counter calibration) then N x (residual + FP64 and FP32 tangent); prints no JSON")
@@ -79,15 +82,16 @@ def say(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def probe(nel, calls):
+def probe(nel, calls, general=False):
     """Counter-collection run: one 256 MiB device-to-device copy (the HBM
     counter calibration), then ``calls`` x (residual + FP64 tangent + FP32
-    tangent) on the workload."""
+    tangent) on the workload (general-geometry kernels with ``general``)."""
     import ctypes as C
     import igap4_amd as ig
     from igap4_amd import _lib
     mesh = ig.MeshIGABeam(L=(4.0, 1.0, 1.0), nelem=nel)
-    with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.05)) as A:
+    flags = _lib.IGAP_GENERAL_GEOMETRY if general else 0
+    with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.05), flags=flags) as A:
         L = _lib.lib()
         src, dst = C.c_void_p(), C.c_void_p()
         _lib.check(L.igap_gpu_malloc(A.ctx, C.byref(src), COPY_BYTES))
@@ -306,7 +310,7 @@ def main():
         pmc_json(nel, *a.pmc_json)
         return
     if a.probe:
-        probe(nel, a.probe)
+        probe(nel, a.probe, a.general_geometry)
         return
     import igap4_amd as ig
 

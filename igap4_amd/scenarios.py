@@ -68,9 +68,13 @@ def transient(nelem=(128, 32, 32), nsteps=20, prec="f32", pc="jacobi", pc_order=
 
 
 def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_order=0, rtol=1e-10, guess="previous",
-         max_steps=None, device=0, verbose=True, atol=1e-30, pc_lag=1, snes_maxit=50, mg_levels=0, pc_fp32=0):
+         max_steps=None, device=0, verbose=True, atol=1e-30, pc_lag=1, snes_maxit=50, mg_levels=0, pc_fp32=0,
+         start_rtol=None):
     """Clamped beam with the tip displacement ramped over ``nsteps`` load
-    steps; each step solved to ||R|| <= max(rtol ||R0||, atol)."""
+    steps; each step solved to ||R|| <= max(rtol ||R0||, atol) with R0 the
+    residual of the initial iterate, or, with ``start_rtol``, to
+    ||R|| <= start_rtol ||R(U_prev; load_k)|| (the previous solution under the
+    step's load, whatever the initial iterate)."""
     mesh = am.MeshIGABeam(L=(4, 1, 1), nelem=nelem)
     A = am.Assm(mesh, am.PhysGradElast(coef=am.SVK_COEF), am.BCClampedTip(tip=0.0), device=device)
     A.set_uhist(np.zeros(A.ndof))
@@ -83,11 +87,12 @@ def ramp(nelem=(64, 16, 16), nsteps=20, tip=0.05, prec="f64", pc="ilu0", pc_orde
             print(f"  step {k}: newton {st['newton_its']} ksp {st['ksp_its']} {st['wall_ms']:.1f} ms", flush=True)
 
     t0 = time.perf_counter()
-    steps = run_load_steps(A, nsteps, tip_total=tip, guess=guess, opts=opts, callback=cb, max_steps=max_steps)
+    steps = run_load_steps(A, nsteps, tip_total=tip, guess=guess, opts=opts, callback=cb, max_steps=max_steps,
+                           start_rtol=start_rtol)
     tt = time.perf_counter() - t0
     out = {"scenario": "ramp", "nelem": list(nelem), "ndof": A.ndof, "nnz": A.nnz, "tip": tip, "load_steps": nsteps,
            "tangent": prec, "pc": pc, "pc_order": pc_order, "pc_lag": pc_lag, "pc_fp32": pc_fp32, "rtol": rtol,
-           "atol": atol, "guess": guess,
+           "atol": atol, "guess": guess, "start_rtol": start_rtol,
            "total_s": tt,
            **_agg(steps)}
     A.close()
@@ -112,6 +117,8 @@ def main():
     ap.add_argument("--mg-levels", type=int, default=0)
     ap.add_argument("--pc-fp32", type=int, default=0)
     ap.add_argument("--guess", default="previous")
+    ap.add_argument("--start-rtol", type=float, default=None,
+                    help="ramp: stop at this fraction of the previous solution's residual under the step's load")
     a = ap.parse_args()
     if a.which == "transient":
         nel = tuple(int(x) for x in (a.nelem or "128,32,32").split(","))
@@ -120,7 +127,8 @@ def main():
     else:
         nel = tuple(int(x) for x in (a.nelem or "64,16,16").split(","))
         r = ramp(nel, a.steps, a.tip, a.prec or "f64", a.pc or "ilu0", a.pc_order, a.rtol, a.guess, a.max_steps,
-                 atol=a.atol, pc_lag=a.pc_lag, snes_maxit=a.snes_maxit, mg_levels=a.mg_levels, pc_fp32=a.pc_fp32)
+                 atol=a.atol, pc_lag=a.pc_lag, snes_maxit=a.snes_maxit, mg_levels=a.mg_levels, pc_fp32=a.pc_fp32,
+                 start_rtol=a.start_rtol)
     print(json.dumps(r), flush=True)
 
 
