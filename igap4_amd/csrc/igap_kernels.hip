@@ -1053,10 +1053,10 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 // ============================================================================
 // Per-element set-up on a separable patch (256 threads, no barrier): the 27
 // control displacements -> sU[27][3]; per direction d and 1-D Gauss point q
-// the B-spline values and derivatives made physical with g = dx_d/dxi_d and
-// g' = d2x_d/dxi_d^2 from the element's three control coordinates along d
-// (x_d depends on the index along d alone):
-//   sPd[d][q][0][i] = N_i,  [1][i] = N_i' / g,  [2][i] = (N_i'' - (g'/g) N_i') / g^2
+// the 1-D rational basis rho_i = N_i / S (S = sum_i N_i) and its derivatives
+// made physical with g = dx_d/dxi_d and g' = d2x_d/dxi_d^2 from the element's
+// three control coordinates along d (x_d depends on the index along d alone):
+//   sPd[d][q][0][i] = rho_i,  [1][i] = rho_i' / g,  [2][i] = (rho_i'' - (g'/g) rho_i') / g^2
 // (the same in the precision T -> sP when given), and g -> sG[d][q].
 template <typename T>
 __device__ __forceinline__ void sep_element(const MeshK &M, const double *__restrict__ Uh, const int *ed, double *sU,
@@ -1076,6 +1076,18 @@ __device__ __forceinline__ void sep_element(const MeshK &M, const double *__rest
         const double x = (-c + 1.0) * kV[2] + c * kV[3];
         double t[9];
         bspline1d(kV, x, t, t + 3, t + 6);
+        // the rational basis of equal weights is N_b / W with W = S_x S_y S_z,
+        // S = sum_i N_i (1 up to the MTOL guard of the B-spline formulas):
+        // a product of the 1-D quotients rho_i = N_i / S, whose derivatives
+        // follow from the 1-D quotient rule
+        const double S0 = t[0] + t[1] + t[2], S1 = t[3] + t[4] + t[5], S2 = t[6] + t[7] + t[8];
+        const double iS = 1.0 / S0;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            t[b] *= iS;
+            t[3 + b] = (t[3 + b] - t[b] * S1) * iS;
+            t[6 + b] = (t[6 + b] - 2.0 * t[3 + b] * S1 - t[b] * S2) * iS;
+        }
         double xc[3];
 #pragma unroll
         for (int b = 0; b < 3; b++) {

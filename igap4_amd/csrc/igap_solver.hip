@@ -1268,7 +1268,8 @@ constexpr int64_t MG_COARSE_NDOF = 1500;
 constexpr int64_t MG_DENSE_MAX = 8000;
 constexpr int MG_CHEB_DEG = 3;
 constexpr int MG_COARSE_DEG = 12;
-constexpr int MG_POWER_ITS = 20;
+constexpr int MG_POWER_ITS = 20;        // power iterations of the first set-up (from a hashed vector)
+constexpr int MG_POWER_ITS_WARM = 8;    // of later ones, from the previous set-up's vector
 
 struct Op1 {   // 1-D operator in CSR form on the device
     int rows = 0, cols = 0;
@@ -1284,6 +1285,8 @@ struct MGLevel {
     double *r = nullptr, *t = nullptr, *b = nullptr, *x = nullptr;
     double *Dinv = nullptr;      // 3x3 diagonal-block inverses (smoother)
     double lmax = 1.0;           // lambda_max(D^-1 A) estimate
+    double *pv = nullptr;        // the power iteration's last vector, the next set-up's start
+    int pv_ready = 0;
     double *dense = nullptr;     // coarsest level: LU factors (column-major)
     rocblas_int *ipiv = nullptr, *info = nullptr;
 };
@@ -1313,7 +1316,7 @@ void mg_destroy(MGHier *g)
             op1_free(L.R[d]);
             op1_free(L.T[d]);
         }
-        double *v[] = {L.r, L.t, L.b, L.x, L.Dinv, L.dense};
+        double *v[] = {L.r, L.t, L.b, L.x, L.pv, L.Dinv, L.dense};
         for (double *q : v)
             if (q) (void)hipFree(q);
         if (L.ipiv) (void)hipFree(L.ipiv);
@@ -1460,8 +1463,11 @@ static int mg_1d(const std::vector<double> &kf, int nel, std::vector<double> &kc
 
 // host: out = (T0 x T1 x T2) in over a 3-component field on (n0,n1,n2)
 // points, ops given as row-major dense matrices (nullptr = identity)
+// sep: the input is separable (component q depends on the index along q
+// alone); component q is then interpolated along q only and copied along the
+// other directions, on which it is constant, so the result is separable too
 static std::vector<double> apply3_host(const std::vector<double> &in, const int nin[3], const std::vector<double> *op[3],
-                                       const int nout[3])
+                                       const int nout[3], bool sep)
 {
     std::vector<double> cur = in;
     int dims[3] = {nin[0], nin[1], nin[2]};
@@ -1484,6 +1490,12 @@ static std::vector<double> apply3_host(const std::vector<double> &in, const int 
                         for (int q = 0; q < 3; q++) acc[q] += wv * cur[3 * idx + q];
                     }
                     const size_t o = ((size_t)i0 * od[1] + i1) * od[2] + i2;
+                    if (sep) {
+                        const int a0 = d == 0 ? 0 : i0, a1 = d == 1 ? 0 : i1, a2 = d == 2 ? 0 : i2;
+                        const size_t idx0 = ((size_t)a0 * dims[1] + a1) * dims[2] + a2;
+                        for (int q = 0; q < 3; q++)
+                            if (q != d) acc[q] = cur[3 * idx0 + q];
+                    }
                     for (int q = 0; q < 3; q++) nxt[3 * o + q] = acc[q];
                 }
         cur.swap(nxt);
@@ -1584,7 +1596,7 @@ static int mg_build_levels(igap_gpu_ctx *c, Solver *s, MGHier *g, int maxlev)
             }
             ncd[d] = nelc[d] + 2;
         }
-        std::vector<double> Xc = apply3_host(cf->X_h, nfd, Top, ncd);
+        std::vector<double> Xc = apply3_host(cf->X_h, nfd, Top, ncd, cf->M.separable != 0);
         std::vector<double> wc((size_t)ncd[0] * ncd[1] * ncd[2], 1.0);
         igap_mesh_desc md;
         std::memset(&md, 0, sizeof md);
@@ -1621,7 +1633,7 @@ static int mg_build_levels(igap_gpu_ctx *c, Solver *s, MGHier *g, int maxlev)
     for (int l = 0; l < g->nlev; l++) {
         MGLevel &L = g->lev[l];
         const size_t b = sizeof(double) * (size_t)L.c->ndof;
-        double **v[] = {&L.r, &L.t, &L.b, &L.x};
+        double **v[] = {&L.r, &L.t, &L.b, &L.x, &L.pv};
         for (double **q : v) DALLOC(q, b);
         DALLOC(&L.Dinv, sizeof(double) * 3 * (size_t)L.c->ndof);
     }
@@ -1682,21 +1694,32 @@ static int mg_smoother_setup(igap_gpu_ctx *c, MGHier *g, int l, int p)
         hipLaunchKernelGGL(k_block_diag_inv<float>, gp, bs, 0, c->stream, ncp, (const int64_t *)cl->d_rowptr, (const int32_t *)cl->d_col,
                            (const float *)cl->d_val32, L.Dinv);
     // power iteration on D^-1 A: v <- D^-1 A v / |v|, normalised on the
-    // device; lambda = |D^-1 A v| for the last unit v (one copy back)
-    hipLaunchKernelGGL(k_fill_hash, gn, bs, 0, c->stream, n, L.x);
+    // device; lambda = |D^-1 A v| for the last unit v (one copy back).  The
+    // first set-up starts from a hashed vector, later ones (a new tangent of
+    // the same patch) from the previous set-up's last vector, with fewer
+    // iterations
+    int its = MG_POWER_ITS;
+    if (L.pv_ready) {
+        HIPCHK(launch_copy_f64(L.x, L.pv, n, c->stream));
+        its = MG_POWER_ITS_WARM;
+    } else {
+        hipLaunchKernelGGL(k_fill_hash, gn, bs, 0, c->stream, n, L.x);
+    }
     int rc;
-    for (int it = 0; it < MG_POWER_ITS; it++) {
+    for (int it = 0; it < its; it++) {
         if ((rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h))) return rc;
         hipLaunchKernelGGL(k_scale_rsqrt, gn, bs, 0, c->stream, n, (const double *)L.s->h, L.x);
         if ((rc = spmv(cl, p, L.x, L.r))) return rc;
         hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.x, L.t, 0.0, 1.0, 0);
     }
     if ((rc = dots(cl, L.s, L.x, n, 1, L.x, L.s->h))) return rc;
+    HIPCHK(launch_copy_f64(L.pv, L.x, n, c->stream));
     double q = 0;
     HIPCHK(hipMemcpyAsync(&q, L.s->h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     const double lam = std::sqrt(q);
     L.lmax = lam > 0 ? lam : 1.0;
+    L.pv_ready = lam > 0 ? 1 : 0;
     return IGAP_OK;
 }
 
