@@ -492,6 +492,93 @@ __device__ __forceinline__ void c_row(const double *st, const PhysK &P, int k, i
     }
 }
 
+// c_row for the separable kernel (slots 1..9, row[b - 1] = C[(k,a),(l,b)]):
+// the same entries with the lane's runtime slot indices turned into one-hot
+// vectors and the 2/3 / -1/3 coefficients into per-lane selects, so that
+// every column is straight-line arithmetic at compile-time register
+// indices, and the second-gradient columns of a second-gradient row come
+// from the 3x3 tensor
+//   T[J'][K''] = F_lJ' (d(K'',K) F_kI c(I,J') + [I != K] d(K'',I) F_kK c(K,J')),
+//   C[(k,IK),(l,JK2)] = g2 (T[J][K2] + [J != K2] T[K2][J]),  c(x,y) = d(x,y) - 1/3.
+template <bool CMP>
+__device__ __forceinline__ void c_row_onehot(const double *st, const PhysK &P, int k, int a, int l, double *row)
+{
+    const double g2 = 2 * P.le * P.le;
+    const double dd = (k == l) ? 1.0 : 0.0;
+    const double *F = st + ST_F;
+    double Fl[3], Fk[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) { Fl[m] = F[3 * l + m]; Fk[m] = F[3 * k + m]; }
+    constexpr double C1 = 2.0 / 3.0, C0 = -1.0 / 3.0;
+    if (a <= 3) {
+        const int L = a - 1;
+        double eL[3], cL[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) { eL[m] = (m == L) ? 1.0 : 0.0; cL[m] = (m == L) ? C1 : C0; }
+        const double FkL = F[3 * k + L], FlL = F[3 * l + L];
+        const double hk[3] = {st[st_h<CMP>(k, L, 0)], st[st_h<CMP>(k, L, 1)], st[st_h<CMP>(k, L, 2)]};
+        const double fdot = Fk[0] * Fl[0] + Fk[1] * Fl[1] + Fk[2] * Fl[2];
+        const double e1 = 0.5 * P.e345 * (fdot - FkL * FlL), e2 = 0.5 * P.e345 * FlL;
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            double v = st[st_n<CMP>(L, M)] * FkL * Fl[M];
+            v += eL[M] * e1 + (1.0 - eL[M]) * (e2 * Fk[M]);
+            const double hh = hk[0] * st[st_h<CMP>(l, M, 0)] + hk[1] * st[st_h<CMP>(l, M, 1)] + hk[2] * st[st_h<CMP>(l, M, 2)];
+            v += g2 * cL[M] * hh;
+            v += dd * st[st_s<CMP>(L, M)];
+            row[M] = v;
+        }
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
+            const int I = PI[pp], K = PK[pp];
+            double v = g2 * cL[I] * hk[K] * Fl[I];
+            double gv = eL[I] * st[st_g<CMP>(I, K)];
+            if (I != K) {
+                v += g2 * cL[K] * hk[I] * Fl[K];
+                gv += eL[K] * st[st_g<CMP>(K, I)];
+            }
+            row[3 + pp] = v + dd * gv;
+        }
+    } else {
+        const int s = a - 4;
+        const int I = (s < 3) ? 0 : (s < 5 ? 1 : 2);
+        const int K = (s < 3) ? s : (s < 5 ? s - 2 : 2);
+        const double nIK = (I != K) ? 1.0 : 0.0;
+        double eI[3], eK[3], cI[3], cK[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            eI[m] = (m == I) ? 1.0 : 0.0;
+            eK[m] = (m == K) ? 1.0 : 0.0;
+            cI[m] = (m == I) ? C1 : C0;
+            cK[m] = (m == K) ? C1 : C0;
+        }
+        const double FkI = F[3 * k + I], FkK = F[3 * k + K];
+        const double GIK = st[st_g<CMP>(I, K)], GKI = st[st_g<CMP>(K, I)];
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            double v = g2 * cI[M] * st[st_h<CMP>(l, M, K)] * FkI;
+            v += nIK * (g2 * cK[M] * st[st_h<CMP>(l, M, I)] * FkK);
+            v += dd * (eI[M] * GIK + nIK * eK[M] * GKI);
+            row[M] = v;
+        }
+        const double f1 = FkI, f2 = nIK * FkK;
+        double T[3][3];
+#pragma unroll
+        for (int J = 0; J < 3; J++)
+#pragma unroll
+            for (int K2 = 0; K2 < 3; K2++) T[J][K2] = Fl[J] * (eK[K2] * f1 * cI[J] + eI[K2] * f2 * cK[J]);
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PJ[6] = {0, 0, 0, 1, 1, 2}, PK2[6] = {0, 1, 2, 1, 2, 2};
+            const int J = PJ[pp], K2 = PK2[pp];
+            double v = T[J][K2];
+            if (J != K2) v += T[K2][J];
+            row[3 + pp] = g2 * v;
+        }
+    }
+}
+
 // The transient rule weights the pointwise tangent of eight states
 // us_j = m0 + s_j d.  F and H are linear in the jet, F_j = F0 + s_j Fd and
 // H_j = H0 + s_j Hd, so every product of two of them sums over the nodes to

@@ -1313,7 +1313,12 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
                 double row[NA];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
-                c_row<1, true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
+                // FP64: the branch-light rows; FP32 keeps the select form,
+                // whose lower register count holds three waves per SIMD
+                if constexpr (sizeof(T) == 8)
+                    c_row_onehot<true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], row);
+                else
+                    c_row<1, true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
                 const double wgt = sWgt[q];
                 const T *pz = &sP[2][qz][0][0];
                 T f[FR];
