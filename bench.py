@@ -462,9 +462,11 @@ def newton_load_steps(nel, timed=2, nsteps=40, tip=0.1, ramp_in=2, fp32_maxit=15
       quadratic     the warm start without its network: quadratic
                     extrapolation of the last three solutions, accepted
                     against linear extrapolation by residual norm; FP64
-      fast          the same start, FP64 tangent, preconditioner in FP32
+      fast          the same start, FP64 tangent, multigrid levels in FP32
+                    smoothing with 16-bit copies of their matrices
       fp32_tangent  the same start, Newton on the FP32 matrix-core tangent
-                    (FP32 preconditioner), at most fp32_maxit iterations
+                    (the preconditioner as in fast), at most fp32_maxit
+                    iterations
 
     Every arm uses the same Newton test and the same Krylov rule (GMRES(200)
     to the reference's rtol 1e-5) and the same preconditioner (geometric
@@ -481,13 +483,15 @@ def newton_load_steps(nel, timed=2, nsteps=40, tip=0.1, ramp_in=2, fp32_maxit=15
     arms = {
         "plain": dict(tangent_prec="f64", pc_fp32=0),
         "quadratic": dict(tangent_prec="f64", pc_fp32=0),
-        "fast": dict(tangent_prec="f64", pc_fp32=1),
-        "fp32_tangent": dict(tangent_prec="f32", pc_fp32=1, snes_maxit=fp32_maxit),
+        "fast": dict(tangent_prec="f64", pc_fp32=2),
+        "fp32_tangent": dict(tangent_prec="f32", pc_fp32=2, snes_maxit=fp32_maxit),
     }
     out = {"ramp": f"tip {tip} in {nsteps} load steps", "criterion": "||R|| <= 1e-10 ||R(U_prev; load_k)||",
            "krylov": "GMRES(200), right preconditioned, rtol 1e-5 (every arm)",
            "preconditioner": "geometric multigrid on the B-spline hierarchy (Chebyshev/point-block-Jacobi "
-                             "smoothing, dense coarsest inverse), set up every 4th Newton iteration",
+                             "smoothing, dense coarsest inverse), set up every 4th Newton iteration; FP64 levels "
+                             "in plain and quadratic, FP32 levels smoothing with 16-bit matrices in fast and "
+                             "fp32_tangent",
            "arms": {k: {"ms_per_step": [], "newton_its": [], "ksp_its": [], "converged": [],
                         "final_rel_residual": [], "max_rel_diff_vs_plain": []} for k in arms}}
     with ig.Assm(mesh, ig.PhysGradElast(coef=ig.SVK_COEF), ig.BCClampedTip(tip=0.0)) as A:

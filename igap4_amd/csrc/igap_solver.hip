@@ -181,6 +181,104 @@ __global__ __launch_bounds__(256) void k_spmv_pb(int64_t ncp, const int64_t *__r
     }
 }
 
+// 16-bit smoothing matrix (multigrid levels with pc_fp32 = 2): block k of
+// control point p (its neighbour pcol[blk(p) + k], blk(p) = rowptr[3p] / 9,
+// nn neighbours) is the 3x3 block row-major as binary16, the tenth half zero,
+// in 5 words, word w at h16[5 blk(p) + w nn + k] (word-major per point, so
+// that a wave's load of one word of 64 blocks is 256 contiguous bytes); row r
+// of the block holds a / rs[3p + r], with rs a power of two per row that puts
+// the row's largest magnitude in [2^14, 2^15) (no overflow; entries down to
+// 2^-28 of it stay normal).  20 bytes per block against 36 in FP32.
+__device__ __forceinline__ uint32_t pack_h2(float a, float b)
+{
+    const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+}
+__device__ __forceinline__ double h_lo(uint32_t w) { return (double)(float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
+__device__ __forceinline__ double h_hi(uint32_t w) { return (double)(float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+
+// one wave per control point: the row scales, then the blocks
+__global__ __launch_bounds__(256) void k_pack_h16(int64_t ncp, const int64_t *__restrict__ rp, const float *__restrict__ val,
+                                                  uint32_t *__restrict__ h16, float *__restrict__ rs)
+{
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= ncp) return;
+    const int64_t b[3] = {rp[3 * p], rp[3 * p + 1], rp[3 * p + 2]};
+    const int nn = (int)((b[1] - b[0]) / 3);
+    float inv[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        float m = 0.0f;
+        for (int j = lane; j < 3 * nn; j += 64) m = fmaxf(m, fabsf(val[b[r] + j]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        const int e = m > 0.0f ? ilogbf(m) - 14 : 0;
+        inv[r] = ldexpf(1.0f, -e);
+        if (lane == 0) rs[3 * p + r] = ldexpf(1.0f, e);
+    }
+    uint32_t *hb = h16 + 5 * (b[0] / 9);
+    for (int k = lane; k < nn; k += 64) {
+        float a[10];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) a[3 * r + j] = val[b[r] + 3 * k + j] * inv[r];
+        a[9] = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 5; w++) hb[w * nn + k] = pack_h2(a[2 * w], a[2 * w + 1]);
+    }
+}
+
+// y = A x with the 16-bit blocks (the layout of k_spmv_pb, FP64 sums, the
+// row scale applied once); MODE as k_spmv_pb
+template <int MODE>
+__global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ pcol,
+                                                  const uint32_t *__restrict__ h16, const float *__restrict__ rs,
+                                                  const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ y)
+{
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= ncp) return;
+    const int64_t b0 = rp[3 * p];
+    const int nn = (int)((rp[3 * p + 1] - b0) / 3);
+    const int32_t *pc = pcol + b0 / 9;
+    const uint32_t *hb = h16 + 5 * (b0 / 9);
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = lane; k < nn; k += 64) {
+        const int64_t q = pc[k];
+        const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
+        const uint32_t w0 = hb[k], w1 = hb[nn + k], w2 = hb[2 * nn + k], w3 = hb[3 * nn + k], w4 = hb[4 * nn + k];
+        s0 += h_lo(w0) * x0 + h_hi(w0) * x1 + h_lo(w1) * x2;
+        s1 += h_hi(w1) * x0 + h_lo(w2) * x1 + h_hi(w2) * x2;
+        s2 += h_lo(w3) * x0 + h_hi(w3) * x1 + h_lo(w4) * x2;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+        s0 *= (double)rs[3 * p];
+        s1 *= (double)rs[3 * p + 1];
+        s2 *= (double)rs[3 * p + 2];
+        if (MODE == 0) {
+            y[3 * p] = s0;
+            y[3 * p + 1] = s1;
+            y[3 * p + 2] = s2;
+        } else if (MODE == 1) {
+            y[3 * p] = b[3 * p] - s0;
+            y[3 * p + 1] = b[3 * p + 1] - s1;
+            y[3 * p + 2] = b[3 * p + 2] - s2;
+        } else {
+            y[3 * p] -= s0;
+            y[3 * p + 1] -= s1;
+            y[3 * p + 2] -= s2;
+        }
+    }
+}
+
 // y[i] = x[idx[i]]  (values or vectors into the coloured order)
 // (grid-stride: launched with grid1)
 template <typename T, typename U>
@@ -1192,9 +1290,9 @@ static int pc_apply(igap_gpu_ctx *c, Solver *s, int pc, int p, int order, const 
 }
 
 // y = A x (mode 0), y = b - A x (mode 1), y -= A x (mode 2)
-static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y, int mode = 0, const double *b = nullptr)
+static int ensure_pcol(igap_gpu_ctx *c)
 {
-    const int64_t n = c->ndof, ncp = n / 3;
+    const int64_t ncp = c->ndof / 3;
     if (!c->d_pcol) {
         if (c->nnz % 9) return fail(IGAP_ERR_STATE, "pattern is not point-blocked (nnz %lld)", (long long)c->nnz);
         int32_t *pcol = nullptr;
@@ -1208,6 +1306,14 @@ static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y, int mode = 0
         }
         c->d_pcol = pcol;
     }
+    return IGAP_OK;
+}
+
+static int spmv(igap_gpu_ctx *c, int p, const double *x, double *y, int mode = 0, const double *b = nullptr)
+{
+    const int64_t n = c->ndof, ncp = n / 3;
+    int rc = ensure_pcol(c);
+    if (rc) return rc;
     const dim3 g((unsigned)((ncp + 3) / 4)), bs(256);
     const int64_t *rp = (const int64_t *)c->d_rowptr;
     const int32_t *pc = (const int32_t *)c->d_pcol;
@@ -1266,6 +1372,10 @@ namespace igap {
 constexpr int MG_MAXLEV = 8;
 constexpr int64_t MG_COARSE_NDOF = 1500;
 constexpr int64_t MG_DENSE_MAX = 8000;
+// Preconditioner precision tag beside IGAP_F64 / IGAP_F32: multigrid levels
+// set up in FP32 whose smoothing and residual products use the 16-bit blocks
+// of k_pack_h16 (pc_fp32 = 2)
+constexpr int PC_F16 = 2;
 constexpr int MG_CHEB_DEG = 3;
 constexpr int MG_COARSE_DEG = 12;
 constexpr int MG_POWER_ITS = 20;        // power iterations of the first set-up (from a hashed vector)
@@ -1289,6 +1399,8 @@ struct MGLevel {
     int pv_ready = 0;
     double *dense = nullptr;     // coarsest level: LU factors (column-major)
     rocblas_int *ipiv = nullptr, *info = nullptr;
+    uint32_t *h16 = nullptr;     // 16-bit smoothing matrix (k_pack_h16 layout), pc_fp32 = 2
+    float *rs = nullptr;         // its row scales
 };
 
 struct MGHier {
@@ -1319,6 +1431,8 @@ void mg_destroy(MGHier *g)
         double *v[] = {L.r, L.t, L.b, L.x, L.pv, L.Dinv, L.dense};
         for (double *q : v)
             if (q) (void)hipFree(q);
+        if (L.h16) (void)hipFree(L.h16);
+        if (L.rs) (void)hipFree(L.rs);
         if (L.ipiv) (void)hipFree(L.ipiv);
         if (L.info) (void)hipFree(L.info);
         if (l > 0 && L.c) igap_gpu_destroy(L.c);
@@ -1647,6 +1761,22 @@ static int mg_build_levels(igap_gpu_ctx *c, Solver *s, MGHier *g, int maxlev)
     return IGAP_OK;
 }
 
+// A x on multigrid level l in the preconditioner's precision (MODE of k_spmv_pb)
+static int lvl_spmv(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *x, double *y, int mode = 0, const double *b = nullptr)
+{
+    MGLevel &L = g->lev[l];
+    if (p != PC_F16 || !L.h16) return spmv(L.c, p == PC_F16 ? IGAP_F32 : p, x, y, mode, b);
+    const int64_t ncp = L.c->ndof / 3;
+    const dim3 gr((unsigned)((ncp + 3) / 4)), bs(256);
+    const int64_t *rp = (const int64_t *)L.c->d_rowptr;
+    const int32_t *pc = (const int32_t *)L.c->d_pcol;
+    if (mode == 0) hipLaunchKernelGGL(k_spmv_h16<0>, gr, bs, 0, c->stream, ncp, rp, pc, (const uint32_t *)L.h16, (const float *)L.rs, x, b, y);
+    else if (mode == 1) hipLaunchKernelGGL(k_spmv_h16<1>, gr, bs, 0, c->stream, ncp, rp, pc, (const uint32_t *)L.h16, (const float *)L.rs, x, b, y);
+    else hipLaunchKernelGGL(k_spmv_h16<2>, gr, bs, 0, c->stream, ncp, rp, pc, (const uint32_t *)L.h16, (const float *)L.rs, x, b, y);
+    HIPCHK(hipGetLastError());
+    return IGAP_OK;
+}
+
 // x = Chebyshev(b; x) with point-block Jacobi on level l: deg steps from
 // x = 0 (zero_init) or from the given x
 static int mg_cheb(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *b, double *x, int zero_init, int deg)
@@ -1664,13 +1794,13 @@ static int mg_cheb(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *b, do
     if (zero_init) {
         hipLaunchKernelGGL(k_cheb_first, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, b, x, L.t, L.r, 1.0 / theta);
     } else {
-        if ((rc = spmv(cl, p, x, L.r, 1, b))) return rc;   // r = b - A x
+        if ((rc = lvl_spmv(c, g, l, p, x, L.r, 1, b))) return rc;   // r = b - A x
         hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, 0.0, 1.0 / theta,
                            1);
     }
     double rho = 1.0 / sigma;
     for (int k = 1; k < deg; k++) {
-        if ((rc = spmv(cl, p, L.t, L.r, 2))) return rc;     // r -= A d
+        if ((rc = lvl_spmv(c, g, l, p, L.t, L.r, 2))) return rc;     // r -= A d
         const double rn = 1.0 / (2.0 * sigma - rho);
         hipLaunchKernelGGL(k_cheb_step, gp, bs, 0, c->stream, ncp, (const double *)L.Dinv, (const double *)L.r, L.t, x, rn * rho,
                            2.0 * rn / delta, 1);
@@ -1727,9 +1857,10 @@ static int mg_smoother_setup(igap_gpu_ctx *c, MGHier *g, int l, int p)
 // coarse states and tangents for the current fine state (U_hist of the fine
 // context, level 0 already holding the Newton iterate), smoothers, and the
 // coarsest level's LU
-static int mg_setup(igap_gpu_ctx *c, Solver *s, int p)
+static int mg_setup(igap_gpu_ctx *c, Solver *s, int pm)
 {
     MGHier *g = s->mg;
+    const int p = pm == PC_F16 ? IGAP_F32 : pm;   // precision of the level tangents
     const bool bc_changed = std::memcmp(g->bc_type, c->bc_h.bc_type, sizeof g->bc_type) != 0;
     if (bc_changed) std::memcpy(g->bc_type, c->bc_h.bc_type, sizeof g->bc_type);
     for (int l = 1; l < g->nlev; l++) {
@@ -1769,6 +1900,18 @@ static int mg_setup(igap_gpu_ctx *c, Solver *s, int p)
         } else {
             int rc = mg_smoother_setup(c, g, l, p);
             if (rc) return rc;
+            if (pm == PC_F16) {
+                igap_gpu_ctx *cl = L.c;
+                const int64_t ncp = cl->ndof / 3;
+                if ((rc = ensure_pcol(cl))) return rc;
+                if (!L.h16) {
+                    DALLOC(&L.h16, sizeof(uint32_t) * 5 * (size_t)(cl->nnz / 9));
+                    DALLOC(&L.rs, sizeof(float) * (size_t)cl->ndof);
+                }
+                hipLaunchKernelGGL(k_pack_h16, dim3((unsigned)((ncp + 3) / 4)), dim3(256), 0, c->stream, ncp,
+                                   (const int64_t *)cl->d_rowptr, (const float *)cl->d_val32, L.h16, L.rs);
+                HIPCHK(hipGetLastError());
+            }
         }
     }
     return IGAP_OK;
@@ -1796,7 +1939,7 @@ static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const d
     MGLevel &C = g->lev[l + 1];
     if ((rc = mg_cheb(c, g, l, p, b, x, 1, MG_CHEB_DEG))) return rc;
     // r = b - A x, restricted (zero on coarse Dirichlet rows)
-    if ((rc = spmv(cl, p, x, L.r, 1, b))) return rc;
+    if ((rc = lvl_spmv(c, g, l, p, x, L.r, 1, b))) return rc;
     if ((rc = mg_transfer(c, g, l, 1, L.r, C.b, C.c->d_mask, 0))) return rc;
     if ((rc = mg_cycle(c, s, p, order, l + 1, C.b, C.x))) return rc;
     // x += P x_c (not on Dirichlet rows), then post-smoothing
@@ -1806,12 +1949,21 @@ static int mg_cycle(igap_gpu_ctx *c, Solver *s, int p, int order, int l, const d
     return IGAP_OK;
 }
 
+// precision of the preconditioner for a tangent of precision p: p itself,
+// FP32 (pc_fp32 = 1, or 2 with ILU(0)), or FP32 multigrid levels smoothing
+// with 16-bit matrices (pc_fp32 = 2)
+static int pc_prec_of(const igap_solver_opts *o, int p)
+{
+    if (!o->pc_fp32) return p;
+    return (o->pc_fp32 == 2 && o->pc == IGAP_PC_MG) ? PC_F16 : IGAP_F32;
+}
+
 // GMRES(m), right preconditioned: solve A d = b (d starts at 0)
 static int gmres(igap_gpu_ctx *c, Solver *s, const igap_solver_opts *o, int p, const double *b, double *d, int *its, double *relres,
                  int *conv)
 {
     *conv = 1;
-    const int pp = o->pc_fp32 ? IGAP_F32 : p;   // preconditioner precision
+    const int pp = pc_prec_of(o, p);   // preconditioner precision
     const int64_t n = c->ndof;
     const int m = s->m;
     const rocblas_int N = (rocblas_int)n;
@@ -1933,6 +2085,7 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
     if (o->pc_order != 0 && o->pc_order != 1) return fail(IGAP_ERR_ARG, "pc_order must be 0 (natural) or 1 (coloured)");
     if (o->pc_lag < 0) return fail(IGAP_ERR_ARG, "pc_lag must be >= 0");
     if (o->pc < IGAP_PC_ILU0 || o->pc > IGAP_PC_MG) return fail(IGAP_ERR_ARG, "pc %d unknown", o->pc);
+    if (o->pc_fp32 < 0 || o->pc_fp32 > 2) return fail(IGAP_ERR_ARG, "pc_fp32 must be 0, 1 or 2");
     HIPCHK(hipSetDevice(c->device));
     std::memset(st, 0, sizeof *st);
     auto T0 = clk::now();
@@ -1975,8 +2128,8 @@ extern "C" int igap_gpu_snes_solve(igap_gpu_ctx *c, const igap_solver_opts *o, i
         st->t_tangent_ms += ms(t, clk::now());
         t = clk::now();
         const bool refactor = (it % (o->pc_lag > 1 ? o->pc_lag : 1)) == 0;
-        const int pp = o->pc_fp32 ? IGAP_F32 : p;   // preconditioner precision
-        if (refactor && pp != p && (o->pc == IGAP_PC_ILU0 || o->pc == IGAP_PC_MG)) {
+        const int pp = pc_prec_of(o, p);   // preconditioner precision
+        if (refactor && (pp == PC_F16 ? IGAP_F32 : pp) != p && (o->pc == IGAP_PC_ILU0 || o->pc == IGAP_PC_MG)) {
             // FP32 copy of the FP64 tangent for the preconditioner
             float *v32 = igap_gpu_csr_val32(c);
             if (!v32) return IGAP_ERR_NOMEM;

@@ -245,6 +245,9 @@ typedef struct {
     int    mg_levels;     /* IGAP_PC_MG: at most this many levels (0: up to 8)            */
     int    pc_fp32;       /* 1: preconditioner (ILU(0) factor or multigrid levels) in FP32 */
                           /* with an FP64 tangent: Newton and GMRES keep the FP64 operator */
+                          /* 2: as 1, and the multigrid levels smooth with 16-bit copies   */
+                          /* of their matrices (binary16 blocks, a power-of-two scale per  */
+                          /* row; 20 instead of 36 bytes per 3x3 block); ILU(0) as 1       */
 } igap_solver_opts;
 
 typedef struct {

@@ -106,13 +106,14 @@ def test_own_ilu_matches_rocsparse(igap, oracle, case):
     assert diff < (1e-4 if prec == "f32" else 1e-12)
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "mixed"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "mixed", "mixed16"])
 def test_multigrid_pc_same_answer_fewer_krylov(igap, prec):
     """Geometric multigrid preconditioner (32x8x8 -> 16x4x4 levels,
     Chebyshev smoothing, dense coarsest solve): Newton reaches the
     ILU(0)-preconditioned answer with fewer Krylov iterations, to 1e-9 of the
     displacement scale with an FP64 tangent (also with the preconditioner in
-    FP32, "mixed") and 1e-7 with the FP32 tangent (both solves stop at
+    FP32, "mixed", and with FP32 levels smoothing with 16-bit matrices,
+    "mixed16") and 1e-7 with the FP32 tangent (both solves stop at
     ||R|| ~ 1e-12, whose displacement error the soft bending modes amplify)."""
     m = igap.MeshIGABeam(L=(4, 1, 1), nelem=(32, 8, 8))
     out = {}
@@ -121,7 +122,7 @@ def test_multigrid_pc_same_answer_fewer_krylov(igap, prec):
         with igap.Assm(m, igap.PhysGradElast(coef=igap.SVK_COEF), igap.BCClampedTip(tip=0.02)) as A:
             A.set_uhist(np.zeros(A.ndof))
             st = A.snes_solve(snes_rtol=1e-11, snes_atol=1e-30, tangent_prec=tp, pc=pc, pc_order=1,
-                              pc_fp32=1 if prec == "mixed" else 0)
+                              pc_fp32={"mixed": 1, "mixed16": 2}.get(prec, 0))
             out[pc] = (A.get_uhist()[:A.ndof].copy(), st)
         print(f"  {pc}: converged {out[pc][1]['converged']} newton {out[pc][1]['newton_its']} "
               f"ksp {out[pc][1]['ksp_its']} |F| {out[pc][1]['fnorm']:.2e}")
