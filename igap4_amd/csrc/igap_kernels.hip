@@ -1189,9 +1189,9 @@ __device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int 
     return M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * (sG[0][qd[0]] * sG[1][qd[1]] * sG[2][qd[2]]);
 }
 
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
-                                                        T *__restrict__ Ke, int nel)
+                                                        T *__restrict__ Ke, int nel, Scatter S)
 {
     constexpr int NA = 9;                       // jet slots 1..9 (no value slot in the static tangent)
     constexpr int FR = 18;                      // (s', cz) columns of F and E1
@@ -1212,8 +1212,18 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int e = blockIdx.x;
-    const int nz = M.nelem[2], ny = M.nelem[1];
-    const int ex = e / (ny * nz), ey = (e / nz) % ny, ez = e % nz;
+    int ex, ey, ez;
+    if (DIRECT) {   // element e of the colour S.c
+        const int k = e % S.nc[2], j = (e / S.nc[2]) % S.nc[1], i = e / (S.nc[1] * S.nc[2]);
+        ex = S.c[0] + 3 * i;
+        ey = S.c[1] + 3 * j;
+        ez = S.c[2] + 3 * k;
+    } else {
+        const int nz = M.nelem[2], ny = M.nelem[1];
+        ex = e / (ny * nz);
+        ey = (e / nz) % ny;
+        ez = e % nz;
+    }
     const int ed[3] = {ex, ey, ez};
 
     sep_element<T>(M, Uh, ed, sU, sPd, sP, sG);
@@ -1361,6 +1371,57 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
         __syncthreads();
     }
 
+    if (DIRECT) {
+        // ---- add (row 3b+i, column 3c+j) and the mirrored entry into the CSR
+        //      values, Dirichlet rows and columns left at zero; the elements
+        //      of one colour share no row.  Row starts and masks of the
+        //      element's 81 rows staged in LDS (the F buffer is free after
+        //      the last pencil), then per bx all loads of the nine entries
+        //      (and mirrors) before their stores (one round trip for all 27
+        //      measured slower: the scattered read-modify-write of the CSR
+        //      values, not its latency, is the cost).
+        int64_t *sRow = reinterpret_cast<int64_t *>(sWork);
+        int32_t *sMsk = reinterpret_cast<int32_t *>(sWork + 81 * sizeof(int64_t));
+        if (tid < 81) {
+            const int b = tid / 3, i = tid - 3 * (tid / 3);
+            const int64_t cp = ((int64_t)M.nb[1] * M.nb[2]) * (ex + b / 9) + (int64_t)M.nb[2] * (ey + (b / 3) % 3) + (ez + b % 3);
+            sRow[tid] = S.rowptr[3 * cp + i];
+            sMsk[tid] = S.mask[3 * cp + i];
+        }
+        __syncthreads();
+        if (rt < 162) {
+            const int i = kBlkI[yp], j = kBlkJ[yp];
+#pragma unroll
+            for (int bx = 0; bx < 3; bx++) {
+                const int b = 9 * bx + 3 * yby + ybz, Bx = ex + bx, By = ey + yby, Bz = ez + ybz;
+                T vp[9], vq[9];
+                int64_t pp[9], pq[9];
+                bool ok[9];
+#pragma unroll
+                for (int t = 0; t < 9; t++) {
+                    const int cx = t / 3, cy = t % 3, c = 9 * cx + 3 * cy + ycz;
+                    const int Cx = ex + cx, Cy = ey + cy, Cz = ez + ycz;
+                    ok[t] = sMsk[3 * b + i] == 0 && sMsk[3 * c + j] == 0;
+                    pp[t] = stencil_pos(M, sRow[3 * b + i], Bx, By, Bz, Cx, Cy, Cz, j);
+                    pq[t] = stencil_pos(M, sRow[3 * c + j], Cx, Cy, Cz, Bx, By, Bz, i);
+                    vp[t] = vq[t] = (T)0;
+                    if (ok[t]) {
+                        vp[t] = Ke[pp[t]];
+                        if (i != j) vq[t] = Ke[pq[t]];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 9; t++) {
+                    const T v = K[(3 * bx + t / 3) * 3 + t % 3];
+                    if (ok[t]) {
+                        Ke[pp[t]] = vp[t] + v;
+                        if (i != j) Ke[pq[t]] = vq[t] + v;
+                    }
+                }
+            }
+        }
+        return;
+    }
     // ---- K_e: row 3b+i, column 3c+j, and the mirrored block (3c+j, 3b+i)
     if (rt < 162) {
         const int i = kBlkI[yp], j = kBlkJ[yp];
@@ -1894,8 +1955,8 @@ hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh,
     dim3 grid(nel), block(256);
     const Scatter S{};
     if (P.torder == 0 && M.separable) {
-        if (prec == 0) hipLaunchKernelGGL(k_tangent_sep<double>, grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel);
-        else hipLaunchKernelGGL(k_tangent_sep<float>, grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel);
+        if (prec == 0) hipLaunchKernelGGL((k_tangent_sep<double, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
+        else hipLaunchKernelGGL((k_tangent_sep<float, false>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel, S);
         return hipGetLastError();
     }
     if (prec == 0) {
@@ -1931,7 +1992,10 @@ hipError_t launch_tangent_direct(const MeshK &M, const PhysK &P, const double *U
                 const int64_t n = (int64_t)S.nc[0] * S.nc[1] * S.nc[2];
                 if (n <= 0) continue;
                 dim3 grid((unsigned)n), block(256);
-                if (prec == 0) {
+                if (P.torder == 0 && M.separable) {
+                    if (prec == 0) hipLaunchKernelGGL((k_tangent_sep<double, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
+                    else hipLaunchKernelGGL((k_tangent_sep<float, true>), grid, block, 0, s, M, P, Uh, ndof, (float *)vals, (int)n, S);
+                } else if (prec == 0) {
                     if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
                     else hipLaunchKernelGGL((k_tangent_elem<double, 3, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
                 } else {

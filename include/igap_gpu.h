@@ -109,7 +109,7 @@ int igap_mesh_igabeam(const double L[3], const int nelem[3], const int periodic[
  * Element kernels of the static physics: on a separable patch -- every
  * control coordinate x_d depends on the control point's index along d alone
  * and all weights are equal, as on the igabeam patches -- the residual and
- * the element matrices of the tangent (element-matrix path) come from
+ * the element tangents (either way to the CSR values) come from
  * sum-factorised kernels over 1-D tables of physical derivatives; otherwise,
  * and with the flag below, from the general-geometry kernels.  Both give the
  * same values up to rounding.
