@@ -137,8 +137,8 @@ def pmc_json(nel, fetch_csv, write_csv):
         copy = [mx for k, (_, mx) in d.items() if "copy" in k.lower() or "elementwise" in k.lower()]
         # the FP64 element tangent of the probe: the sum-factorised kernel on
         # the (separable) beam, the general-geometry one otherwise
-        tang = [mean for k, (mean, _) in d.items() if "k_tangent_sep<double>" in k] or \
-               [mean for k, (mean, _) in d.items() if "k_tangent_elem<double, 1>" in k]
+        tang = [mean for k, (mean, _) in d.items() if "k_tangent_sep<double, false>" in k] or \
+               [mean for k, (mean, _) in d.items() if "k_tangent_elem<double, 1, false>" in k]
         if not copy or not tang:
             raise SystemExit(f"{path}: no copy or tangent dispatch found for {name}")
         factor = max(copy) * 1024.0 / COPY_BYTES
@@ -214,10 +214,10 @@ def kernel_rooflines(nel, nelem, nnz, avg, avg32, gen, gen32, genr):
                 "measured_GBs": None if mb is None else mb / (kernel_ms * 1e-3) / 1e9}
 
     return {
-        "k_tangent_sep<double>": flop(avg[1], F_K, PEAK_FP64_TFLOPS, "valu", F_SEP),
-        "k_tangent_sep<float>": flop(avg32[1], F_K, PEAK_FP32_TFLOPS, "valu", F_SEP),
-        "k_tangent_elem<double,1>": flop(gen, F_K, PEAK_FP64_TFLOPS, "mfma"),
-        "k_tangent_elem<float,1>": flop(gen32, F_K, PEAK_FP32_TFLOPS, "mfma"),
+        "k_tangent_sep<double, false>": flop(avg[1], F_K, PEAK_FP64_TFLOPS, "valu", F_SEP),
+        "k_tangent_sep<float, false>": flop(avg32[1], F_K, PEAK_FP32_TFLOPS, "valu", F_SEP),
+        "k_tangent_elem<double, 1, false>": flop(gen, F_K, PEAK_FP64_TFLOPS, "mfma"),
+        "k_tangent_elem<float, 1, false>": flop(gen32, F_K, PEAK_FP32_TFLOPS, "mfma"),
         "k_residual_sep": flop(avg[0], F_R_STATIC, PEAK_FP64_TFLOPS, "valu"),
         "k_residual_elem<1>": flop(genr, F_R_STATIC, PEAK_FP64_TFLOPS, "valu"),
         "k_tangent_gather_rows<double,double>": hbm(avg[2], 8 * (6561 * nelem + nnz),
@@ -409,7 +409,7 @@ def main():
         # sum-factorised kernel executes F_SEP of them, so frac can pass 1
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": "k_tangent_sep<double>", "kernel_ms": avg[1],
+                     "kernel": "k_tangent_sep<double, false>", "kernel_ms": avg[1],
                      "executed_flop_per_element": F_SEP,
                      "executed_tflops": F_SEP * nelem / tk / 1e12},
         "kernels_ms": {"residual_elem": avg[0], "tangent_elem_f64": avg[1], "tangent_gather_f64": avg[2],
