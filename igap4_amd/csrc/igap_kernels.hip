@@ -1058,17 +1058,29 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 // three control coordinates along d (x_d depends on the index along d alone):
 //   sPd[d][q][0][i] = rho_i,  [1][i] = rho_i' / g,  [2][i] = (rho_i'' - (g'/g) rho_i') / g^2
 // (the same in the precision T -> sP when given), and g -> sG[d][q].
-template <typename T>
+template <typename T, int NH = 1>
 __device__ __forceinline__ void sep_element(const MeshK &M, const double *__restrict__ Uh, const int *ed, double *sU,
-                                            double (*sPd)[4][3][3], T (*sP)[4][3][3], double (*sG)[4])
+                                            double (*sPd)[4][3][3], T (*sP)[4][3][3], double (*sG)[4], int64_t ndof = 0,
+                                            double *sUd = nullptr)
 {
     const int tid = threadIdx.x;
     if (tid < 27) {
         const int bx = tid / 9, by = (tid / 3) % 3, bz = tid % 3;
         const int64_t oc = ((int64_t)M.nb[1] * M.nb[2]) * own1(ed[0] + bx, M.nb[0], M.periodic[0])
                          + (int64_t)M.nb[2] * own1(ed[1] + by, M.nb[1], M.periodic[1]) + own1(ed[2] + bz, M.nb[2], M.periodic[2]);
+        if (NH == 1) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) sU[3 * tid + k] = Uh[3 * oc + k];
+            for (int k = 0; k < 3; k++) sU[3 * tid + k] = Uh[3 * oc + k];
+        } else {
+            // the transient rule's two fields: Um = (U^1 + U^2)/2 -> sU and
+            // Ud = (U - U^2)/2 -> sUd (rule node j: Um + s_j Ud)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double u0 = Uh[3 * oc + k], u1 = Uh[ndof + 3 * oc + k], u2 = Uh[2 * ndof + 3 * oc + k];
+                sU[3 * tid + k] = 0.5 * (u1 + u2);
+                sUd[3 * tid + k] = 0.5 * (u0 - u2);
+            }
+        }
     } else if (tid >= 64 && tid < 64 + 12) {
         const int d = (tid - 64) >> 2, q = (tid - 64) & 3;
         const double *kV = M.knot[d] + ed[d];
@@ -1116,10 +1128,10 @@ __device__ __forceinline__ void sep_element(const MeshK &M, const double *__rest
 // Physical displacement jets at the 64 Gauss points of a separable element by
 // sum factorisation over the physical tables (z, y, then x; sWork holds 1188
 // doubles), after sep_element and a barrier.  Thread 4q + 1 + k returns in J
-// the jet of component k at point q (par_periodic's macro-field added),
-// thread 4q zeros.
+// the jet of component k at point q (par_periodic's macro-field added unless
+// macro is false), thread 4q zeros.
 __device__ __forceinline__ void sep_fields(const MeshK &M, const int *ed, const double *sU, const double (*sPd)[4][3][3],
-                                           double *sWork, double *J)
+                                           double *sWork, double *J, bool macro = true)
 {
     const int tid = threadIdx.x;
     double *sA = sWork, *sB = sA + 3 * 108;
@@ -1172,7 +1184,7 @@ __device__ __forceinline__ void sep_fields(const MeshK &M, const int *ed, const 
             const double c = M.gq[qd[d]];
             xq[d] = (-c + 1.0) * kV[2] + c * kV[3];
         }
-        add_periodic1(M, xq, k, J);
+        if (macro) add_periodic1(M, xq, k, J);
     } else {
 #pragma unroll
         for (int a = 0; a < 10; a++) J[a] = 0.0;
@@ -1189,85 +1201,32 @@ __device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int 
     return M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * (sG[0][qd[0]] * sG[1][qd[1]] * sG[2][qd[2]]);
 }
 
-template <typename T, bool DIRECT>
-__global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
-                                                        T *__restrict__ Ke, int nel, Scatter S)
+// The pencil loop of the separable tangents, after stage A and a barrier
+// (256 threads).  rows(q, p, ar, row) gives row[b - 1] = C_q[(k, ar + 1),
+// (l, b)], b = 1..9, of block p = (k, l) at Gauss point q (unweighted; row
+// arrives zeroed); sWgt holds the point weights, sF and sE the F and E1
+// buffers.  The z pass runs on 108 threads (block p, column (s', cz) of E1),
+// the y/x pass on 162 (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]),
+// numbered from the rotated thread index rt so that the workgroups sharing a
+// CU put them on different SIMDs.  The y pass of pencil c runs beside the
+// pointwise stage of pencil c + 1 (F and E1 have buffers of their own), so a
+// pencil takes two barriers.  Returns in K the 27 entries [bx][cx][cy] of
+// thread rt < 162.
+template <typename T, class Rows>
+__device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], const double *sWgt, T *sF, T *sE, Rows rows, T *K)
 {
-    constexpr int NA = 9;                       // jet slots 1..9 (no value slot in the static tangent)
-    constexpr int FR = 18;                      // (s', cz) columns of F and E1
-    constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
-    constexpr int WORK_F = 4 * 6 * NA * FR * (int)sizeof(T);       // F of the pencil's four points
-    __shared__ double sPd[3][4][3][3];          // 1-D tables of physical derivatives [d][q_d][order][i]
-    __shared__ T sP[3][4][3][3];                // the same in the contraction's precision
-    __shared__ double sG[3][4];                 // dx_d / dxi_d at the 1-D Gauss points
-    __shared__ double sU[27 * 3];
-    __shared__ double sWgt[64];
-    __shared__ double sSt[64 * STWC];          // strain-measure states, compact layout
-    __shared__ T sE[6 * 18 * FR];               // E1 of the current pencil
-    __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
-    T *sF = reinterpret_cast<T *>(sWork);
+    constexpr int NA = 9;    // jet slots 1..9 (no value slot in the pointwise rows)
+    constexpr int FR = 18;   // (s', cz) columns of F and E1
     // jet slot a -> order in z, x-y part s
     constexpr int SDZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
     constexpr int SXY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
-
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int e = blockIdx.x;
-    int ex, ey, ez;
-    if (DIRECT) {   // element e of the colour S.c
-        const int k = e % S.nc[2], j = (e / S.nc[2]) % S.nc[1], i = e / (S.nc[1] * S.nc[2]);
-        ex = S.c[0] + 3 * i;
-        ey = S.c[1] + 3 * j;
-        ez = S.c[2] + 3 * k;
-    } else {
-        const int nz = M.nelem[2], ny = M.nelem[1];
-        ex = e / (ny * nz);
-        ey = (e / nz) % ny;
-        ez = e % nz;
-    }
-    const int ed[3] = {ex, ey, ez};
-
-    sep_element<T>(M, Uh, ed, sU, sPd, sP, sG);
-    __syncthreads();
-    // ---- stage A: displacement jets, strain-measure states, point weights
-    {
-        double J[10];
-        sep_fields(M, ed, sU, sPd, reinterpret_cast<double *>(sWork), J);
-        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
-        double u[30];
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-#pragma unroll
-            for (int a = 0; a < 10; a++) u[10 * k + a] = __shfl(J[a], lq + 1 + k, 64);
-        if (part == 0) {
-            State st;
-            phys_state(u, P, st);
-            double *sp = sSt + q * STWC;
-            constexpr int PR[6] = {0, 1, 2, 4, 5, 8};   // index pairs I <= J in slot order
-#pragma unroll
-            for (int i = 0; i < 9; i++) { sp[i] = st.F[i]; sp[STC_G + i] = st.G[i]; }
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-#pragma unroll
-                for (int t = 0; t < 6; t++) sp[STC_H + 6 * k + t] = u[10 * k + 4 + t];
-#pragma unroll
-            for (int t = 0; t < 6; t++) { sp[STC_S + t] = st.S[PR[t]]; sp[STC_N + t] = st.N[PR[t]]; }
-            sWgt[q] = sep_weight(M, ed, q, sG);
-        }
-    }
-    __syncthreads();
-
-    // The z pass runs on 108 threads (block p, column (s', cz) of E1), the
-    // y/x pass on 162 (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]),
-    // numbered from a rotation of the thread index so that the workgroups
-    // sharing a CU put them on different SIMDs.  The y pass of pencil c runs
-    // beside the pointwise stage of pencil c + 1 (F and E1 have buffers of
-    // their own), so a pencil takes two barriers.
-    const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
+    (void)tid;
     const int zp = rt / 18, zr = rt - 18 * zp;
     const int yp = rt / 27, yu = rt - 27 * yp;
     const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
     constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
-    T E2[27], K[27];   // [dx][dx'][cy], [bx][cx][cy]
+    T E2[27];   // [dx][dx'][cy]
 #pragma unroll
     for (int i = 0; i < 27; i++) { E2[i] = (T)0; K[i] = (T)0; }
 
@@ -1323,12 +1282,7 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
                 double row[NA];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
-                // FP64: the branch-light rows; FP32 keeps the select form,
-                // whose lower register count holds three waves per SIMD
-                if constexpr (sizeof(T) == 8)
-                    c_row_onehot<true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], row);
-                else
-                    c_row<1, true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
+                rows(q, p, ar, row);
                 const double wgt = sWgt[q];
                 const T *pz = &sP[2][qz][0][0];
                 T f[FR];
@@ -1371,7 +1325,19 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
         __syncthreads();
     }
 
-    if (DIRECT) {
+}
+
+// Write-out of the separable tangents: thread rt < 162 holds the entries
+// K[bx][cx][cy] of block (i, j) = (kBlkI[yp], kBlkJ[yp]) for its (by, bz, cz);
+// sWork (at least 972 bytes) is free.
+template <typename T, bool DIRECT>
+__device__ __forceinline__ void sep_write(const MeshK &M, const Scatter &S, T *__restrict__ Ke, int e, int ex, int ey, int ez, int rt,
+                                          const T *K, char *sWork)
+{
+    const int tid = threadIdx.x;
+    const int yp = rt / 27, yu = rt - 27 * yp;
+    const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
+    if constexpr (DIRECT) {
         // ---- add (row 3b+i, column 3c+j) and the mirrored entry into the CSR
         //      values, Dirichlet rows and columns left at zero; the elements
         //      of one colour share no row.  Row starts and masks of the
@@ -1438,6 +1404,265 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
                     if (i != j) out[(3 * c + j) * 81 + 3 * b + i] = v;
                 }
     }
+}
+
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                        T *__restrict__ Ke, int nel, Scatter S)
+{
+    constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
+    constexpr int WORK_F = 4 * 6 * 9 * 18 * (int)sizeof(T);        // F of the pencil's four points (9 rows x 18 columns)
+    __shared__ double sPd[3][4][3][3];          // 1-D tables of physical derivatives [d][q_d][order][i]
+    __shared__ T sP[3][4][3][3];                // the same in the contraction's precision
+    __shared__ double sG[3][4];                 // dx_d / dxi_d at the 1-D Gauss points
+    __shared__ double sU[27 * 3];
+    __shared__ double sWgt[64];
+    __shared__ double sSt[64 * STWC];          // strain-measure states, compact layout
+    __shared__ T sE[6 * 18 * 18];               // E1 of the current pencil
+    __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
+    T *sF = reinterpret_cast<T *>(sWork);
+
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x;
+    int ex, ey, ez;
+    if (DIRECT) {   // element e of the colour S.c
+        const int k = e % S.nc[2], j = (e / S.nc[2]) % S.nc[1], i = e / (S.nc[1] * S.nc[2]);
+        ex = S.c[0] + 3 * i;
+        ey = S.c[1] + 3 * j;
+        ez = S.c[2] + 3 * k;
+    } else {
+        const int nz = M.nelem[2], ny = M.nelem[1];
+        ex = e / (ny * nz);
+        ey = (e / nz) % ny;
+        ez = e % nz;
+    }
+    const int ed[3] = {ex, ey, ez};
+
+    sep_element<T>(M, Uh, ed, sU, sPd, sP, sG);
+    __syncthreads();
+    // ---- stage A: displacement jets, strain-measure states, point weights
+    {
+        double J[10];
+        sep_fields(M, ed, sU, sPd, reinterpret_cast<double *>(sWork), J);
+        const int q = tid >> 2, part = tid & 3, lq = (tid & 63) & ~3;
+        double u[30];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int a = 0; a < 10; a++) u[10 * k + a] = __shfl(J[a], lq + 1 + k, 64);
+        if (part == 0) {
+            State st;
+            phys_state(u, P, st);
+            double *sp = sSt + q * STWC;
+            constexpr int PR[6] = {0, 1, 2, 4, 5, 8};   // index pairs I <= J in slot order
+#pragma unroll
+            for (int i = 0; i < 9; i++) { sp[i] = st.F[i]; sp[STC_G + i] = st.G[i]; }
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int t = 0; t < 6; t++) sp[STC_H + 6 * k + t] = u[10 * k + 4 + t];
+#pragma unroll
+            for (int t = 0; t < 6; t++) { sp[STC_S + t] = st.S[PR[t]]; sp[STC_N + t] = st.N[PR[t]]; }
+            sWgt[q] = sep_weight(M, ed, q, sG);
+        }
+    }
+    __syncthreads();
+
+    const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
+    T K[27];
+    // FP64: the branch-light rows; FP32 keeps the select form, whose lower
+    // register count holds three waves per SIMD
+    auto rows = [&](int q, int p, int ar, double *row) {
+        if constexpr (sizeof(T) == 8)
+            c_row_onehot<true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], row);
+        else
+            c_row<1, true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
+    };
+    sep_pencils<T>(rt, sP, sWgt, sF, sE, rows, K);
+    sep_write<T, DIRECT>(M, S, Ke, e, ex, ey, ez, rt, K, sWork);
+}
+
+// Transient tangent on a separable patch (k_tangent_sep_t, gradelasttime):
+// the rule-weighted pointwise rows of k_tangent_elem<..., 3> (c_row_mom: the
+// eight rule-node states us_j = Um + s_j Ud collapsed into moments) in the
+// pencil loop of k_tangent_sep, plus the inertia term.  The inertia block
+// D[(k,0),(k,0)] = w_q (cii / (2 dt) + rho / dt^2) pairs only basis values,
+// and w_q and the basis value are products over the directions, so its part
+// of K^kk is mc Mx[bx][cx] My[by][cy] Mz[bz][cz] with the 1-D matrices
+// M_d[b][c] = sum_q gw_q dX_d g_d(q) rho_b(q) rho_c(q).  Stage A: the jets of
+// Um and Ud (two field passes), then the moment states of all 64 points,
+// compact and in the contraction's precision (one lane per rule node, sums
+// over 8-lane groups), so that the FP32 kernel keeps two workgroups per CU
+// (FP64: 95 KB of LDS, one).
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t(MeshK M, PhysK P, const double *__restrict__ Uh,
+                                                                               int64_t ndof, T *__restrict__ Ke, int nel, Scatter S)
+{
+    constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of 3 components
+    constexpr int WORK_F = 4 * 6 * 9 * 18 * (int)sizeof(T);        // F of the pencil's four points
+    constexpr int WORK_J = 64 * 54 * (int)sizeof(T);               // jets of Um and Ud, slots 1..9
+    constexpr int WORK = WORK_A > WORK_F ? (WORK_A > WORK_J ? WORK_A : WORK_J) : (WORK_F > WORK_J ? WORK_F : WORK_J);
+    __shared__ double sPd[3][4][3][3];
+    __shared__ T sP[3][4][3][3];
+    __shared__ double sG[3][4];
+    __shared__ double sU[27 * 3], sUd[27 * 3];
+    __shared__ double sWgt[64];
+    __shared__ double sMass[3][3][3];          // 1-D mass matrices M_d[b][c]
+    __shared__ T sMom[64 * MWC];               // compact moment states
+    __shared__ T sE[6 * 18 * 18];
+    __shared__ __attribute__((aligned(16))) char sWork[WORK];
+    T *sF = reinterpret_cast<T *>(sWork);
+    T *sJet = reinterpret_cast<T *>(sWork);     // [q][h][k][a - 1], stage A only
+
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x;
+    int ex, ey, ez;
+    if (DIRECT) {
+        const int k = e % S.nc[2], j = (e / S.nc[2]) % S.nc[1], i = e / (S.nc[1] * S.nc[2]);
+        ex = S.c[0] + 3 * i;
+        ey = S.c[1] + 3 * j;
+        ez = S.c[2] + 3 * k;
+    } else {
+        const int nz = M.nelem[2], ny = M.nelem[1];
+        ex = e / (ny * nz);
+        ey = (e / nz) % ny;
+        ez = e % nz;
+    }
+    const int ed[3] = {ex, ey, ez};
+
+    sep_element<T, 3>(M, Uh, ed, sU, sPd, sP, sG, ndof, sUd);
+    __syncthreads();
+    // ---- stage A: jets of Um and Ud, point weights, 1-D mass matrices
+    {
+        double J1[10], J2[10];
+        sep_fields(M, ed, sU, sPd, reinterpret_cast<double *>(sWork), J1, true);
+        __syncthreads();
+        sep_fields(M, ed, sUd, sPd, reinterpret_cast<double *>(sWork), J2, false);
+        __syncthreads();
+        const int q = tid >> 2, part = tid & 3;
+        if (part > 0) {
+            const int k = part - 1;
+            T *jq = sJet + q * 54;
+#pragma unroll
+            for (int a = 1; a < 10; a++) {
+                jq[9 * k + a - 1] = (T)J1[a];
+                jq[27 + 9 * k + a - 1] = (T)J2[a];
+            }
+        } else {
+            sWgt[q] = sep_weight(M, ed, q, sG);
+        }
+        if (tid < 27) {
+            const int d = tid / 9, b = (tid / 3) % 3, c = tid % 3;
+            const double dX = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
+            double m = 0.0;
+#pragma unroll
+            for (int qq = 0; qq < 4; qq++) m += M.gw[qq] * dX * sG[d][qq] * sPd[d][qq][0][b] * sPd[d][qq][0][c];
+            sMass[d][b][c] = m;
+        }
+    }
+    __syncthreads();
+    // ---- moment states: lane 8g + j takes rule node j of point g (32 points
+    //      per round, two rounds); node sums over the 8-lane group one entry
+    //      at a time; F^ and H^ from the jets, 3-4 entries per lane
+    {
+        Mom mu = {0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            mu.m0 += P.ct[j];
+            mu.m1 += P.ct[j] * P.s[j];
+            mu.m2 += P.ct[j] * P.s[j] * P.s[j];
+        }
+        mu.factor();
+        const double c12 = 1.0 / (mu.a11 * mu.a22), c22 = 1.0 / mu.d2;
+        const int j = tid & 7;
+        for (int round = 0; round < 2; round++) {
+            const int q = (tid >> 3) + 32 * round;
+            const T *jq = sJet + q * 54;
+            T *ms = sMom + q * MWC;
+            {
+                double us[30];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    us[10 * k] = 0.0;
+#pragma unroll
+                    for (int a = 1; a < 10; a++) us[10 * k + a] = (double)jq[9 * k + a - 1] + P.s[j] * (double)jq[27 + 9 * k + a - 1];
+                }
+                State st;
+                phys_state(us, P, st);
+                const double w0 = P.ct[j], w1 = w0 * P.s[j], w2 = w1 * P.s[j];
+                constexpr int PR[6] = {0, 1, 2, 4, 5, 8};   // index pairs I <= J in slot order
+#pragma unroll
+                for (int t = 0; t < 6; t++) {
+                    double sv = w0 * st.S[PR[t]], n0 = w0 * st.N[PR[t]], n1 = w1 * st.N[PR[t]], n2 = w2 * st.N[PR[t]];
+#pragma unroll
+                    for (int o = 1; o < 8; o <<= 1) {
+                        sv += __shfl_xor(sv, o, 64);
+                        n0 += __shfl_xor(n0, o, 64);
+                        n1 += __shfl_xor(n1, o, 64);
+                        n2 += __shfl_xor(n2, o, 64);
+                    }
+                    if (j == 0) {
+                        ms[MC_S + t] = (T)sv;
+                        ms[MC_N11 + t] = (T)(n0 / mu.m0);
+                        ms[MC_N12 + t] = (T)((n1 - mu.lf * n0) * c12);
+                        ms[MC_N22 + t] = (T)(((mu.lf * n0 - n1) * mu.lf + (n2 - mu.lf * n1)) * c22);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 9; i++) {
+                    double gv = w0 * st.G[i];
+#pragma unroll
+                    for (int o = 1; o < 8; o <<= 1) gv += __shfl_xor(gv, o, 64);
+                    if (j == 0) ms[MC_G + i] = (T)gv;
+                }
+            }
+            // F^ (r < 9: F_kL, k = r / 3, L = r % 3) and H^ (r >= 9: H_k,IK in
+            // slot 4 + t of component k = (r - 9) / 6): X^_1 = a11 (X0 + lf Xd),
+            // X^_2 = a22 Xd
+            for (int r = j; r < 27; r += 8) {
+                int src;
+                double x0;
+                if (r < 9) {
+                    src = 9 * (r / 3) + (r % 3);
+                    x0 = (double)jq[src] + ((r / 3) == (r % 3) ? 1.0 : 0.0);
+                } else {
+                    src = 9 * ((r - 9) / 6) + 3 + (r - 9) % 6;
+                    x0 = (double)jq[src];
+                }
+                const double xd = (double)jq[27 + src];
+                const double x1 = mu.a11 * (x0 + mu.lf * xd), x2 = mu.a22 * xd;
+                if (r < 9) {
+                    ms[MC_F1 + r] = (T)x1;
+                    ms[MC_F2 + r] = (T)x2;
+                } else {
+                    ms[MC_H1 + r - 9] = (T)x1;
+                    ms[MC_H2 + r - 9] = (T)x2;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
+    T K[27];
+    auto rows = [&](int q, int p, int ar, double *row) { c_row_momc<T>(sMom + q * MWC, P, kBlkI[p], ar + 1, kBlkJ[p], row); };
+    sep_pencils<T>(rt, sP, sWgt, sF, sE, rows, K);
+    // inertia on the diagonal blocks
+    if (rt < 162) {
+        const int yp = rt / 27, yu = rt - 27 * yp;
+        const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
+        if (kBlkI[yp] == kBlkJ[yp]) {
+            const double mc = P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
+#pragma unroll
+            for (int bx = 0; bx < 3; bx++)
+#pragma unroll
+                for (int cx = 0; cx < 3; cx++)
+#pragma unroll
+                    for (int cy = 0; cy < 3; cy++)
+                        K[(3 * bx + cx) * 3 + cy] += (T)(mc * sMass[0][bx][cx] * sMass[1][yby][cy] * sMass[2][ybz][ycz]);
+        }
+    }
+    sep_write<T, DIRECT>(M, S, Ke, e, ex, ey, ez, rt, K, sWork);
 }
 
 // ============================================================================
@@ -1959,6 +2184,11 @@ hipError_t launch_tangent_elem(const MeshK &M, const PhysK &P, const double *Uh,
         else hipLaunchKernelGGL((k_tangent_sep<float, false>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel, S);
         return hipGetLastError();
     }
+    if (P.torder == 2 && M.separable) {
+        if (prec == 0) hipLaunchKernelGGL((k_tangent_sep_t<double, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
+        else hipLaunchKernelGGL((k_tangent_sep_t<float, false>), grid, block, 0, s, M, P, Uh, ndof, (float *)Ke, nel, S);
+        return hipGetLastError();
+    }
     if (prec == 0) {
         if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
         else hipLaunchKernelGGL((k_tangent_elem<double, 3, false>), grid, block, 0, s, M, P, Uh, ndof, (double *)Ke, nel, S);
@@ -1995,6 +2225,9 @@ hipError_t launch_tangent_direct(const MeshK &M, const PhysK &P, const double *U
                 if (P.torder == 0 && M.separable) {
                     if (prec == 0) hipLaunchKernelGGL((k_tangent_sep<double, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
                     else hipLaunchKernelGGL((k_tangent_sep<float, true>), grid, block, 0, s, M, P, Uh, ndof, (float *)vals, (int)n, S);
+                } else if (P.torder == 2 && M.separable) {
+                    if (prec == 0) hipLaunchKernelGGL((k_tangent_sep_t<double, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
+                    else hipLaunchKernelGGL((k_tangent_sep_t<float, true>), grid, block, 0, s, M, P, Uh, ndof, (float *)vals, (int)n, S);
                 } else if (prec == 0) {
                     if (P.torder == 0) hipLaunchKernelGGL((k_tangent_elem<double, 1, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);
                     else hipLaunchKernelGGL((k_tangent_elem<double, 3, true>), grid, block, 0, s, M, P, Uh, ndof, (double *)vals, (int)n, S);

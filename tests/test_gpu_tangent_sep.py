@@ -1,5 +1,5 @@
 """The sum-factorised residual and tangent of separable patches
-(k_residual_sep, k_tangent_sep).
+(k_residual_sep, k_tangent_sep; the transient tangent k_tangent_sep_t).
 
 A patch is separable when every control coordinate x_d depends on the control
 point's index along d alone and all weights are equal (the test of
@@ -13,7 +13,11 @@ g' term of the second derivative is exercised), a graded patch (non-uniform
 knots, control coordinates a nonlinear function of the index, all weights
 2), a fully periodic patch with the par_periodic macro-field, and a truly
 distorted igabeamtest patch, which is not separable and takes the general
-kernel whatever the flag."""
+kernel whatever the flag.  The "_t" cases use the transient physics
+(gradelasttime, three history levels of a random state; the tangent's
+rule-weighted rows and inertia term, src/assmcore_iga.c:607-764 with
+phys/gradelasttime.c) on the same patches: the tangent takes the
+sum-factorised kernel, the residual the general one."""
 import numpy as np
 import pytest
 
@@ -48,6 +52,8 @@ def grade(mesh):
 
 
 def build(igap, oracle, name):
+    torder = 2 if name.endswith("_t") else 0
+    name = name[:-2] if torder else name
     par_periodic = None
     if name == "beam":
         kw = dict(nelem=(16, 4, 4), L=(4, 1, 1))
@@ -69,11 +75,23 @@ def build(igap, oracle, name):
         bc_o, bc_g = oracle.make_bc(), igap.BCDefault()
     else:
         bc_o, bc_g = oracle.clamped_tip_bc(tip=0.02), igap.BCClampedTip(tip=0.02)
-    U = oracle.glibc_rand_state(mo.ndof) * 0.05 if name != "beam" else oracle.smooth_state(mo, amp=0.02)
-    return mo, mg, bc_o, bc_g, U
+    nh = torder + 1
+    if name == "beam":
+        U = oracle.smooth_state(mo, amp=0.02, levels=nh)
+        for h in range(1, nh):   # distinct history levels (the rule's Ud != 0)
+            U[h * mo.ndof:(h + 1) * mo.ndof] *= 1.0 - 0.12 * h
+    else:
+        U = oracle.glibc_rand_state(nh * mo.ndof) * 0.05
+    return mo, mg, bc_o, bc_g, U, torder
 
 
-CASES = ["beam", "graded", "periodic", "distorted"]
+def phys(igap, oracle, torder):
+    if torder == 0:
+        return oracle.make_phys(0), igap.PhysGradElast()
+    return oracle.make_phys(2), igap.PhysGradElastTime()
+
+
+CASES = ["beam", "graded", "periodic", "distorted", "beam_t", "graded_t", "periodic_t", "distorted_t"]
 
 
 @pytest.fixture(scope="module")
@@ -83,14 +101,14 @@ def built(igap, oracle):
 
 def test_separable_cases(built):
     for n in CASES:
-        assert separable(built[n][1]) == (n != "distorted"), n
+        assert separable(built[n][1]) == (not n.startswith("distorted")), n
         assert np.array_equal(built[n][0].X, built[n][1].X) and np.array_equal(built[n][0].w, built[n][1].w)
 
 
 @pytest.mark.parametrize("case", CASES)
 def test_tangent_vs_oracle(igap, oracle, built, case):
-    mo, mg, bc_o, bc_g, U = built[case]
-    ph_o, ph_g = oracle.make_phys(0), igap.PhysGradElast()
+    mo, mg, bc_o, bc_g, U, torder = built[case]
+    ph_o, ph_g = phys(igap, oracle, torder)
     rp_o, col_o, K_o = oracle.tangent(mo, ph_o, bc_o, U, nthreads=NTHREADS)
     with igap.Assm(mg, ph_g, bc_g) as A:
         rp, col, K = A.tangent(U, prec="f64")
@@ -103,8 +121,8 @@ def test_tangent_vs_oracle(igap, oracle, built, case):
 @pytest.mark.parametrize("case", CASES)
 def test_residual_vs_oracle_and_general(igap, oracle, built, case):
     from igap4_amd import _lib
-    mo, mg, bc_o, bc_g, U = built[case]
-    ph_o, ph_g = oracle.make_phys(0), igap.PhysGradElast()
+    mo, mg, bc_o, bc_g, U, torder = built[case]
+    ph_o, ph_g = phys(igap, oracle, torder)
     R_o = oracle.residual(mo, ph_o, bc_o, U, nthreads=NTHREADS)
     with igap.Assm(mg, ph_g, bc_g) as A:
         R = A.residual(U)
@@ -114,16 +132,16 @@ def test_residual_vs_oracle_and_general(igap, oracle, built, case):
     e_o, e_g = rowfloor_err(R, R_o), rowfloor_err(R, Rg)
     print(f"[{case}] residual: max rel err vs oracle {e_o:.3e}, vs general kernel {e_g:.3e}")
     assert np.array_equal(R, R2)
-    if case == "distorted":
+    if case.startswith("distorted") or torder:   # the general residual kernel either way
         assert np.array_equal(R, Rg)
     assert e_o < TOL64 and e_g < TOL64
 
 
 @pytest.mark.parametrize("case", CASES)
-def test_sum_factorised_vs_general_kernel(igap, built, case):
-    _, mg, _, bc_g, U = built[case]
+def test_sum_factorised_vs_general_kernel(igap, oracle, built, case):
+    _, mg, _, bc_g, U, torder = built[case]
     from igap4_amd import _lib
-    ph = igap.PhysGradElast()
+    ph = phys(igap, oracle, torder)[1]
     out = {}
     for name, fl in (("default", 0), ("general", _lib.IGAP_GENERAL_GEOMETRY)):
         with igap.Assm(mg, ph, bc_g, flags=fl) as A:
@@ -133,7 +151,7 @@ def test_sum_factorised_vs_general_kernel(igap, built, case):
     e64 = rowfloor_err(out["default"][0], out["general"][0], rp)
     e32 = rowfloor_err(out["default"][1], out["general"][0], rp)
     print(f"[{case}] default vs general FP64 {e64:.3e}; default FP32 vs general FP64 {e32:.3e}")
-    if case == "distorted":
+    if case.startswith("distorted"):
         # not separable: both flags take the general kernel
         assert np.array_equal(out["default"][0], out["general"][0])
         assert np.array_equal(out["default"][1], out["general"][1])
@@ -159,3 +177,33 @@ def test_full_patch_and_repeatable(igap):
     e64, e32 = rowfloor_err(k1, kg, rp), rowfloor_err(f1, kg, rp)
     print(f"[64x16x16] sum-factorised vs general FP64 {e64:.3e}, FP32 vs general FP64 {e32:.3e}")
     assert e64 < TOL64 and e32 < TOL32
+
+
+def test_transient_full_patch_direct_and_repeatable(igap):
+    """64x16x16 transient beam (three smooth history levels): the
+    sum-factorised tangent agrees with the general kernel on every entry in
+    both precisions, is bitwise repeatable, and its direct scatter (27 colour
+    launches into the CSR values) agrees with the element-matrix path."""
+    from igap4_amd import _lib
+    mesh = igap.MeshIGABeam(L=(4, 1, 1), nelem=(64, 16, 16))
+    U = igap.smooth_state(mesh, amp=0.02, levels=3)
+    n = U.size // 3
+    U[n:2 * n] *= 0.88   # distinct history levels (the rule's Ud != 0)
+    U[2 * n:] *= 0.76
+    ph, bc = igap.PhysGradElastTime(), igap.BCClampedTip(tip=0.05)
+    with igap.Assm(mesh, ph, bc, flags=_lib.IGAP_TANGENT_ELEMENT_MATRICES) as A:
+        rp, _, k1 = A.tangent(U)
+        _, _, k2 = A.tangent(U)
+        _, _, f1 = A.tangent(U, prec="f32")
+    with igap.Assm(mesh, ph, bc, flags=_lib.IGAP_TANGENT_DIRECT) as A:
+        _, _, kd = A.tangent(U)
+        _, _, kd2 = A.tangent(U)
+        _, _, fd = A.tangent(U, prec="f32")
+    with igap.Assm(mesh, ph, bc, flags=_lib.IGAP_GENERAL_GEOMETRY) as A:
+        _, _, kg = A.tangent(U)
+    assert np.array_equal(k1, k2) and np.array_equal(kd, kd2)
+    e64, e32 = rowfloor_err(k1, kg, rp), rowfloor_err(f1, kg, rp)
+    ed, ed32 = rowfloor_err(kd, k1, rp), rowfloor_err(fd, kg, rp)
+    print(f"[64x16x16 transient] sum-factorised vs general FP64 {e64:.3e}, FP32 {e32:.3e}; "
+          f"direct vs element matrices {ed:.3e}, direct FP32 vs general FP64 {ed32:.3e}")
+    assert e64 < TOL64 and e32 < TOL32 and ed < TOL64 and ed32 < TOL32
