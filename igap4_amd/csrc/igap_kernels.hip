@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 template <typename T, int NH = 1>
 __device__ __forceinline__ void sep_element(const MeshK &M, const double *__restrict__ Uh, const int *ed, double *sU,
                                             double (*sPd)[4][3][3], T (*sP)[4][3][3], double (*sG)[4], int64_t ndof = 0,
-                                            double *sUd = nullptr)
+                                            double *sUd = nullptr, double *sV = nullptr, double cv1 = 0.0, double cv2 = 0.0)
 {
     const int tid = threadIdx.x;
     if (tid < 27) {
@@ -1073,12 +1073,15 @@ __device__ __forceinline__ void sep_element(const MeshK &M, const double *__rest
             for (int k = 0; k < 3; k++) sU[3 * tid + k] = Uh[3 * oc + k];
         } else {
             // the transient rule's two fields: Um = (U^1 + U^2)/2 -> sU and
-            // Ud = (U - U^2)/2 -> sUd (rule node j: Um + s_j Ud)
+            // Ud = (U - U^2)/2 -> sUd (rule node j: Um + s_j Ud); with sV, the
+            // nodal cv1 (U - U^2) + cv2 (U + U^2 - 2 U^1)
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const double u0 = Uh[3 * oc + k], u1 = Uh[ndof + 3 * oc + k], u2 = Uh[2 * ndof + 3 * oc + k];
                 sU[3 * tid + k] = 0.5 * (u1 + u2);
                 sUd[3 * tid + k] = 0.5 * (u0 - u2);
+                // the inertia combination of the residual's value slot
+                if (sV) sV[3 * tid + k] = cv1 * (u0 - u2) + cv2 * (u0 + u2 - 2 * u1);
             }
         }
     } else if (tid >= 64 && tid < 64 + 12) {
@@ -1665,6 +1668,64 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t
     sep_write<T, DIRECT>(M, S, Ke, e, ex, ey, ez, rt, K, sWork);
 }
 
+// Transposed sum factorisation of the weighted pointwise residual jets
+// sR[q * RS + 10 k + a] (after a barrier; sWork holds 1188 doubles): returns
+// R_e[tid] = sum_q sum_a r_q[k][a] R_q[a][b] (tid = 3 b + k < 81), the
+// reference's projection (src/assmcore_iga.c:230-252) as x -> y -> z passes
+// over the physical tables; 0 for tid >= 81.
+template <int RS>
+__device__ __forceinline__ double sep_project(const double (*sPd)[4][3][3], const double *sR, double *sWork)
+{
+    // ---- x^T: C1[k][bx][qy][qz][s6] = sum_qx sum_{a: S6(a) = s6} r[q][k][a] Px[qx][DX(a)][bx]
+    const int tid = threadIdx.x;
+    double *C1 = sWork, *C2 = sWork + 864;
+    for (int i = tid; i < 864; i += 256) {   // s6 slowest: the switch below is mostly wave-uniform
+        const int s6 = i / 144, t = i - 144 * s6, qz = t & 3, qy = (t >> 2) & 3, bx = (t >> 4) % 3, k = t / 48;
+        double acc = 0.0;
+#pragma unroll
+        for (int qx = 0; qx < 4; qx++) {
+            const double *rq = sR + (16 * qx + 4 * qy + qz) * RS + 10 * k;
+            const double *tx = &sPd[0][qx][0][0];   // tx[3 * dx + bx]
+            switch (s6) {
+                case 0: acc += rq[0] * tx[bx] + rq[1] * tx[3 + bx] + rq[4] * tx[6 + bx]; break;
+                case 1: acc += rq[2] * tx[bx] + rq[5] * tx[3 + bx]; break;
+                case 2: acc += rq[3] * tx[bx] + rq[6] * tx[3 + bx]; break;
+                case 3: acc += rq[7] * tx[bx]; break;
+                case 4: acc += rq[8] * tx[bx]; break;
+                default: acc += rq[9] * tx[bx]; break;
+            }
+        }
+        C1[6 * t + s6] = acc;   // [k][bx][qy][qz][s6]
+    }
+    __syncthreads();
+    // ---- y^T: C2[k][bx][by][qz][dz] = sum_qy sum_{s6: dz(s6) = dz} C1[..][s6] Py[qy][dy(s6)][by]
+    for (int i = tid; i < 324; i += 256) {   // dz slowest (mostly wave-uniform branches)
+        const int dz = i / 108, t = i - 108 * dz, qz = t & 3, u = t >> 2, by = u % 3, bx = (u / 3) % 3, k = u / 9;
+        double acc = 0.0;
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const double *c = C1 + (((k * 3 + bx) * 4 + qy) * 4 + qz) * 6;
+            const double *ty = &sPd[1][qy][0][0];   // ty[3 * dy + by]
+            if (dz == 0) acc += c[0] * ty[by] + c[1] * ty[3 + by] + c[3] * ty[6 + by];
+            else if (dz == 1) acc += c[2] * ty[by] + c[4] * ty[3 + by];
+            else acc += c[5] * ty[by];
+        }
+        C2[3 * t + dz] = acc;   // [k][bx][by][qz][dz]
+    }
+    __syncthreads();
+    // ---- z^T: R_e[3 b + k] = sum_qz sum_dz C2[k][bx][by][qz][dz] Pz[qz][dz][bz]
+    double acc = 0.0;
+    if (tid < 81) {
+        const int k = tid % 3, b = tid / 3, bx = b / 9, by = (b / 3) % 3, bz = b % 3;
+        const double *c = C2 + ((k * 3 + bx) * 3 + by) * 12;
+#pragma unroll
+        for (int qz = 0; qz < 4; qz++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) acc += c[3 * qz + dz] * sPd[2][qz][dz][bz];
+    }
+    return acc;
+}
+
 // ============================================================================
 // Residual on a separable patch (k_residual_sep, static physics; volume loop
 // of Residual_iga, src/assmcore_iga.c:118-259): the displacement jets of
@@ -1712,52 +1773,104 @@ __global__ __launch_bounds__(256, 4) void k_residual_sep(MeshK M, PhysK P, const
         for (int a = 0; a < 30; a++) sJet[q * 30 + a] = w * r[a];
     }
     __syncthreads();
-    // ---- x^T: C1[k][bx][qy][qz][s6] = sum_qx sum_{a: S6(a) = s6} r[q][k][a] Px[qx][DX(a)][bx]
-    double *C1 = sWork, *C2 = sWork + 864;
-    for (int i = tid; i < 864; i += 256) {   // s6 slowest: the switch below is mostly wave-uniform
-        const int s6 = i / 144, t = i - 144 * s6, qz = t & 3, qy = (t >> 2) & 3, bx = (t >> 4) % 3, k = t / 48;
-        double acc = 0.0;
+    const double v = sep_project<30>(sPd, sJet, sWork);
+    if (tid < 81) Re[(int64_t)e * 81 + tid] = v;
+}
+
+// Transient residual on a separable patch (k_residual_sep_t, gradelasttime):
+// the jets of Um and Ud, the rule's eight nodes per point (one lane per node
+// at Um + s_j Ud, weighted by cr_j and summed over the 8-lane group), the
+// point weight, and the projection of k_residual_sep.  The value slot's inertia r[10 k] = cii (u - u^2) / (2 dt)
+// + rho (u + u^2 - 2 u^1) / dt^2 is linear in the nodal values, so its part of
+// R_e is the element mass matrix Mx x My x Mz applied to the nodal
+// combination (k_tangent_sep_t's 1-D mass matrices).
+__global__ __launch_bounds__(256, 2) void k_residual_sep_t(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+                                                           double *__restrict__ Re, int nel)
+{
+    __shared__ double sPd[3][4][3][3];
+    __shared__ double sG[3][4];
+    __shared__ double sU[27 * 3], sUd[27 * 3], sV[27 * 3];
+    __shared__ double sMass[3][3][3];
+    __shared__ double sWork[3 * (108 + 288)];   // field passes, then x^T (864) and y^T (324)
+    __shared__ double sJet[64 * 60];            // jets [q][h][k][a] of Um and Ud, then the weighted r_q [q][k][a]
+    const int tid = threadIdx.x;
+    const int e = blockIdx.x;
+    const int nz = M.nelem[2], ny = M.nelem[1];
+    const int ed[3] = {e / (ny * nz), (e / nz) % ny, e % nz};
+
+    sep_element<double, 3>(M, Uh, ed, sU, sPd, nullptr, sG, ndof, sUd, sV, P.cii / (2. * P.dt), P.rho / (P.dt * P.dt));
+    __syncthreads();
+    {
+        double J1[10], J2[10];
+        sep_fields(M, ed, sU, sPd, sWork, J1, true);
+        __syncthreads();
+        sep_fields(M, ed, sUd, sPd, sWork, J2, false);
+        const int q = tid >> 2, part = tid & 3;
+        if (part > 0) {
 #pragma unroll
-        for (int qx = 0; qx < 4; qx++) {
-            const double *rq = sJet + (16 * qx + 4 * qy + qz) * 30 + 10 * k;
-            const double *tx = &sPd[0][qx][0][0];   // tx[3 * dx + bx]
-            switch (s6) {
-                case 0: acc += rq[0] * tx[bx] + rq[1] * tx[3 + bx] + rq[4] * tx[6 + bx]; break;
-                case 1: acc += rq[2] * tx[bx] + rq[5] * tx[3 + bx]; break;
-                case 2: acc += rq[3] * tx[bx] + rq[6] * tx[3 + bx]; break;
-                case 3: acc += rq[7] * tx[bx]; break;
-                case 4: acc += rq[8] * tx[bx]; break;
-                default: acc += rq[9] * tx[bx]; break;
+            for (int a = 0; a < 10; a++) {
+                sJet[q * 60 + 10 * (part - 1) + a] = J1[a];
+                sJet[q * 60 + 30 + 10 * (part - 1) + a] = J2[a];
             }
         }
-        C1[6 * t + s6] = acc;   // [k][bx][qy][qz][s6]
-    }
-    __syncthreads();
-    // ---- y^T: C2[k][bx][by][qz][dz] = sum_qy sum_{s6: dz(s6) = dz} C1[..][s6] Py[qy][dy(s6)][by]
-    for (int i = tid; i < 324; i += 256) {   // dz slowest (mostly wave-uniform branches)
-        const int dz = i / 108, t = i - 108 * dz, qz = t & 3, u = t >> 2, by = u % 3, bx = (u / 3) % 3, k = u / 9;
-        double acc = 0.0;
+        if (tid < 27) {
+            const int d = tid / 9, b = (tid / 3) % 3, c = tid % 3;
+            const double dX = M.knot[d][ed[d] + 3] - M.knot[d][ed[d] + 2];
+            double m = 0.0;
 #pragma unroll
-        for (int qy = 0; qy < 4; qy++) {
-            const double *c = C1 + (((k * 3 + bx) * 4 + qy) * 4 + qz) * 6;
-            const double *ty = &sPd[1][qy][0][0];   // ty[3 * dy + by]
-            if (dz == 0) acc += c[0] * ty[by] + c[1] * ty[3 + by] + c[3] * ty[6 + by];
-            else if (dz == 1) acc += c[2] * ty[by] + c[4] * ty[3 + by];
-            else acc += c[5] * ty[by];
+            for (int qq = 0; qq < 4; qq++) m += M.gw[qq] * dX * sG[d][qq] * sPd[d][qq][0][b] * sPd[d][qq][0][c];
+            sMass[d][b][c] = m;
         }
-        C2[3 * t + dz] = acc;   // [k][bx][by][qz][dz]
     }
     __syncthreads();
-    // ---- z^T: R_e[3 b + k] = sum_qz sum_dz C2[k][bx][by][qz][dz] Pz[qz][dz][bz]
+    // ---- rule-weighted residual jets of the stored energy, times the weight:
+    //      lane 8g + j takes rule node j of point g + 32 round, the node sums
+    //      over the 8-lane group one entry at a time
+    {
+        const int j = tid & 7;
+        for (int round = 0; round < 2; round++) {
+            const int q = (tid >> 3) + 32 * round;
+            const double *J = sJet + q * 60;
+            double r[30];
+            {
+                double us[30];
+#pragma unroll
+                for (int a = 0; a < 30; a++) us[a] = J[a] + P.s[j] * J[30 + a];
+                State st;
+                phys_state(us, P, st);
+#pragma unroll
+                for (int a = 0; a < 30; a++) r[a] = 0.0;
+                phys_residual_acc(st, P.cr[j], r);
+            }
+            const double w = (j == 0) ? sep_weight(M, ed, q, sG) : 0.0;
+#pragma unroll
+            for (int a = 0; a < 30; a++) {
+                if (a % 10 == 0) continue;   // value slots: the inertia, added below
+                double v = r[a];
+#pragma unroll
+                for (int o = 1; o < 8; o <<= 1) v += __shfl_xor(v, o, 64);
+                r[a] = v;
+            }
+            if (j == 0) {
+#pragma unroll
+                for (int a = 0; a < 30; a++) sJet[q * 60 + a] = (a % 10 == 0) ? 0.0 : w * r[a];
+            }
+        }
+    }
+    __syncthreads();
+    double v = sep_project<60>(sPd, sJet, sWork);
     if (tid < 81) {
         const int k = tid % 3, b = tid / 3, bx = b / 9, by = (b / 3) % 3, bz = b % 3;
-        const double *c = C2 + ((k * 3 + bx) * 3 + by) * 12;
-        double acc = 0.0;
+        double in = 0.0;
 #pragma unroll
-        for (int qz = 0; qz < 4; qz++)
+        for (int cx = 0; cx < 3; cx++)
 #pragma unroll
-            for (int dz = 0; dz < 3; dz++) acc += c[3 * qz + dz] * sPd[2][qz][dz][bz];
-        Re[(int64_t)e * 81 + tid] = acc;
+            for (int cy = 0; cy < 3; cy++) {
+                const double mxy = sMass[0][bx][cx] * sMass[1][by][cy];
+#pragma unroll
+                for (int cz = 0; cz < 3; cz++) in += mxy * sMass[2][bz][cz] * sV[3 * (9 * cx + 3 * cy + cz) + k];
+            }
+        Re[(int64_t)e * 81 + tid] = v + in;
     }
 }
 
@@ -2161,6 +2274,7 @@ hipError_t launch_residual_elem(const MeshK &M, const PhysK &P, const double *Uh
 {
     dim3 grid(nel), block(256);
     if (P.torder == 0 && M.separable) hipLaunchKernelGGL(k_residual_sep, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
+    else if (P.torder == 2 && M.separable) hipLaunchKernelGGL(k_residual_sep_t, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     else if (P.torder == 0) hipLaunchKernelGGL(k_residual_elem<1>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     else hipLaunchKernelGGL(k_residual_elem<3>, grid, block, 0, s, M, P, Uh, ndof, Re, nel);
     return hipGetLastError();

@@ -1,5 +1,5 @@
 """The sum-factorised residual and tangent of separable patches
-(k_residual_sep, k_tangent_sep; the transient tangent k_tangent_sep_t).
+(k_residual_sep, k_tangent_sep; transient: k_residual_sep_t, k_tangent_sep_t).
 
 A patch is separable when every control coordinate x_d depends on the control
 point's index along d alone and all weights are equal (the test of
@@ -16,8 +16,8 @@ distorted igabeamtest patch, which is not separable and takes the general
 kernel whatever the flag.  The "_t" cases use the transient physics
 (gradelasttime, three history levels of a random state; the tangent's
 rule-weighted rows and inertia term, src/assmcore_iga.c:607-764 with
-phys/gradelasttime.c) on the same patches: the tangent takes the
-sum-factorised kernel, the residual the general one."""
+phys/gradelasttime.c) on the same patches: the tangent and the residual
+take the sum-factorised kernels (k_tangent_sep_t, k_residual_sep_t)."""
 import numpy as np
 import pytest
 
@@ -132,7 +132,7 @@ def test_residual_vs_oracle_and_general(igap, oracle, built, case):
     e_o, e_g = rowfloor_err(R, R_o), rowfloor_err(R, Rg)
     print(f"[{case}] residual: max rel err vs oracle {e_o:.3e}, vs general kernel {e_g:.3e}")
     assert np.array_equal(R, R2)
-    if case.startswith("distorted") or torder:   # the general residual kernel either way
+    if case.startswith("distorted"):   # not separable: the general residual kernel either way
         assert np.array_equal(R, Rg)
     assert e_o < TOL64 and e_g < TOL64
 
