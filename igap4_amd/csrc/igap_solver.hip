@@ -760,9 +760,14 @@ __global__ void k_diag_inv(int64_t n, const int64_t *__restrict__ rp, const int3
 {
     const int64_t st = (int64_t)gridDim.x * blockDim.x;   // grid-stride: launched with grid1
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += st) {
-        double dv = 1.0;
-        for (int64_t j = rp[r]; j < rp[r + 1]; j++)
-            if (col[j] == r) dv = (double)val[j];
+        // the diagonal by bisection (columns sorted; 1 where the row has none)
+        int64_t lo = rp[r], hi = rp[r + 1] - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < r) lo = mid + 1;
+            else hi = mid;
+        }
+        const double dv = (lo < rp[r + 1] && col[lo] == r) ? (double)val[lo] : 1.0;
         dinv[r] = dv != 0.0 ? 1.0 / dv : 1.0;
     }
 }
