@@ -228,7 +228,6 @@ __global__ __launch_bounds__(256, (NH == 1 ? 4 : 2)) void k_residual_elem(MeshK 
                                                        double *__restrict__ Re, int nel)
 {
     constexpr int STR = 4 + 3 * NH;            // X(3) w(1) U(3*NH) per control point
-    constexpr int NF = 4 + 3 * NH;             // fields w, w X (3), w U (3 per level)
     constexpr int WA = gp_fields_per_pass<NH>() * (108 + 288);   // stage-A buffers
     constexpr int GEO_R = 38;                  // wgt, Gi[9], X2[18], W[10]
     constexpr int WG = 64 * GEO_R;             // the Gauss-point geometry (after stage A)
@@ -363,7 +362,6 @@ __global__ __launch_bounds__(256, (NH == 1 ? 1 : 2)) void k_density_elem(MeshK M
                                                       double *__restrict__ Hd, int nel)
 {
     constexpr int STR = 4 + 3 * NH;
-    constexpr int NF = 4 + 3 * NH;
     constexpr int GEO_R = 38;
     constexpr int ND = (NH == 1) ? ND_STATIC : ND_TRANSIENT;
     __shared__ double sT[3][4][3][3];
@@ -1223,8 +1221,6 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
     // jet slot a -> order in z, x-y part s
     constexpr int SDZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
     constexpr int SXY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    (void)tid;
     const int zp = rt / 18, zr = rt - 18 * zp;
     const int yp = rt / 27, yu = rt - 27 * yp;
     const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
@@ -1276,12 +1272,30 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
             }
         }
         if (chunk == 16) break;
-        // ---- rows of D_q and their z contraction F_q: wave wv takes point
-        //      qz = wv of the pencil, lane 9p + r row r + 1 of block p
+        // ---- rows of D_q and their z contraction F_q: the pencil's 216 rows
+        //      sorted by kind, so that a wave mostly takes one branch of the
+        //      row function -- tasks 0..71 the first-gradient rows (point
+        //      t / 18, block (t % 18) / 3, row t % 3 + 1), 72..215 the
+        //      second-gradient rows (point u / 36, block (u % 36) / 6, row
+        //      u % 6 + 4, u = t - 72); the mixed wave (tasks 64..127) is the
+        //      one without y-pass work (rt 192..255)
         {
-            const int qz = wv, q = 4 * chunk + qz;
-            if (lane < 6 * NA) {
-                const int p = lane / NA, ar = lane - p * NA;
+            const int t = (rt + 128) & 255;
+            if (t < 216) {
+                int qz, p, ar;
+                if (t < 72) {
+                    qz = t / 18;
+                    const int r = t - 18 * qz;
+                    p = r / 3;
+                    ar = r - 3 * p;
+                } else {
+                    const int u = t - 72;
+                    qz = u / 36;
+                    const int r = u - 36 * qz;
+                    p = r / 6;
+                    ar = 3 + r - 6 * p;
+                }
+                const int q = 4 * chunk + qz;
                 double row[NA];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
