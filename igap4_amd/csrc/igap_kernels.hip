@@ -1424,7 +1424,7 @@ __device__ __forceinline__ void sep_write(const MeshK &M, const Scatter &S, T *_
 }
 
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+__global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                         T *__restrict__ Ke, int nel, Scatter S)
 {
     constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
@@ -1487,13 +1487,10 @@ __global__ __launch_bounds__(256, 2) void k_tangent_sep(MeshK M, PhysK P, const 
 
     const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
     T K[27];
-    // FP64: the branch-light rows; FP32 keeps the select form, whose lower
-    // register count holds three waves per SIMD
+    // the branch-light rows (FP32: capped at 168 VGPRs for three workgroups
+    // per CU, 5 of them spilled -- faster than the select form of c_row)
     auto rows = [&](int q, int p, int ar, double *row) {
-        if constexpr (sizeof(T) == 8)
-            c_row_onehot<true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], row);
-        else
-            c_row<1, true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], 1.0, row);
+        c_row_onehot<true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], row);
     };
     sep_pencils<T>(rt, sP, sWgt, sF, sE, rows, K);
     sep_write<T, DIRECT>(M, S, Ke, e, ex, ey, ez, rt, K, sWork);
