@@ -770,6 +770,110 @@ __device__ __forceinline__ void c_row_momc(const TS *ms, const PhysK &P, int k, 
     }
 }
 
+// c_row_momc with c_row_onehot's straight-line columns: the lane's slot
+// indices as one-hot vectors, the products of two moment-weighted fields as
+// dot2 pairs, the second-gradient block of a second-gradient row from the
+// 3x3 tensor T[J][K2] = e_K[K2] c(I,J) d1[J] + e_I[K2] c(K,J) d2[J] with
+// d1[J] = (F_kI . F_lJ), d2[J] = [I != K] (F_kK . F_lJ).
+template <typename TS>
+__device__ __forceinline__ void c_row_momc_onehot(const TS *ms, const PhysK &P, int k, int a, int l, double *row)
+{
+    auto F1 = [&](int i) { return (double)ms[MC_F1 + i]; };
+    auto F2 = [&](int i) { return (double)ms[MC_F2 + i]; };
+    auto H1 = [&](int kk, int I, int K) { return (double)ms[MC_H1 + 6 * kk + sidx(I, K) - 4]; };
+    auto H2 = [&](int kk, int I, int K) { return (double)ms[MC_H2 + 6 * kk + sidx(I, K) - 4]; };
+    auto sym = [&](int base, int I, int K) { return (double)ms[base + sidx(I, K) - 4]; };
+    const double g2 = 2 * P.le * P.le;
+    const double dd = (k == l) ? 1.0 : 0.0;
+    double Fl1[3], Fl2[3], Fk1[3], Fk2[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        Fl1[m] = F1(3 * l + m);
+        Fl2[m] = F2(3 * l + m);
+        Fk1[m] = F1(3 * k + m);
+        Fk2[m] = F2(3 * k + m);
+    }
+    constexpr double C1 = 2.0 / 3.0, C0 = -1.0 / 3.0;
+    if (a <= 3) {
+        const int L = a - 1;
+        double eL[3], cL[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) { eL[m] = (m == L) ? 1.0 : 0.0; cL[m] = (m == L) ? C1 : C0; }
+        const double FkL1 = F1(3 * k + L), FkL2 = F2(3 * k + L), FlL1 = F1(3 * l + L), FlL2 = F2(3 * l + L);
+        double hk1[3], hk2[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) { hk1[c] = H1(k, L, c); hk2[c] = H2(k, L, c); }
+        const double full = dot2(Fk1[0], Fk2[0], Fl1[0], Fl2[0]) + dot2(Fk1[1], Fk2[1], Fl1[1], Fl2[1]) + dot2(Fk1[2], Fk2[2], Fl1[2], Fl2[2]);
+        const double eLL = full - dot2(FkL1, FkL2, FlL1, FlL2);
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            double v = sym(MC_N11, L, M) * FkL1 * Fl1[M] + sym(MC_N12, L, M) * (FkL1 * Fl2[M] + FkL2 * Fl1[M])
+                     + sym(MC_N22, L, M) * FkL2 * Fl2[M];
+            const double e = eL[M] * eLL + (1.0 - eL[M]) * dot2(Fk1[M], Fk2[M], FlL1, FlL2);
+            v += 0.5 * P.e345 * e;
+            double hh = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) hh += dot2(hk1[c], hk2[c], H1(l, M, c), H2(l, M, c));
+            v += g2 * cL[M] * hh;
+            v += dd * sym(MC_S, L, M);
+            row[M] = v;
+        }
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
+            const int I = PI[pp], K = PK[pp];
+            double v = g2 * cL[I] * dot2(hk1[K], hk2[K], Fl1[I], Fl2[I]);
+            double gv = eL[I] * (double)ms[MC_G + 3 * I + K];
+            if (I != K) {
+                v += g2 * cL[K] * dot2(hk1[I], hk2[I], Fl1[K], Fl2[K]);
+                gv += eL[K] * (double)ms[MC_G + 3 * K + I];
+            }
+            row[3 + pp] = v + dd * gv;
+        }
+    } else {
+        const int s = a - 4;
+        const int I = (s < 3) ? 0 : (s < 5 ? 1 : 2);
+        const int K = (s < 3) ? s : (s < 5 ? s - 2 : 2);
+        const double nIK = (I != K) ? 1.0 : 0.0;
+        double eI[3], eK[3], cI[3], cK[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            eI[m] = (m == I) ? 1.0 : 0.0;
+            eK[m] = (m == K) ? 1.0 : 0.0;
+            cI[m] = (m == I) ? C1 : C0;
+            cK[m] = (m == K) ? C1 : C0;
+        }
+        const double FkI1 = F1(3 * k + I), FkI2 = F2(3 * k + I), FkK1 = F1(3 * k + K), FkK2 = F2(3 * k + K);
+        const double GIK = (double)ms[MC_G + 3 * I + K], GKI = (double)ms[MC_G + 3 * K + I];
+#pragma unroll
+        for (int M = 0; M < 3; M++) {
+            double v = g2 * cI[M] * dot2(H1(l, M, K), H2(l, M, K), FkI1, FkI2);
+            v += nIK * (g2 * cK[M] * dot2(H1(l, M, I), H2(l, M, I), FkK1, FkK2));
+            v += dd * (eI[M] * GIK + nIK * eK[M] * GKI);
+            row[M] = v;
+        }
+        double d1[3], d2[3];
+#pragma unroll
+        for (int J = 0; J < 3; J++) {
+            d1[J] = dot2(FkI1, FkI2, Fl1[J], Fl2[J]);
+            d2[J] = nIK * dot2(FkK1, FkK2, Fl1[J], Fl2[J]);
+        }
+        double T[3][3];
+#pragma unroll
+        for (int J = 0; J < 3; J++)
+#pragma unroll
+            for (int K2 = 0; K2 < 3; K2++) T[J][K2] = eK[K2] * cI[J] * d1[J] + eI[K2] * cK[J] * d2[J];
+#pragma unroll
+        for (int pp = 0; pp < 6; pp++) {
+            constexpr int PJ[6] = {0, 0, 0, 1, 1, 2}, PK2[6] = {0, 1, 2, 1, 2, 2};
+            const int J = PJ[pp], K2 = PK2[pp];
+            double v = T[J][K2];
+            if (J != K2) v += T[K2][J];
+            row[3 + pp] = g2 * v;
+        }
+    }
+}
+
 __constant__ const int kPairI[6] = {0, 0, 0, 1, 1, 2};
 __constant__ const int kPairK[6] = {0, 1, 2, 1, 2, 2};
 

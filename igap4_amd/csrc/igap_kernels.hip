@@ -1659,7 +1659,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t
 
     const int rt = (tid + 64 * ((e ^ (e >> 3)) & 3)) & 255;
     T K[27];
-    auto rows = [&](int q, int p, int ar, double *row) { c_row_momc<T>(sMom + q * MWC, P, kBlkI[p], ar + 1, kBlkJ[p], row); };
+    auto rows = [&](int q, int p, int ar, double *row) { c_row_momc_onehot<T>(sMom + q * MWC, P, kBlkI[p], ar + 1, kBlkJ[p], row); };
     sep_pencils<T>(rt, sP, sWgt, sF, sE, rows, K);
     // inertia on the diagonal blocks
     if (rt < 162) {
