@@ -695,83 +695,8 @@ __device__ __forceinline__ void c_row_mom(const double *ms, const PhysK &P, int 
 constexpr int MC_F1 = 0, MC_F2 = 9, MC_H1 = 18, MC_H2 = 36, MC_S = 54, MC_G = 60, MC_N11 = 69, MC_N12 = 75, MC_N22 = 81, MWC = 87;
 
 // c_row_mom (slots 1..9, row[b - 1] = C[(k,a),(l,b)]) from the compact moment
-// state, stored in TS
-template <typename TS>
-__device__ __forceinline__ void c_row_momc(const TS *ms, const PhysK &P, int k, int a, int l, double *row)
-{
-    auto F1 = [&](int i) { return (double)ms[MC_F1 + i]; };
-    auto F2 = [&](int i) { return (double)ms[MC_F2 + i]; };
-    auto H1 = [&](int kk, int I, int K) { return (double)ms[MC_H1 + 6 * kk + sidx(I, K) - 4]; };
-    auto H2 = [&](int kk, int I, int K) { return (double)ms[MC_H2 + 6 * kk + sidx(I, K) - 4]; };
-    auto sym = [&](int base, int I, int K) { return (double)ms[base + sidx(I, K) - 4]; };
-    const double g2 = 2 * P.le * P.le;
-    const bool dia = (k == l);
-    if (a <= 3) {
-        const int L = a - 1;
-        const double FkL1 = F1(3 * k + L), FkL2 = F2(3 * k + L);
-#pragma unroll
-        for (int M = 0; M < 3; M++) {
-            const double Fl1 = F1(3 * l + M), Fl2 = F2(3 * l + M);
-            double v = sym(MC_N11, L, M) * FkL1 * Fl1 + sym(MC_N12, L, M) * (FkL1 * Fl2 + FkL2 * Fl1) + sym(MC_N22, L, M) * FkL2 * Fl2;
-            double e = 0.0;
-            if (M == L) {
-#pragma unroll
-                for (int J = 0; J < 3; J++)
-                    if (J != L) e += dot2(F1(3 * k + J), F2(3 * k + J), F1(3 * l + J), F2(3 * l + J));
-            } else {
-                e = dot2(F1(3 * k + M), F2(3 * k + M), F1(3 * l + L), F2(3 * l + L));
-            }
-            v += 0.5 * P.e345 * e;
-            double hh = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; c++) hh += dot2(H1(k, L, c), H2(k, L, c), H1(l, M, c), H2(l, M, c));
-            v += g2 * ((M == L) ? 2.0 / 3.0 : -1.0 / 3.0) * hh;
-            if (dia) v += sym(MC_S, L, M);
-            row[M] = v;
-        }
-#pragma unroll
-        for (int pp = 0; pp < 6; pp++) {
-            constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
-            const int I = PI[pp], K = PK[pp];
-            double v = g2 * ((L == I) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(H1(k, L, K), H2(k, L, K), F1(3 * l + I), F2(3 * l + I));
-            if (I != K) v += g2 * ((L == K) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(H1(k, L, I), H2(k, L, I), F1(3 * l + K), F2(3 * l + K));
-            if (dia) {
-                if (L == I) v += (double)ms[MC_G + 3 * I + K];
-                if (I != K && L == K) v += (double)ms[MC_G + 3 * K + I];
-            }
-            row[3 + pp] = v;
-        }
-    } else {
-        constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
-        const int I = PI[a - 4], K = PK[a - 4];
-        const bool off = (I != K);
-        const double FkI1 = F1(3 * k + I), FkI2 = F2(3 * k + I), FkK1 = F1(3 * k + K), FkK2 = F2(3 * k + K);
-#pragma unroll
-        for (int M = 0; M < 3; M++) {
-            double v = g2 * ((M == I) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(H1(l, M, K), H2(l, M, K), FkI1, FkI2);
-            if (off) v += g2 * ((M == K) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(H1(l, M, I), H2(l, M, I), FkK1, FkK2);
-            if (dia) {
-                if (M == I) v += (double)ms[MC_G + 3 * I + K];
-                if (off && M == K) v += (double)ms[MC_G + 3 * K + I];
-            }
-            row[M] = v;
-        }
-#pragma unroll
-        for (int pp = 0; pp < 6; pp++) {
-            constexpr int PJ[6] = {0, 0, 0, 1, 1, 2}, PK2[6] = {0, 1, 2, 1, 2, 2};
-            const int J = PJ[pp], K2 = PK2[pp];
-            double v = 0.0;
-            if (K == K2) v += ((I == J) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkI1, FkI2, F1(3 * l + J), F2(3 * l + J));
-            if (J != K2 && K == J) v += ((I == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkI1, FkI2, F1(3 * l + K2), F2(3 * l + K2));
-            if (off && I == K2) v += ((K == J) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkK1, FkK2, F1(3 * l + J), F2(3 * l + J));
-            if (off && J != K2 && I == J) v += ((K == K2) ? 2.0 / 3.0 : -1.0 / 3.0) * dot2(FkK1, FkK2, F1(3 * l + K2), F2(3 * l + K2));
-            row[3 + pp] = g2 * v;
-        }
-    }
-}
-
-// c_row_momc with c_row_onehot's straight-line columns: the lane's slot
-// indices as one-hot vectors, the products of two moment-weighted fields as
+// state, stored in TS, with c_row_onehot's straight-line columns: the lane's
+// slot indices as one-hot vectors, the products of two moment-weighted fields as
 // dot2 pairs, the second-gradient block of a second-gradient row from the
 // 3x3 tensor T[J][K2] = e_K[K2] c(I,J) d1[J] + e_I[K2] c(K,J) d2[J] with
 // d1[J] = (F_kI . F_lJ), d2[J] = [I != K] (F_kK . F_lJ).
