@@ -1746,7 +1746,11 @@ __device__ __forceinline__ double sep_project(const double (*sPd)[4][3][3], cons
 // with R_q the tensor product of the physical tables, as the transposed sum
 // factorisation x -> y -> z; no quotient or chain rule to undo.
 // ============================================================================
-__global__ __launch_bounds__(256, 4) void k_residual_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
+// Six workgroups per CU (26 KB of LDS each, 80 VGPRs; the one-wave pointwise
+// stage spills 32 of them): 7.6 ms at 256x64x64 against 8.3 at five and 9.9
+// at four, where nothing spills -- the barriers of the field and projection
+// passes are what the extra workgroups hide.
+__global__ __launch_bounds__(256, 6) void k_residual_sep(MeshK M, PhysK P, const double *__restrict__ Uh, int64_t ndof,
                                                          double *__restrict__ Re, int nel)
 {
     __shared__ double sPd[3][4][3][3];
