@@ -1202,6 +1202,10 @@ __device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int 
     return M.gw[qd[0]] * M.gw[qd[1]] * M.gw[qd[2]] * dX[0] * dX[1] * dX[2] * (sG[0][qd[0]] * sG[1][qd[1]] * sG[2][qd[2]]);
 }
 
+// Row stride of the pencil's F buffer (18 columns): 19, so that the lanes of a
+// row store, each writing its own row, spread over the LDS banks
+constexpr int SEP_FS = 19;
+
 // The pencil loop of the separable tangents, after stage A and a barrier
 // (256 threads).  rows(q, p, ar, row) gives row[b - 1] = C_q[(k, ar + 1),
 // (l, b)], b = 1..9, of block p = (k, l) at Gauss point q (unweighted; row
@@ -1312,7 +1316,7 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
 #pragma unroll
                     for (int cz = 0; cz < 3; cz++) f[3 * SXY[a] + cz] += v * pz[3 * SDZ[a] + cz];
                 }
-                T *o = sF + ((qz * 6 + p) * NA + ar) * FR;
+                T *o = sF + ((qz * 6 + p) * NA + ar) * SEP_FS;
 #pragma unroll
                 for (int i = 0; i < FR; i++) o[i] = f[i];
             }
@@ -1325,12 +1329,12 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
             for (int i = 0; i < 18; i++) e1[i] = (T)0;
 #pragma unroll
             for (int qz = 0; qz < 4; qz++) {
-                const T *Fp = sF + (qz * 6 + zp) * NA * FR + zr;
+                const T *Fp = sF + (qz * 6 + zp) * NA * SEP_FS + zr;
                 const T *pz = &sP[2][qz][0][0];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) {
                     const int a = bb + 1;
-                    const T fv = Fp[FR * bb];
+                    const T fv = Fp[SEP_FS * bb];
 #pragma unroll
                     for (int bz = 0; bz < 3; bz++) e1[3 * SXY[a] + bz] += pz[3 * SDZ[a] + bz] * fv;
                 }
@@ -1428,13 +1432,17 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(M
                                                         T *__restrict__ Ke, int nel, Scatter S)
 {
     constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
-    constexpr int WORK_F = 4 * 6 * 9 * 18 * (int)sizeof(T);        // F of the pencil's four points (9 rows x 18 columns)
+    constexpr int WORK_F = 4 * 6 * 9 * SEP_FS * (int)sizeof(T);    // F of the pencil's four points (9 rows x 18 columns)
     __shared__ double sPd[3][4][3][3];          // 1-D tables of physical derivatives [d][q_d][order][i]
     __shared__ T sP[3][4][3][3];                // the same in the contraction's precision
     __shared__ double sG[3][4];                 // dx_d / dxi_d at the 1-D Gauss points
     __shared__ double sU[27 * 3];
     __shared__ double sWgt[64];
-    __shared__ double sSt[64 * STWC];          // strain-measure states, compact layout
+    // strain-measure states, compact layout, at a stride of 50 doubles: a wave's
+    // rows cover the pencil's four points, and 48 would put points two apart
+    // on the same LDS banks
+    constexpr int SSTR = 50;
+    __shared__ double sSt[64 * SSTR];
     __shared__ T sE[6 * 18 * 18];               // E1 of the current pencil
     __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
     T *sF = reinterpret_cast<T *>(sWork);
@@ -1470,7 +1478,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(M
         if (part == 0) {
             State st;
             phys_state(u, P, st);
-            double *sp = sSt + q * STWC;
+            double *sp = sSt + q * SSTR;
             constexpr int PR[6] = {0, 1, 2, 4, 5, 8};   // index pairs I <= J in slot order
 #pragma unroll
             for (int i = 0; i < 9; i++) { sp[i] = st.F[i]; sp[STC_G + i] = st.G[i]; }
@@ -1490,7 +1498,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(M
     // the branch-light rows (FP32: capped at 168 VGPRs for three workgroups
     // per CU, 5 of them spilled -- faster than the select form of c_row)
     auto rows = [&](int q, int p, int ar, double *row) {
-        c_row_onehot<true>(sSt + q * STWC, P, kBlkI[p], ar + 1, kBlkJ[p], row);
+        c_row_onehot<true>(sSt + q * SSTR, P, kBlkI[p], ar + 1, kBlkJ[p], row);
     };
     sep_pencils<T>(rt, sP, sWgt, sF, sE, rows, K);
     sep_write<T, DIRECT>(M, S, Ke, e, ex, ey, ez, rt, K, sWork);
@@ -1513,7 +1521,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t
                                                                                int64_t ndof, T *__restrict__ Ke, int nel, Scatter S)
 {
     constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of 3 components
-    constexpr int WORK_F = 4 * 6 * 9 * 18 * (int)sizeof(T);        // F of the pencil's four points
+    constexpr int WORK_F = 4 * 6 * 9 * SEP_FS * (int)sizeof(T);    // F of the pencil's four points
     constexpr int WORK_J = 64 * 54 * (int)sizeof(T);               // jets of Um and Ud, slots 1..9
     constexpr int WORK = WORK_A > WORK_F ? (WORK_A > WORK_J ? WORK_A : WORK_J) : (WORK_F > WORK_J ? WORK_F : WORK_J);
     __shared__ double sPd[3][4][3][3];
