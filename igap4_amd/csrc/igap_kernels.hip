@@ -900,9 +900,38 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
             }
         }
         if (NH != 1) __syncthreads();   // node states are read by the whole wave below
-        // ---- B2: D_q = w_q C_q on the blocks i <= j for Gauss point q0 + wv;
-        //      lane 9p + a' (10p + a' transient) builds row a' of block p
-        {
+        // ---- B2: D_q = w_q C_q on the blocks i <= j for the chunk's points.
+        //      Static: the chunk's 216 rows sorted by kind across the waves
+        //      (tasks 0..71 first-gradient rows: point t / 18, block
+        //      (t % 18) / 3, row t % 3 + 1; 72..215 second-gradient rows:
+        //      point u / 36, block (u % 36) / 6, row u % 6 + 4, u = t - 72),
+        //      so that three waves run one branch of the one-hot rows.
+        //      Transient: wave wv takes point q0 + wv, lane 10p + a' row a'
+        //      of block p.
+        if (NH == 1) {
+            if (tid < 216) {
+                int ql, p, ar;
+                if (tid < 72) {
+                    ql = tid / 18;
+                    const int r = tid - 18 * ql;
+                    p = r / 3;
+                    ar = r - 3 * p;
+                } else {
+                    const int u = tid - 72;
+                    ql = u / 36;
+                    const int r = u - 36 * ql;
+                    p = r / 6;
+                    ar = 3 + r - 6 * p;
+                }
+                const int q = q0 + ql;
+                const double wgt = sGeo[q * GEOK];
+                double row[NA];
+                c_row_onehot<false>(sSt + q * STW, P, kBlkI[p], ar + A0, kBlkJ[p], row);
+                T *D = sD[ql] + p * NN + ar * RS;
+#pragma unroll
+                for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
+            }
+        } else {
             const int ql = wv, q = q0 + ql;
             if (lane < 6 * NA) {
                 const int p = lane / NA, ar = lane - p * NA, k = kBlkI[p], l = kBlkJ[p];
@@ -910,9 +939,8 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
                 double row[NA];
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
-                if (NH == 1) c_row<A0>(sSt + q * STW, P, k, ar + A0, l, 1.0, row);
-                else c_row_mom<A0>(sSt + ql * MW, P, k, ar + A0, l, row);
-                if (NH != 1 && ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
+                c_row_mom<A0>(sSt + ql * MW, P, k, ar + A0, l, row);
+                if (ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
                 T *D = sD[ql] + p * NN + ar * RS;
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
