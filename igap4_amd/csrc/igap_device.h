@@ -700,14 +700,19 @@ constexpr int MC_F1 = 0, MC_F2 = 9, MC_H1 = 18, MC_H2 = 36, MC_S = 54, MC_G = 60
 // dot2 pairs, the second-gradient block of a second-gradient row from the
 // 3x3 tensor T[J][K2] = e_K[K2] c(I,J) d1[J] + e_I[K2] c(K,J) d2[J] with
 // d1[J] = (F_kI . F_lJ), d2[J] = [I != K] (F_kK . F_lJ).
-template <typename TS>
+template <typename TS, bool CMP = true>
 __device__ __forceinline__ void c_row_momc_onehot(const TS *ms, const PhysK &P, int k, int a, int l, double *row)
 {
-    auto F1 = [&](int i) { return (double)ms[MC_F1 + i]; };
-    auto F2 = [&](int i) { return (double)ms[MC_F2 + i]; };
-    auto H1 = [&](int kk, int I, int K) { return (double)ms[MC_H1 + 6 * kk + sidx(I, K) - 4]; };
-    auto H2 = [&](int kk, int I, int K) { return (double)ms[MC_H2 + 6 * kk + sidx(I, K) - 4]; };
-    auto sym = [&](int base, int I, int K) { return (double)ms[base + sidx(I, K) - 4]; };
+    // CMP: the compact layout (MC_*); otherwise the full one (M*) of k_tangent_elem
+    constexpr int OF1 = CMP ? MC_F1 : MF1, OF2 = CMP ? MC_F2 : MF2, OH1 = CMP ? MC_H1 : MH1, OH2 = CMP ? MC_H2 : MH2;
+    constexpr int OS = CMP ? MC_S : MS, OG = CMP ? MC_G : MG;
+    constexpr int ON11 = CMP ? MC_N11 : MN11, ON12 = CMP ? MC_N12 : MN12, ON22 = CMP ? MC_N22 : MN22;
+    auto F1 = [&](int i) { return (double)ms[OF1 + i]; };
+    auto F2 = [&](int i) { return (double)ms[OF2 + i]; };
+    auto hidx = [&](int kk, int I, int K) { return CMP ? 6 * kk + sidx(I, K) - 4 : 9 * kk + 3 * I + K; };
+    auto H1 = [&](int kk, int I, int K) { return (double)ms[OH1 + hidx(kk, I, K)]; };
+    auto H2 = [&](int kk, int I, int K) { return (double)ms[OH2 + hidx(kk, I, K)]; };
+    auto sym = [&](int base, int I, int K) { return (double)ms[base + (CMP ? sidx(I, K) - 4 : 3 * I + K)]; };
     const double g2 = 2 * P.le * P.le;
     const double dd = (k == l) ? 1.0 : 0.0;
     double Fl1[3], Fl2[3], Fk1[3], Fk2[3];
@@ -732,15 +737,15 @@ __device__ __forceinline__ void c_row_momc_onehot(const TS *ms, const PhysK &P, 
         const double eLL = full - dot2(FkL1, FkL2, FlL1, FlL2);
 #pragma unroll
         for (int M = 0; M < 3; M++) {
-            double v = sym(MC_N11, L, M) * FkL1 * Fl1[M] + sym(MC_N12, L, M) * (FkL1 * Fl2[M] + FkL2 * Fl1[M])
-                     + sym(MC_N22, L, M) * FkL2 * Fl2[M];
+            double v = sym(ON11, L, M) * FkL1 * Fl1[M] + sym(ON12, L, M) * (FkL1 * Fl2[M] + FkL2 * Fl1[M])
+                     + sym(ON22, L, M) * FkL2 * Fl2[M];
             const double e = eL[M] * eLL + (1.0 - eL[M]) * dot2(Fk1[M], Fk2[M], FlL1, FlL2);
             v += 0.5 * P.e345 * e;
             double hh = 0.0;
 #pragma unroll
             for (int c = 0; c < 3; c++) hh += dot2(hk1[c], hk2[c], H1(l, M, c), H2(l, M, c));
             v += g2 * cL[M] * hh;
-            v += dd * sym(MC_S, L, M);
+            v += dd * sym(OS, L, M);
             row[M] = v;
         }
 #pragma unroll
@@ -748,10 +753,10 @@ __device__ __forceinline__ void c_row_momc_onehot(const TS *ms, const PhysK &P, 
             constexpr int PI[6] = {0, 0, 0, 1, 1, 2}, PK[6] = {0, 1, 2, 1, 2, 2};
             const int I = PI[pp], K = PK[pp];
             double v = g2 * cL[I] * dot2(hk1[K], hk2[K], Fl1[I], Fl2[I]);
-            double gv = eL[I] * (double)ms[MC_G + 3 * I + K];
+            double gv = eL[I] * (double)ms[OG + 3 * I + K];
             if (I != K) {
                 v += g2 * cL[K] * dot2(hk1[I], hk2[I], Fl1[K], Fl2[K]);
-                gv += eL[K] * (double)ms[MC_G + 3 * K + I];
+                gv += eL[K] * (double)ms[OG + 3 * K + I];
             }
             row[3 + pp] = v + dd * gv;
         }
@@ -769,7 +774,7 @@ __device__ __forceinline__ void c_row_momc_onehot(const TS *ms, const PhysK &P, 
             cK[m] = (m == K) ? C1 : C0;
         }
         const double FkI1 = F1(3 * k + I), FkI2 = F2(3 * k + I), FkK1 = F1(3 * k + K), FkK2 = F2(3 * k + K);
-        const double GIK = (double)ms[MC_G + 3 * I + K], GKI = (double)ms[MC_G + 3 * K + I];
+        const double GIK = (double)ms[OG + 3 * I + K], GKI = (double)ms[OG + 3 * K + I];
 #pragma unroll
         for (int M = 0; M < 3; M++) {
             double v = g2 * cI[M] * dot2(H1(l, M, K), H2(l, M, K), FkI1, FkI2);

@@ -906,8 +906,7 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
         //      (t % 18) / 3, row t % 3 + 1; 72..215 second-gradient rows:
         //      point u / 36, block (u % 36) / 6, row u % 6 + 4, u = t - 72),
         //      so that three waves run one branch of the one-hot rows.
-        //      Transient: wave wv takes point q0 + wv, lane 10p + a' row a'
-        //      of block p.
+        //      Transient: likewise with the value rows first.
         if (NH == 1) {
             if (tid < 216) {
                 int ql, p, ar;
@@ -931,20 +930,42 @@ __global__ __launch_bounds__(256, 2) void k_tangent_elem(MeshK M, PhysK P, const
 #pragma unroll
                 for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
             }
-        } else {
-            const int ql = wv, q = q0 + ql;
-            if (lane < 6 * NA) {
-                const int p = lane / NA, ar = lane - p * NA, k = kBlkI[p], l = kBlkJ[p];
-                const double wgt = sGeo[q * GEOK];
-                double row[NA];
-#pragma unroll
-                for (int bb = 0; bb < NA; bb++) row[bb] = 0.0;
-                c_row_mom<A0>(sSt + ql * MW, P, k, ar + A0, l, row);
-                if (ar == 0 && k == l) row[0] += P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
-                T *D = sD[ql] + p * NN + ar * RS;
-#pragma unroll
-                for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
+        } else if (tid < 240) {
+            // the chunk's 240 rows sorted by kind: 0..23 value rows (the
+            // inertia alone; point t / 6, block t % 6), 24..95 first-gradient
+            // rows, 96..239 second-gradient rows
+            int ql, p, ar;
+            if (tid < 24) {
+                ql = tid / 6;
+                p = tid - 6 * ql;
+                ar = 0;
+            } else if (tid < 96) {
+                const int u = tid - 24;
+                ql = u / 18;
+                const int r = u - 18 * ql;
+                p = r / 3;
+                ar = 1 + r - 3 * p;
+            } else {
+                const int u = tid - 96;
+                ql = u / 36;
+                const int r = u - 36 * ql;
+                p = r / 6;
+                ar = 4 + r - 6 * p;
             }
+            const int q = q0 + ql, k = kBlkI[p], l = kBlkJ[p];
+            const double wgt = sGeo[q * GEOK];
+            double row[NA];
+            row[0] = 0.0;   // value column: the inertia alone
+            if (ar == 0) {
+#pragma unroll
+                for (int bb = 1; bb < NA; bb++) row[bb] = 0.0;
+                if (k == l) row[0] = P.cii / (2. * P.dt) + P.rho / (P.dt * P.dt);
+            } else {
+                c_row_momc_onehot<double, false>(sSt + ql * MW, P, k, ar, l, row + 1);
+            }
+            T *D = sD[ql] + p * NN + ar * RS;
+#pragma unroll
+            for (int bb = 0; bb < NA; bb++) D[bb] = (T)(wgt * row[bb]);
         }
         __syncthreads();
         // ---- B3: matrix-core contraction over (q, a) of this chunk.  The
