@@ -1255,12 +1255,70 @@ __device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int 
 // row store, each writing its own row, spread over the LDS banks
 constexpr int SEP_FS = 19;
 
+// jet slot a = 1..9 of the pointwise rows -> order in z, x-y part s = (dx, dy)
+constexpr int SEP_DZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
+constexpr int SEP_XY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
+
+// z pass of one E1 column (block p, column (s', cz)) = task zt < 108 of the
+// pencil: E1[p][(s,bz)][(s',cz)] = sum_qz sum_{a: s(a) = s} Pz[qz][dz(a)][bz]
+// F_qz[p][a][(s',cz)].  HALVES: only the x-y parts s of half h ({0, 1, 3}:
+// five jet slots; {2, 4, 5}: four; h uniform over the wave); otherwise all
+// six.  Every entry takes its terms in the same order either way.
+template <typename T, bool HALVES>
+__device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T *sE, int zt, int h)
+{
+    constexpr int NA = 9, FR = 18;
+    constexpr int ZH[6] = {0, 0, 1, 0, 1, 1};              // half of s
+    constexpr int ZS[6] = {0, 1, 0, 2, 1, 2};              // place of s in its half
+    constexpr int ZI[2][3] = {{0, 1, 3}, {2, 4, 5}};       // s at each place
+    constexpr int NS = HALVES ? 3 : 6;
+    const int zp = zt / 18, zr = zt - 18 * zp;
+    T e1[3 * NS];   // [s][bz]
+#pragma unroll
+    for (int i = 0; i < 3 * NS; i++) e1[i] = (T)0;
+#pragma unroll
+    for (int qz = 0; qz < 4; qz++) {
+        const T *Fp = sF + (qz * 6 + zp) * NA * SEP_FS + zr;
+        const T *pz = &sP[2][qz][0][0];
+        if (!HALVES) {
+#pragma unroll
+            for (int bb = 0; bb < NA; bb++) {
+                const int a = bb + 1;
+                const T fv = Fp[SEP_FS * bb];
+#pragma unroll
+                for (int bz = 0; bz < 3; bz++) e1[3 * SEP_XY[a] + bz] += pz[3 * SEP_DZ[a] + bz] * fv;
+            }
+        } else if (h == 0) {
+#pragma unroll
+            for (int bb = 0; bb < NA; bb++) {
+                const int a = bb + 1;
+                if (ZH[SEP_XY[a]] != 0) continue;
+                const T fv = Fp[SEP_FS * bb];
+#pragma unroll
+                for (int bz = 0; bz < 3; bz++) e1[3 * ZS[SEP_XY[a]] + bz] += pz[3 * SEP_DZ[a] + bz] * fv;
+            }
+        } else {
+#pragma unroll
+            for (int bb = 0; bb < NA; bb++) {
+                const int a = bb + 1;
+                if (ZH[SEP_XY[a]] != 1) continue;
+                const T fv = Fp[SEP_FS * bb];
+#pragma unroll
+                for (int bz = 0; bz < 3; bz++) e1[3 * ZS[SEP_XY[a]] + bz] += pz[3 * SEP_DZ[a] + bz] * fv;
+            }
+        }
+    }
+    T *o = sE + zp * 18 * FR + zr;
+#pragma unroll
+    for (int i = 0; i < 3 * NS; i++) o[FR * (HALVES ? 3 * ZI[h][i / 3] + i % 3 : i)] = e1[i];
+}
+
 // The pencil loop of the separable tangents, after stage A and a barrier
 // (256 threads).  rows(q, p, ar, row) gives row[b - 1] = C_q[(k, ar + 1),
 // (l, b)], b = 1..9, of block p = (k, l) at Gauss point q (unweighted; row
 // arrives zeroed); sWgt holds the point weights, sF and sE the F and E1
-// buffers.  The z pass runs on 108 threads (block p, column (s', cz) of E1),
-// the y/x pass on 162 (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]),
+// buffers.  The z pass runs on 108 threads (block p, column (s', cz) of E1;
+// FP64: 216, a column per half of the x-y parts), the y/x pass on 162 (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]),
 // numbered from the rotated thread index rt so that the workgroups sharing a
 // CU put them on different SIMDs.  The y pass of pencil c runs beside the
 // pointwise stage of pencil c + 1 (F and E1 have buffers of their own), so a
@@ -1271,10 +1329,6 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
 {
     constexpr int NA = 9;    // jet slots 1..9 (no value slot in the pointwise rows)
     constexpr int FR = 18;   // (s', cz) columns of F and E1
-    // jet slot a -> order in z, x-y part s
-    constexpr int SDZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
-    constexpr int SXY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
-    const int zp = rt / 18, zr = rt - 18 * zp;
     const int yp = rt / 27, yu = rt - 27 * yp;
     const int ybz = yu / 9, ycz = (yu / 3) % 3, yby = yu % 3;
     constexpr int XYX[6] = {0, 1, 0, 2, 1, 0}, XYY[6] = {0, 0, 1, 0, 1, 2};
@@ -1363,7 +1417,7 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
                     const int a = bb + 1;
                     const T v = (T)(wgt * row[bb]);
 #pragma unroll
-                    for (int cz = 0; cz < 3; cz++) f[3 * SXY[a] + cz] += v * pz[3 * SDZ[a] + cz];
+                    for (int cz = 0; cz < 3; cz++) f[3 * SEP_XY[a] + cz] += v * pz[3 * SEP_DZ[a] + cz];
                 }
                 T *o = sF + ((qz * 6 + p) * NA + ar) * SEP_FS;
 #pragma unroll
@@ -1371,26 +1425,15 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
             }
         }
         __syncthreads();
-        // ---- z pass over the pencil's four points -> E1[p][(s,bz)][(s',cz)]
-        if (rt < 108) {
-            T e1[18];   // [s][bz]
-#pragma unroll
-            for (int i = 0; i < 18; i++) e1[i] = (T)0;
-#pragma unroll
-            for (int qz = 0; qz < 4; qz++) {
-                const T *Fp = sF + (qz * 6 + zp) * NA * SEP_FS + zr;
-                const T *pz = &sP[2][qz][0][0];
-#pragma unroll
-                for (int bb = 0; bb < NA; bb++) {
-                    const int a = bb + 1;
-                    const T fv = Fp[SEP_FS * bb];
-#pragma unroll
-                    for (int bz = 0; bz < 3; bz++) e1[3 * SXY[a] + bz] += pz[3 * SDZ[a] + bz] * fv;
-                }
-            }
-            T *o = sE + zp * 18 * FR + zr;
-#pragma unroll
-            for (int i = 0; i < 18; i++) o[FR * i] = e1[i];
+        // ---- z pass over the pencil's four points -> E1[p][(s,bz)][(s',cz)].
+        //      FP64: on all four waves, each half of rt taking one half of
+        //      the x-y parts s (a wave runs one branch); FP32, capped at 168
+        //      VGPRs for three workgroups per CU, keeps whole columns on 108
+        //      threads (the halves spill there)
+        if constexpr (sizeof(T) == 8) {
+            if ((rt & 127) < 108) sep_zcol<T, true>(sF, sP, sE, rt & 127, rt >> 7);
+        } else if (rt < 108) {
+            sep_zcol<T, false>(sF, sP, sE, rt, 0);
         }
         __syncthreads();
     }
