@@ -231,35 +231,41 @@ __global__ __launch_bounds__(256) void k_pack_h16(int64_t ncp, const int64_t *__
 }
 
 // y = A x with the 16-bit blocks (the layout of k_spmv_pb, FP64 sums, the
-// row scale applied once); MODE as k_spmv_pb
-template <int MODE>
+// row scale applied once); MODE as k_spmv_pb.  PPW control points per wave,
+// 64 / PPW lanes each: a point's 20-byte blocks are few (at most 125), so a
+// wave per point spends its time on the dependent loads (row start, neighbour
+// index, x) rather than on the blocks.
+template <int MODE, int PPW>
 __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ pcol,
                                                   const uint32_t *__restrict__ h16, const float *__restrict__ rs,
                                                   const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ y)
 {
-    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (p >= ncp) return;
-    const int64_t b0 = rp[3 * p];
-    const int nn = (int)((rp[3 * p + 1] - b0) / 3);
-    const int32_t *pc = pcol + b0 / 9;
-    const uint32_t *hb = h16 + 5 * (b0 / 9);
+    constexpr int LW = 64 / PPW;   // lanes per control point
+    const int lane = threadIdx.x & 63, sl = lane & (LW - 1);
+    const int64_t p = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * PPW + lane / LW;
+    const bool ok = p < ncp;
     double s0 = 0, s1 = 0, s2 = 0;
-    for (int k = lane; k < nn; k += 64) {
-        const int64_t q = pc[k];
-        const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
-        const uint32_t w0 = hb[k], w1 = hb[nn + k], w2 = hb[2 * nn + k], w3 = hb[3 * nn + k], w4 = hb[4 * nn + k];
-        s0 += h_lo(w0) * x0 + h_hi(w0) * x1 + h_lo(w1) * x2;
-        s1 += h_hi(w1) * x0 + h_lo(w2) * x1 + h_hi(w2) * x2;
-        s2 += h_lo(w3) * x0 + h_hi(w3) * x1 + h_lo(w4) * x2;
+    if (ok) {
+        const int64_t b0 = rp[3 * p];
+        const int nn = (int)((rp[3 * p + 1] - b0) / 3);
+        const int32_t *pc = pcol + b0 / 9;
+        const uint32_t *hb = h16 + 5 * (b0 / 9);
+        for (int k = sl; k < nn; k += LW) {
+            const int64_t q = pc[k];
+            const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
+            const uint32_t w0 = hb[k], w1 = hb[nn + k], w2 = hb[2 * nn + k], w3 = hb[3 * nn + k], w4 = hb[4 * nn + k];
+            s0 += h_lo(w0) * x0 + h_hi(w0) * x1 + h_lo(w1) * x2;
+            s1 += h_hi(w1) * x0 + h_lo(w2) * x1 + h_hi(w2) * x2;
+            s2 += h_lo(w3) * x0 + h_hi(w3) * x1 + h_lo(w4) * x2;
+        }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = LW / 2; o > 0; o >>= 1) {
         s0 += __shfl_xor(s0, o, 64);
         s1 += __shfl_xor(s1, o, 64);
         s2 += __shfl_xor(s2, o, 64);
     }
-    if (lane == 0) {
+    if (ok && sl == 0) {
         s0 *= (double)rs[3 * p];
         s1 *= (double)rs[3 * p + 1];
         s2 *= (double)rs[3 * p + 2];
@@ -278,6 +284,7 @@ __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__
         }
     }
 }
+constexpr int H16_PPW = 2;   // control points per wave of k_spmv_h16
 
 // y[i] = x[idx[i]]  (values or vectors into the coloured order)
 // (grid-stride: launched with grid1)
@@ -1772,12 +1779,14 @@ static int lvl_spmv(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *x, d
     MGLevel &L = g->lev[l];
     if (p != PC_F16 || !L.h16) return spmv(L.c, p == PC_F16 ? IGAP_F32 : p, x, y, mode, b);
     const int64_t ncp = L.c->ndof / 3;
-    const dim3 gr((unsigned)((ncp + 3) / 4)), bs(256);
+    const dim3 gr((unsigned)((ncp + 4 * H16_PPW - 1) / (4 * H16_PPW))), bs(256);
     const int64_t *rp = (const int64_t *)L.c->d_rowptr;
     const int32_t *pc = (const int32_t *)L.c->d_pcol;
-    if (mode == 0) hipLaunchKernelGGL(k_spmv_h16<0>, gr, bs, 0, c->stream, ncp, rp, pc, (const uint32_t *)L.h16, (const float *)L.rs, x, b, y);
-    else if (mode == 1) hipLaunchKernelGGL(k_spmv_h16<1>, gr, bs, 0, c->stream, ncp, rp, pc, (const uint32_t *)L.h16, (const float *)L.rs, x, b, y);
-    else hipLaunchKernelGGL(k_spmv_h16<2>, gr, bs, 0, c->stream, ncp, rp, pc, (const uint32_t *)L.h16, (const float *)L.rs, x, b, y);
+    const uint32_t *hv = (const uint32_t *)L.h16;
+    const float *rs = (const float *)L.rs;
+    if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW>), gr, bs, 0, c->stream, ncp, rp, pc, hv, rs, x, b, y);
+    else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW>), gr, bs, 0, c->stream, ncp, rp, pc, hv, rs, x, b, y);
+    else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW>), gr, bs, 0, c->stream, ncp, rp, pc, hv, rs, x, b, y);
     HIPCHK(hipGetLastError());
     return IGAP_OK;
 }
