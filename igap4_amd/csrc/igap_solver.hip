@@ -230,14 +230,38 @@ __global__ __launch_bounds__(256) void k_pack_h16(int64_t ncp, const int64_t *__
     }
 }
 
+// Point grid of a patch without periodic directions, for products that
+// compute their CSR positions: the rows of control point p = (px, py, pz)
+// list the points q with |q_d - p_d| <= 2 (clipped to the grid) in
+// lexicographic order, so its neighbour count, its first block
+// (rowptr[3p] / 9) and its k-th neighbour are arithmetic.
+struct PointGrid {
+    int nb[3];
+    int64_t L12;   // stencil_prefix(nb1, nb1) * stencil_prefix(nb2, nb2)
+    int64_t L2;    // stencil_prefix(nb2, nb2)
+};
+
+// sum over the points a' < a of one direction (n points) of their clipped
+// stencil lengths min(a' + 2, n - 1) - max(a' - 2, 0) + 1
+__host__ __device__ __forceinline__ int64_t stencil_prefix(int64_t a, int64_t n)
+{
+    const int64_t m = a < (n - 2 > 0 ? n - 2 : 0) ? a : (n - 2 > 0 ? n - 2 : 0);
+    const int64_t smin = m * (m - 1) / 2 + 2 * m + (a - m) * (n - 1);
+    const int64_t smax = a >= 3 ? (a - 3) * (a - 2) / 2 : 0;
+    return smin - smax + a;
+}
+
 // y = A x with the 16-bit blocks (the layout of k_spmv_pb, FP64 sums, the
 // row scale applied once); MODE as k_spmv_pb.  PPW control points per wave,
 // 64 / PPW lanes each: a point's 20-byte blocks are few (at most 125), so a
 // wave per point spends its time on the dependent loads (row start, neighbour
-// index, x) rather than on the blocks.
-template <int MODE, int PPW>
+// index, x) rather than on the blocks.  GRID: no periodic direction, the
+// first block, the count and the points of the neighbours from the point grid
+// G instead of rowptr and the neighbour index (the blocks and x the only
+// loads).
+template <int MODE, int PPW, bool GRID>
 __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ pcol,
-                                                  const uint32_t *__restrict__ h16, const float *__restrict__ rs,
+                                                  PointGrid G, const uint32_t *__restrict__ h16, const float *__restrict__ rs,
                                                   const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ y)
 {
     constexpr int LW = 64 / PPW;   // lanes per control point
@@ -245,7 +269,26 @@ __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__
     const int64_t p = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * PPW + lane / LW;
     const bool ok = p < ncp;
     double s0 = 0, s1 = 0, s2 = 0;
-    if (ok) {
+    if (ok && GRID) {
+        const int n1 = G.nb[1], n2 = G.nb[2];
+        const int pz = (int)(p % n2), py = (int)((p / n2) % n1), px = (int)(p / ((int64_t)n1 * n2));
+        const int lox = px - 2 > 0 ? px - 2 : 0, loy = py - 2 > 0 ? py - 2 : 0, loz = pz - 2 > 0 ? pz - 2 : 0;
+        const int lx = (px + 2 < G.nb[0] - 1 ? px + 2 : G.nb[0] - 1) - lox + 1;
+        const int ly = (py + 2 < n1 - 1 ? py + 2 : n1 - 1) - loy + 1;
+        const int lz = (pz + 2 < n2 - 1 ? pz + 2 : n2 - 1) - loz + 1;
+        const int nn = lx * ly * lz, lyz = ly * lz;
+        const int64_t blk = stencil_prefix(px, G.nb[0]) * G.L12 + lx * (stencil_prefix(py, n1) * G.L2 + ly * stencil_prefix(pz, n2));
+        const uint32_t *hb = h16 + 5 * blk;
+        for (int k = sl; k < nn; k += LW) {
+            const int kx = k / lyz, r = k - kx * lyz, ky = r / lz, kz = r - ky * lz;
+            const int64_t q = ((int64_t)(lox + kx) * n1 + (loy + ky)) * n2 + (loz + kz);
+            const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
+            const uint32_t w0 = hb[k], w1 = hb[nn + k], w2 = hb[2 * nn + k], w3 = hb[3 * nn + k], w4 = hb[4 * nn + k];
+            s0 += h_lo(w0) * x0 + h_hi(w0) * x1 + h_lo(w1) * x2;
+            s1 += h_hi(w1) * x0 + h_lo(w2) * x1 + h_hi(w2) * x2;
+            s2 += h_lo(w3) * x0 + h_hi(w3) * x1 + h_lo(w4) * x2;
+        }
+    } else if (ok) {
         const int64_t b0 = rp[3 * p];
         const int nn = (int)((rp[3 * p + 1] - b0) / 3);
         const int32_t *pc = pcol + b0 / 9;
@@ -1784,9 +1827,20 @@ static int lvl_spmv(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *x, d
     const int32_t *pc = (const int32_t *)L.c->d_pcol;
     const uint32_t *hv = (const uint32_t *)L.h16;
     const float *rs = (const float *)L.rs;
-    if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW>), gr, bs, 0, c->stream, ncp, rp, pc, hv, rs, x, b, y);
-    else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW>), gr, bs, 0, c->stream, ncp, rp, pc, hv, rs, x, b, y);
-    else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW>), gr, bs, 0, c->stream, ncp, rp, pc, hv, rs, x, b, y);
+    const MeshK &M = L.c->M;
+    PointGrid G;
+    for (int d = 0; d < 3; d++) G.nb[d] = M.nb[d];
+    G.L2 = stencil_prefix(M.nb[2], M.nb[2]);
+    G.L12 = stencil_prefix(M.nb[1], M.nb[1]) * G.L2;
+    if (!M.periodic[0] && !M.periodic[1] && !M.periodic[2]) {
+        if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW, true>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
+        else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW, true>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
+        else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW, true>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
+    } else {
+        if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW, false>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
+        else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW, false>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
+        else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW, false>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
+    }
     HIPCHK(hipGetLastError());
     return IGAP_OK;
 }
