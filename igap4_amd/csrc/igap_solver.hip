@@ -279,14 +279,33 @@ __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__
         const int nn = lx * ly * lz, lyz = ly * lz;
         const int64_t blk = stencil_prefix(px, G.nb[0]) * G.L12 + lx * (stencil_prefix(py, n1) * G.L2 + ly * stencil_prefix(pz, n2));
         const uint32_t *hb = h16 + 5 * blk;
-        for (int k = sl; k < nn; k += LW) {
-            const int kx = k / lyz, r = k - kx * lyz, ky = r / lz, kz = r - ky * lz;
-            const int64_t q = ((int64_t)(lox + kx) * n1 + (loy + ky)) * n2 + (loz + kz);
-            const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
-            const uint32_t w0 = hb[k], w1 = hb[nn + k], w2 = hb[2 * nn + k], w3 = hb[3 * nn + k], w4 = hb[4 * nn + k];
-            s0 += h_lo(w0) * x0 + h_hi(w0) * x1 + h_lo(w1) * x2;
-            s1 += h_hi(w1) * x0 + h_lo(w2) * x1 + h_hi(w2) * x2;
-            s2 += h_lo(w3) * x0 + h_hi(w3) * x1 + h_lo(w4) * x2;
+        // at most 125 neighbours: a fixed number of rounds, every load issued
+        // before the first sum
+        constexpr int NIT = (125 + LW - 1) / LW;
+        uint32_t w[NIT][5];
+        double xv[NIT][3];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int k = sl + it * LW;
+            if (k < nn) {
+                const int kx = k / lyz, r = k - kx * lyz, ky = r / lz, kz = r - ky * lz;
+                const int64_t q = ((int64_t)(lox + kx) * n1 + (loy + ky)) * n2 + (loz + kz);
+#pragma unroll
+                for (int j = 0; j < 3; j++) xv[it][j] = x[3 * q + j];
+#pragma unroll
+                for (int j = 0; j < 5; j++) w[it][j] = hb[j * nn + k];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; j++) xv[it][j] = 0.0;
+#pragma unroll
+                for (int j = 0; j < 5; j++) w[it][j] = 0u;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            s0 += h_lo(w[it][0]) * xv[it][0] + h_hi(w[it][0]) * xv[it][1] + h_lo(w[it][1]) * xv[it][2];
+            s1 += h_hi(w[it][1]) * xv[it][0] + h_lo(w[it][2]) * xv[it][1] + h_hi(w[it][2]) * xv[it][2];
+            s2 += h_lo(w[it][3]) * xv[it][0] + h_hi(w[it][3]) * xv[it][1] + h_lo(w[it][4]) * xv[it][2];
         }
     } else if (ok) {
         const int64_t b0 = rp[3 * p];
