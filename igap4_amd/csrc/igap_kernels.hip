@@ -1254,6 +1254,10 @@ __device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int 
 // Row stride of the pencil's F buffer (18 columns): 19, so that the lanes of a
 // row store, each writing its own row, spread over the LDS banks
 constexpr int SEP_FS = 19;
+// Block stride of the pencil's E1 buffer (18 x 18 per block): 327, 7 modulo
+// the 32 bank pairs, so that the y pass's lanes of neighbouring blocks read
+// disjoint banks (324 puts blocks 4 pairs apart and overlaps them)
+constexpr int SEP_ES = 327;
 
 // jet slot a = 1..9 of the pointwise rows -> order in z, x-y part s = (dx, dy)
 constexpr int SEP_DZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
@@ -1308,7 +1312,7 @@ __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T 
             }
         }
     }
-    T *o = sE + zp * 18 * FR + zr;
+    T *o = sE + zp * SEP_ES + zr;
 #pragma unroll
     for (int i = 0; i < 3 * NS; i++) o[FR * (HALVES ? 3 * ZI[h][i / 3] + i % 3 : i)] = e1[i];
 }
@@ -1340,7 +1344,7 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
         if (chunk > 0 && rt < 162) {
             // ---- y pass of the previous pencil, x pass after a qx's fourth qy
             const int pc = chunk - 1, qx = pc >> 2, qy = pc & 3;
-            const T *E = sE + yp * 18 * FR;
+            const T *E = sE + yp * SEP_ES;
             T pyb[3], pyc[9];
 #pragma unroll
             for (int d = 0; d < 3; d++) pyb[d] = sP[1][qy][d][yby];
@@ -1535,7 +1539,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(M
     // on the same LDS banks
     constexpr int SSTR = 50;
     __shared__ double sSt[64 * SSTR];
-    __shared__ T sE[6 * 18 * 18];               // E1 of the current pencil
+    __shared__ T sE[6 * SEP_ES];                // E1 of the current pencil
     __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
     T *sF = reinterpret_cast<T *>(sWork);
 
@@ -1623,7 +1627,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t
     __shared__ double sWgt[64];
     __shared__ double sMass[3][3][3];          // 1-D mass matrices M_d[b][c]
     __shared__ T sMom[64 * MWC];               // compact moment states
-    __shared__ T sE[6 * 18 * 18];
+    __shared__ T sE[6 * SEP_ES];
     __shared__ __attribute__((aligned(16))) char sWork[WORK];
     T *sF = reinterpret_cast<T *>(sWork);
     T *sJet = reinterpret_cast<T *>(sWork);     // [q][h][k][a - 1], stage A only
