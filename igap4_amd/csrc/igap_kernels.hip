@@ -1263,7 +1263,7 @@ constexpr int SEP_ES = 327;
 constexpr int SEP_DZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
 constexpr int SEP_XY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
 
-// z pass of one E1 column (block p, column (s', cz)) = task zt < 108 of the
+// z pass of one E1 column (block p, column (s', cz)), task zt < 108 of the
 // pencil: E1[p][(s,bz)][(s',cz)] = sum_qz sum_{a: s(a) = s} Pz[qz][dz(a)][bz]
 // F_qz[p][a][(s',cz)].  HALVES: only the x-y parts s of half h ({0, 1, 3}:
 // five jet slots; {2, 4, 5}: four; h uniform over the wave); otherwise all
@@ -1276,7 +1276,10 @@ __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T 
     constexpr int ZS[6] = {0, 1, 0, 2, 1, 2};              // place of s in its half
     constexpr int ZI[2][3] = {{0, 1, 3}, {2, 4, 5}};       // s at each place
     constexpr int NS = HALVES ? 3 : 6;
-    const int zp = zt / 18, zr = zt - 18 * zp;
+    // whole columns (FP32): consecutive tasks in consecutive blocks, so that
+    // a wave's lanes read the F rows of all six blocks at each column (fewer
+    // LDS bank conflicts; measured slower for the halves)
+    const int zp = HALVES ? zt / 18 : zt % 6, zr = HALVES ? zt - 18 * zp : zt / 6;
     T e1[3 * NS];   // [s][bz]
 #pragma unroll
     for (int i = 0; i < 3 * NS; i++) e1[i] = (T)0;
