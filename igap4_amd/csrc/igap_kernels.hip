@@ -1254,10 +1254,15 @@ __device__ __forceinline__ double sep_weight(const MeshK &M, const int *ed, int 
 // Row stride of the pencil's F buffer (18 columns): 19, so that the lanes of a
 // row store, each writing its own row, spread over the LDS banks
 constexpr int SEP_FS = 19;
-// Block stride of the pencil's E1 buffer (18 x 18 per block): 327, 7 modulo
+// Block stride of the F buffer (9 rows per block): 178, 18 modulo the 32 LDS
+// bank pairs, so that the z pass's lanes (18 columns of each of four blocks)
+// and the rows' stores put at most two (z pass) or two to three (stores)
+// lanes on a bank pair; 171 = 9 x 19 gives three
+constexpr int SEP_FB = 178;
+// Block stride of the pencil's E1 buffer (18 x 18 per block): 331, 11 modulo
 // the 32 bank pairs, so that the y pass's lanes of neighbouring blocks read
 // disjoint banks (324 puts blocks 4 pairs apart and overlaps them)
-constexpr int SEP_ES = 327;
+constexpr int SEP_ES = 331;
 
 // jet slot a = 1..9 of the pointwise rows -> order in z, x-y part s = (dx, dy)
 constexpr int SEP_DZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
@@ -1285,7 +1290,7 @@ __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T 
     for (int i = 0; i < 3 * NS; i++) e1[i] = (T)0;
 #pragma unroll
     for (int qz = 0; qz < 4; qz++) {
-        const T *Fp = sF + (qz * 6 + zp) * NA * SEP_FS + zr;
+        const T *Fp = sF + (qz * 6 + zp) * SEP_FB + zr;
         const T *pz = &sP[2][qz][0][0];
         if (!HALVES) {
 #pragma unroll
@@ -1426,7 +1431,7 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
 #pragma unroll
                     for (int cz = 0; cz < 3; cz++) f[3 * SEP_XY[a] + cz] += v * pz[3 * SEP_DZ[a] + cz];
                 }
-                T *o = sF + ((qz * 6 + p) * NA + ar) * SEP_FS;
+                T *o = sF + (qz * 6 + p) * SEP_FB + ar * SEP_FS;
 #pragma unroll
                 for (int i = 0; i < FR; i++) o[i] = f[i];
             }
@@ -1531,7 +1536,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(M
                                                         T *__restrict__ Ke, int nel, Scatter S)
 {
     constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of the 3 displacement components
-    constexpr int WORK_F = 4 * 6 * 9 * SEP_FS * (int)sizeof(T);    // F of the pencil's four points (9 rows x 18 columns)
+    constexpr int WORK_F = 4 * 6 * SEP_FB * (int)sizeof(T);        // F of the pencil's four points (9 rows x 18 columns)
     __shared__ double sPd[3][4][3][3];          // 1-D tables of physical derivatives [d][q_d][order][i]
     __shared__ T sP[3][4][3][3];                // the same in the contraction's precision
     __shared__ double sG[3][4];                 // dx_d / dxi_d at the 1-D Gauss points
@@ -1620,7 +1625,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t
                                                                                int64_t ndof, T *__restrict__ Ke, int nel, Scatter S)
 {
     constexpr int WORK_A = 3 * (108 + 288) * (int)sizeof(double);   // field passes of 3 components
-    constexpr int WORK_F = 4 * 6 * 9 * SEP_FS * (int)sizeof(T);    // F of the pencil's four points
+    constexpr int WORK_F = 4 * 6 * SEP_FB * (int)sizeof(T);        // F of the pencil's four points
     constexpr int WORK_J = 64 * 54 * (int)sizeof(T);               // jets of Um and Ud, slots 1..9
     constexpr int WORK = WORK_A > WORK_F ? (WORK_A > WORK_J ? WORK_A : WORK_J) : (WORK_F > WORK_J ? WORK_F : WORK_J);
     __shared__ double sPd[3][4][3][3];
