@@ -1281,10 +1281,7 @@ __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T 
     constexpr int ZS[6] = {0, 1, 0, 2, 1, 2};              // place of s in its half
     constexpr int ZI[2][3] = {{0, 1, 3}, {2, 4, 5}};       // s at each place
     constexpr int NS = HALVES ? 3 : 6;
-    // whole columns (FP32): consecutive tasks in consecutive blocks, so that
-    // a wave's lanes read the F rows of all six blocks at each column (fewer
-    // LDS bank conflicts; measured slower for the halves)
-    const int zp = HALVES ? zt / 18 : zt % 6, zr = HALVES ? zt - 18 * zp : zt / 6;
+    const int zp = zt / 18, zr = zt - 18 * zp;
     T e1[3 * NS];   // [s][bz]
 #pragma unroll
     for (int i = 0; i < 3 * NS; i++) e1[i] = (T)0;
