@@ -252,16 +252,15 @@ __host__ __device__ __forceinline__ int64_t stencil_prefix(int64_t a, int64_t n)
 }
 
 // y = A x with the 16-bit blocks (the layout of k_spmv_pb, FP64 sums, the
-// row scale applied once); MODE as k_spmv_pb.  PPW control points per wave,
-// 64 / PPW lanes each: a point's 20-byte blocks are few (at most 125), so a
-// wave per point spends its time on the dependent loads (row start, neighbour
-// index, x) rather than on the blocks.  GRID: no periodic direction, the
-// first block, the count and the points of the neighbours from the point grid
-// G instead of rowptr and the neighbour index (the blocks and x the only
-// loads).
-template <int MODE, int PPW, bool GRID>
-__global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__restrict__ rp, const int32_t *__restrict__ pcol,
-                                                  PointGrid G, const uint32_t *__restrict__ h16, const float *__restrict__ rs,
+// row scale applied once); MODE as k_spmv_pb.  Multigrid levels have no
+// periodic direction, so the point's first block, its neighbour count and
+// its neighbours come from the point grid G instead of rowptr and the
+// neighbour index (the blocks and x are the only loads).  PPW control points
+// per wave, 64 / PPW lanes each: a point's 20-byte blocks are few (at most
+// 125), so a wave per point spends its time waiting on its loads rather
+// than on the blocks; every load of a lane is issued before the sums.
+template <int MODE, int PPW>
+__global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, PointGrid G, const uint32_t *__restrict__ h16, const float *__restrict__ rs,
                                                   const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ y)
 {
     constexpr int LW = 64 / PPW;   // lanes per control point
@@ -269,7 +268,7 @@ __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__
     const int64_t p = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * PPW + lane / LW;
     const bool ok = p < ncp;
     double s0 = 0, s1 = 0, s2 = 0;
-    if (ok && GRID) {
+    if (ok) {
         const int n1 = G.nb[1], n2 = G.nb[2];
         const int pz = (int)(p % n2), py = (int)((p / n2) % n1), px = (int)(p / ((int64_t)n1 * n2));
         const int lox = px - 2 > 0 ? px - 2 : 0, loy = py - 2 > 0 ? py - 2 : 0, loz = pz - 2 > 0 ? pz - 2 : 0;
@@ -279,8 +278,7 @@ __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__
         const int nn = lx * ly * lz, lyz = ly * lz;
         const int64_t blk = stencil_prefix(px, G.nb[0]) * G.L12 + lx * (stencil_prefix(py, n1) * G.L2 + ly * stencil_prefix(pz, n2));
         const uint32_t *hb = h16 + 5 * blk;
-        // at most 125 neighbours: a fixed number of rounds, every load issued
-        // before the first sum
+        // at most 125 neighbours: a fixed number of rounds
         constexpr int NIT = (125 + LW - 1) / LW;
         uint32_t w[NIT][5];
         double xv[NIT][3];
@@ -306,19 +304,6 @@ __global__ __launch_bounds__(256) void k_spmv_h16(int64_t ncp, const int64_t *__
             s0 += h_lo(w[it][0]) * xv[it][0] + h_hi(w[it][0]) * xv[it][1] + h_lo(w[it][1]) * xv[it][2];
             s1 += h_hi(w[it][1]) * xv[it][0] + h_lo(w[it][2]) * xv[it][1] + h_hi(w[it][2]) * xv[it][2];
             s2 += h_lo(w[it][3]) * xv[it][0] + h_hi(w[it][3]) * xv[it][1] + h_lo(w[it][4]) * xv[it][2];
-        }
-    } else if (ok) {
-        const int64_t b0 = rp[3 * p];
-        const int nn = (int)((rp[3 * p + 1] - b0) / 3);
-        const int32_t *pc = pcol + b0 / 9;
-        const uint32_t *hb = h16 + 5 * (b0 / 9);
-        for (int k = sl; k < nn; k += LW) {
-            const int64_t q = pc[k];
-            const double x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
-            const uint32_t w0 = hb[k], w1 = hb[nn + k], w2 = hb[2 * nn + k], w3 = hb[3 * nn + k], w4 = hb[4 * nn + k];
-            s0 += h_lo(w0) * x0 + h_hi(w0) * x1 + h_lo(w1) * x2;
-            s1 += h_hi(w1) * x0 + h_lo(w2) * x1 + h_hi(w2) * x2;
-            s2 += h_lo(w3) * x0 + h_hi(w3) * x1 + h_lo(w4) * x2;
         }
     }
 #pragma unroll
@@ -1842,24 +1827,17 @@ static int lvl_spmv(igap_gpu_ctx *c, MGHier *g, int l, int p, const double *x, d
     if (p != PC_F16 || !L.h16) return spmv(L.c, p == PC_F16 ? IGAP_F32 : p, x, y, mode, b);
     const int64_t ncp = L.c->ndof / 3;
     const dim3 gr((unsigned)((ncp + 4 * H16_PPW - 1) / (4 * H16_PPW))), bs(256);
-    const int64_t *rp = (const int64_t *)L.c->d_rowptr;
-    const int32_t *pc = (const int32_t *)L.c->d_pcol;
     const uint32_t *hv = (const uint32_t *)L.h16;
     const float *rs = (const float *)L.rs;
     const MeshK &M = L.c->M;
+    if (M.periodic[0] || M.periodic[1] || M.periodic[2]) return fail(IGAP_ERR_STATE, "16-bit multigrid level on a periodic patch");
     PointGrid G;
     for (int d = 0; d < 3; d++) G.nb[d] = M.nb[d];
     G.L2 = stencil_prefix(M.nb[2], M.nb[2]);
     G.L12 = stencil_prefix(M.nb[1], M.nb[1]) * G.L2;
-    if (!M.periodic[0] && !M.periodic[1] && !M.periodic[2]) {
-        if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW, true>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
-        else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW, true>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
-        else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW, true>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
-    } else {
-        if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW, false>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
-        else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW, false>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
-        else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW, false>), gr, bs, 0, c->stream, ncp, rp, pc, G, hv, rs, x, b, y);
-    }
+    if (mode == 0) hipLaunchKernelGGL((k_spmv_h16<0, H16_PPW>), gr, bs, 0, c->stream, ncp, G, hv, rs, x, b, y);
+    else if (mode == 1) hipLaunchKernelGGL((k_spmv_h16<1, H16_PPW>), gr, bs, 0, c->stream, ncp, G, hv, rs, x, b, y);
+    else hipLaunchKernelGGL((k_spmv_h16<2, H16_PPW>), gr, bs, 0, c->stream, ncp, G, hv, rs, x, b, y);
     HIPCHK(hipGetLastError());
     return IGAP_OK;
 }
