@@ -1259,10 +1259,11 @@ constexpr int SEP_FS = 19;
 // and the rows' stores put at most two (z pass) or two to three (stores)
 // lanes on a bank pair; 171 = 9 x 19 gives three
 constexpr int SEP_FB = 178;
-// Block stride of the pencil's E1 buffer (18 x 18 per block): 331, 11 modulo
-// the 32 bank pairs, so that the y pass's lanes of neighbouring blocks read
-// disjoint banks (324 puts blocks 4 pairs apart and overlaps them)
-constexpr int SEP_ES = 331;
+// E1 of the pencil in LDS as [block p][bz][cz][s][s'] (s, s' the x-y parts):
+// a y-pass thread's 36 values (its bz, cz) contiguous, read two at a time.
+// Strides (even: 16-byte aligned pairs): SEP_EC per (bz, cz), SEP_EB per block.
+constexpr int SEP_EC = 38;
+constexpr int SEP_EB = 342;
 
 // jet slot a = 1..9 of the pointwise rows -> order in z, x-y part s = (dx, dy)
 constexpr int SEP_DZ[10] = {0, 0, 0, 1, 0, 0, 1, 0, 1, 2};
@@ -1276,7 +1277,7 @@ constexpr int SEP_XY[10] = {0, 1, 2, 0, 3, 4, 1, 5, 2, 0};
 template <typename T, bool HALVES>
 __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T *sE, int zt, int h)
 {
-    constexpr int NA = 9, FR = 18;
+    constexpr int NA = 9;
     constexpr int ZH[6] = {0, 0, 1, 0, 1, 1};              // half of s
     constexpr int ZS[6] = {0, 1, 0, 2, 1, 2};              // place of s in its half
     constexpr int ZI[2][3] = {{0, 1, 3}, {2, 4, 5}};       // s at each place
@@ -1317,9 +1318,10 @@ __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T 
             }
         }
     }
-    T *o = sE + zp * SEP_ES + zr;
+    const int sp = zr / 3, cz = zr - 3 * sp;   // column (s', cz)
+    T *o = sE + zp * SEP_EB + cz * SEP_EC + sp;
 #pragma unroll
-    for (int i = 0; i < 3 * NS; i++) o[FR * (HALVES ? 3 * ZI[h][i / 3] + i % 3 : i)] = e1[i];
+    for (int i = 0; i < 3 * NS; i++) o[(i % 3) * 3 * SEP_EC + 6 * (HALVES ? ZI[h][i / 3] : i / 3)] = e1[i];
 }
 
 // The pencil loop of the separable tangents, after stage A and a barrier
@@ -1349,20 +1351,29 @@ __device__ __forceinline__ void sep_pencils(int rt, const T (*sP)[4][3][3], cons
         if (chunk > 0 && rt < 162) {
             // ---- y pass of the previous pencil, x pass after a qx's fourth qy
             const int pc = chunk - 1, qx = pc >> 2, qy = pc & 3;
-            const T *E = sE + yp * SEP_ES;
+            typedef T pair_t __attribute__((ext_vector_type(2)));
+            const pair_t *E = reinterpret_cast<const pair_t *>(sE + yp * SEP_EB + (3 * ybz + ycz) * SEP_EC);   // [s1][s2]
             T pyb[3], pyc[9];
 #pragma unroll
             for (int d = 0; d < 3; d++) pyb[d] = sP[1][qy][d][yby];
 #pragma unroll
             for (int i = 0; i < 9; i++) pyc[i] = sP[1][qy][i / 3][i % 3];   // [dy'][cy]
 #pragma unroll
-            for (int s1 = 0; s1 < 6; s1++)
+            for (int s1 = 0; s1 < 6; s1++) {
+                T ev[6];
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    const pair_t x = E[3 * s1 + v];
+                    ev[2 * v] = x[0];
+                    ev[2 * v + 1] = x[1];
+                }
 #pragma unroll
                 for (int s2 = 0; s2 < 6; s2++) {
-                    const T t1 = pyb[XYY[s1]] * E[(3 * s1 + ybz) * FR + 3 * s2 + ycz];
+                    const T t1 = pyb[XYY[s1]] * ev[s2];
 #pragma unroll
                     for (int cy = 0; cy < 3; cy++) E2[(3 * XYX[s1] + XYX[s2]) * 3 + cy] += t1 * pyc[3 * XYY[s2] + cy];
                 }
+            }
             if (qy == 3) {
                 T px[9], G[27];   // px[dx][b]; G[dx][cx][cy]
 #pragma unroll
@@ -1544,7 +1555,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 2 : 3) void k_tangent_sep(M
     // on the same LDS banks
     constexpr int SSTR = 50;
     __shared__ double sSt[64 * SSTR];
-    __shared__ T sE[6 * SEP_ES];                // E1 of the current pencil
+    __shared__ __attribute__((aligned(16))) T sE[6 * SEP_EB];   // E1 of the current pencil
     __shared__ __attribute__((aligned(16))) char sWork[WORK_A > WORK_F ? WORK_A : WORK_F];
     T *sF = reinterpret_cast<T *>(sWork);
 
@@ -1632,7 +1643,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8) ? 1 : 2) void k_tangent_sep_t
     __shared__ double sWgt[64];
     __shared__ double sMass[3][3][3];          // 1-D mass matrices M_d[b][c]
     __shared__ T sMom[64 * MWC];               // compact moment states
-    __shared__ T sE[6 * SEP_ES];
+    __shared__ __attribute__((aligned(16))) T sE[6 * SEP_EB];
     __shared__ __attribute__((aligned(16))) char sWork[WORK];
     T *sF = reinterpret_cast<T *>(sWork);
     T *sJet = reinterpret_cast<T *>(sWork);     // [q][h][k][a - 1], stage A only
