@@ -1329,7 +1329,8 @@ __device__ __forceinline__ void sep_zcol(const T *sF, const T (*sP)[4][3][3], T 
 // (l, b)], b = 1..9, of block p = (k, l) at Gauss point q (unweighted; row
 // arrives zeroed); sWgt holds the point weights, sF and sE the F and E1
 // buffers.  The z pass runs on 108 threads (block p, column (s', cz) of E1;
-// FP64: 216, a column per half of the x-y parts), the y/x pass on 162 (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]),
+// FP64: 216, a column per half of the x-y parts), the y/x pass on 162
+// (block p, bz, cz, by: E2[dx][dx'][cy], K[bx][cx][cy]), all
 // numbered from the rotated thread index rt so that the workgroups sharing a
 // CU put them on different SIMDs.  The y pass of pencil c runs beside the
 // pointwise stage of pencil c + 1 (F and E1 have buffers of their own), so a
