@@ -120,6 +120,7 @@ _SIGS = {
     "igap_gpu_solver_default_opts": (C.c_int, [C.POINTER(SolverOpts)]),
     "igap_gpu_snes_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     "igap_gpu_pc_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "igap_gpu_tangent_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     # include/igap_ws.h
     "igap_ws_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_uint64,
                                  C.POINTER(C.c_void_p)]),

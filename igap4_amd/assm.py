@@ -323,6 +323,25 @@ class Assm:
             for p in ptrs:
                 lib().igap_gpu_free(self.ctx, p)
 
+    def tangent_apply(self, x, prec="f64"):
+        """y = K x (MatMult) with the context's own tangent values of ``prec``
+        (the last tangent assembled into them).  Host vectors in and out."""
+        p = _lib.IGAP_F64 if prec in ("f64", 0) else _lib.IGAP_F32
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        ptrs = []
+        try:
+            for _ in range(2):
+                q = C.c_void_p()
+                check(lib().igap_gpu_malloc(self.ctx, C.byref(q), 8 * self.ndof))
+                ptrs.append(q)
+            self.htod(ptrs[0], x)
+            check(lib().igap_gpu_tangent_apply(self.ctx, p, ptrs[0], ptrs[1]))
+            self.synchronize()
+            return self.dtoh(ptrs[1], self.ndof)
+        finally:
+            for q in ptrs:
+                lib().igap_gpu_free(self.ctx, q)
+
     # ------------------------------------------------------------ timing
     def timing(self, on=True):
         check(lib().igap_gpu_timing_enable(self.ctx, 1 if on else 0))

@@ -2309,6 +2309,18 @@ extern "C" int igap_gpu_pc_apply(igap_gpu_ctx *c, int which, const double *b, do
     return pc_apply(c, s, s->pc_kind, s->pc_prec, s->pc_ord, b, x);
 }
 
+// MatMult on the context's own tangent values: y = K x (FP64 sums)
+extern "C" int igap_gpu_tangent_apply(igap_gpu_ctx *c, int prec, const double *x, double *y)
+{
+    if (!c || !x || !y) return fail(IGAP_ERR_ARG, "null argument");
+    if (prec != IGAP_F64 && prec != IGAP_F32) return fail(IGAP_ERR_ARG, "prec %d (IGAP_F64 or IGAP_F32)", prec);
+    if (x == y) return fail(IGAP_ERR_ARG, "x and y must differ");
+    if ((prec == IGAP_F64 && !c->d_val64) || (prec == IGAP_F32 && !c->d_val32))
+        return fail(IGAP_ERR_STATE, "no %s tangent values assembled yet", prec == IGAP_F64 ? "FP64" : "FP32");
+    HIPCHK(hipSetDevice(c->device));
+    return spmv(c, prec, x, y);
+}
+
 // Self test of the point-block ILU(0): factor the context's current tangent
 // values (precision prec) in the coloured order with k_mcb_ilu and with
 // rocSPARSE csrilu0; *diff = max |a - b| / max |b| over the factor entries.

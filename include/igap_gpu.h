@@ -193,6 +193,13 @@ double *igap_gpu_rvec(igap_gpu_ctx *ctx);
  * prec = IGAP_F64 writes vals as double, IGAP_F32 as float (FP32 MFMA);
  * vals may be NULL for the context's own arrays. */
 int igap_gpu_tangent(igap_gpu_ctx *ctx, const double *U, void *vals, int prec);
+/* MatMult (PETSc's) on the tangent IGAP4 assembles into assm->TT
+ * (src/interface.c:191,219): y = K x with the context's own tangent values
+ * of precision prec (those of the last igap_gpu_tangent with vals = NULL,
+ * or of the last Newton refresh), FP64 sums.  Device vectors of ndof
+ * doubles, x != y; stream-ordered.  IGAP_ERR_STATE when the context holds
+ * no tangent values of that precision. */
+int igap_gpu_tangent_apply(igap_gpu_ctx *ctx, int prec, const double *x, double *y);
 
 /* Pointwise physics on the device with the context's physics: the Phys
  * callbacks residual / tangent / density (typedefs residual_ptr, tangent_ptr,
