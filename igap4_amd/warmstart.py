@@ -207,6 +207,86 @@ def train_selected(net, samples, epochs=60, lr=2e-3, seed=0, holdout=0.1, log=No
             "train_samples": len(tr), "val_samples": len(va), "correction_enabled": enabled}
 
 
+def residual_shifted_target(y, rq, kmul, km, alpha):
+    """(r, t) for the network's output y of one sample: r = rq + K (km y),
+    the candidate's linearised residual over its scale, and the shifted mean
+    square target t = y - alpha km (K r).  The gradient of the mean square
+    loss |y - t|^2 / N (N = y.size) at t is (2 alpha / N) times the gradient
+    of |r|^2 / 2 with respect to y, for K symmetric (kmul(v) = K v) and km
+    the 0/1 mask of the dofs the network may move."""
+    r = rq + kmul(y * km)
+    return r, y - alpha * (km * kmul(r))
+
+
+def residual_finetune(net, A, samples, cq, s2, keep, bcs, epochs=30, lr=5e-4, seed=0, holdout=0.1, cosine=False,
+                      log=None):
+    """Train the network on the residual of the start it builds, not on the
+    distance to the next solution.  For sample i the candidate is
+    c_q + s2_i * y (c_q the quadratic extrapolation, y the network's output)
+    and its residual, linearised about c_q with the context's own tangent K
+    (assembled by the caller), is s2_i (rq_i + K y) with rq_i = R(c_q)/s2_i;
+    ``cq[i]`` and ``s2[i]`` are sample i's c_q and scale.
+    The library trains on a mean square error only; with the target shifted
+    to y - alpha g, g = m (K (rq_i + K (m y))) with m the mask of the free
+    dofs (K symmetric), the gradient it
+    back-propagates is alpha g, the gradient of |rq_i + K (m y)|^2 / 2, so Adam
+    minimises the linearised residual.  alpha is fixed per call, so the
+    samples keep their relative weights (residual_shifted_target).  With
+    ``cosine`` the learning rate decays from lr to 0 over the epochs on a
+    half cosine, else it stays lr.  The last ``holdout`` fraction is
+    scored with the true residual after every epoch, median over the samples
+    of |R(c_q + s2 y)| / |R(c_q)| (``bcs[i]()`` sets sample i's load), and
+    the epoch with the lowest median is kept.  Returns a dict."""
+    nv = max(1, int(round(holdout * len(samples))))
+    tr = list(range(len(samples) - nv))
+    va = list(range(len(samples) - nv, len(samples)))
+    rng = np.random.default_rng(seed)
+    km = keep.astype(np.float64)
+
+
+    rq, r0 = [], []
+    for i in range(len(samples)):
+        bcs[i]()
+        r = A.residual(cq[i])
+        rq.append(r / s2[i])
+        r0.append(float(np.linalg.norm(r)))
+
+    def val_ratio():
+        out = []
+        for i in va:
+            bcs[i]()
+            y = net.forward(samples[i][0]).astype(np.float64).reshape(-1)
+            r1 = float(np.linalg.norm(A.residual(cq[i] + s2[i] * y * km)))
+            out.append(r1 / r0[i] if r0[i] > 0 else 1.0)
+        return float(np.median(out))
+
+    y0 = net.forward(samples[tr[0]][0]).astype(np.float64).reshape(-1)
+    g0 = y0 - residual_shifted_target(y0, rq[tr[0]], A.tangent_apply, km, 1.0)[1]
+    t0 = samples[tr[0]][1].astype(np.float64).reshape(-1)
+    alpha = 0.1 * _rms(t0) / _rms(g0)
+    best, best_p, hist, vhist = val_ratio(), net.get_params(), [], []
+    v0 = best
+    for ep in range(epochs):
+        tot = 0.0
+        lr_ep = 0.5 * lr * (1.0 + np.cos(np.pi * ep / epochs)) if cosine else lr
+        for i in rng.permutation(tr):
+            x = samples[i][0]
+            y = net.forward(x).astype(np.float64).reshape(-1)
+            r, t = residual_shifted_target(y, rq[i], A.tangent_apply, km, alpha)
+            tot += float(np.dot(r, r))
+            net.train_step(x, t.astype(np.float32).reshape(-1, 3), lr_ep)
+        hist.append(tot / len(tr))
+        v = val_ratio()
+        vhist.append(v)
+        if v < best:
+            best, best_p = v, net.get_params()
+        if log:
+            log(ep, hist[-1], v)
+    net.set_params(best_p)
+    return {"train_residual": hist, "val_ratio": vhist, "val_ratio_start": v0, "val_ratio_best": best,
+            "alpha": alpha}
+
+
 # ------------------------------------------------------- coarse-space filter
 def _bs2_rows(kv, nfun, x):
     """Values of the nfun quadratic B-splines of the open knot vector kv at the
@@ -354,7 +434,7 @@ class LearnedGuess:
 def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr=2e-3, prec="f32", pc_order=1,
         seed=0, device=0, verbose=True, rtol=1e-10, gen_rtol=1e-12, ksp_factor=0.1,
         arms=("plain", "linear", "quadratic", "learned"), pc="ilu0", coarse_filter=1, filter_targets=False,
-        neval=None, smooth=False):
+        neval=None, smooth=False, residual_epochs=0, residual_lr=5e-4, residual_cosine=False):
     """Load-step comparison: steps 1..ntrain generate the training trajectory
     (FP64, linear-extrapolation guesses, solved to gen_rtol so that the
     differences the network learns are not dominated by solver tolerance),
@@ -380,7 +460,10 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     neval: evaluate only the first neval steps after ntrain; smooth: every
     candidate of the extrapolating arms (and of the held-out test) is
     smoothed once by the multigrid's finest-level smoother before the
-    residual comparison (pc "mg" only)."""
+    residual comparison (pc "mg" only); residual_epochs: after the mean
+    square training, this many epochs of residual_finetune (the linearised
+    residual of the candidate as the loss) at residual_lr, decaying on a
+    half cosine with residual_cosine."""
     if smooth and pc != "mg":
         raise ValueError("smooth needs pc='mg' (the multigrid's smoother)")
     from . import assm as am
@@ -428,8 +511,26 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
     # the acceptance metric on the held-out steps: residual norm of the
     # quadratic extrapolation with and without the learned correction
     keepm = (mask.reshape(-1) == 0)
+    ft = None
+    if residual_epochs:
+        # K at the last training state, in the context's own FP64 values
+        A.set_uhist(U[-1])
+        A.tangent_dev(None, None, _lib.IGAP_F64)
+        cqs, s2s, bcs = [], [], []
+        for i in range(len(samples)):
+            k = i + 3
+            dU1, dU2 = U[k - 1] - U[k - 2], U[k - 2] - U[k - 3]
+            cqs.append(2.0 * U[k - 1] - U[k - 2] + (dU1 - dU2) * keepm)
+            s2s.append(_rms(dU1 - dU2))
+            bcs.append(lambda kk=k: A.set_bc(tip_bc(tip_total, kk / nsteps)))
+        ft = residual_finetune(net, A, samples, cqs, s2s, keepm, bcs, epochs=residual_epochs, lr=residual_lr,
+                               seed=seed, cosine=residual_cosine, log=lambda ep, l, v: say(f"  residual epoch {ep}: |r|^2 {l:.3e} "
+                                                                  f"held-out ratio {v:.3g}"))
+        del cqs
+        say(f"  residual training: held-out ratio {ft['val_ratio_start']:.3g} -> {ft['val_ratio_best']:.3g}")
+    corr_on = bool(tr["correction_enabled"] or (ft is not None and ft["val_ratio_best"] < 1.0))
     ratios_val = []
-    if tr["correction_enabled"]:
+    if corr_on:
         for i in range(len(samples) - tr["val_samples"], len(samples)):
             k = i + 3
             A.set_bc(tip_bc(tip_total, k / nsteps))
@@ -445,7 +546,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
                 cl = smooth_candidate(A, cl, cl)
             rq = float(np.linalg.norm(A.residual(cq)))
             ratios_val.append(float(np.linalg.norm(A.residual(cl))) / rq if rq > 0 else 1.0)
-    use_corr = bool(tr["correction_enabled"] and ratios_val and np.median(ratios_val) < 1.0)
+    use_corr = bool(corr_on and ratios_val and np.median(ratios_val) < 1.0)
     say(f"  held-out residual ratio learned/quadratic: "
         f"{np.median(ratios_val) if ratios_val else float('nan'):.3g}; correction {'used' if use_corr else 'off'}")
     t_train = time.perf_counter() - t0
@@ -499,6 +600,7 @@ def run(nelem=(64, 16, 16), nsteps=200, ntrain=100, tip_total=0.5, epochs=60, lr
         "val_residual_ratio_learned_over_quadratic": float(np.median(ratios_val)) if ratios_val else None,
         "correction_used": use_corr, "coarse_filter_levels": int(coarse_filter), "filter_targets": bool(filter_targets),
         "smooth": bool(smooth),
+        "residual_training": None if ft is None else {k: v for k, v in ft.items() if k != "train_residual"},
         "t_generate_s": t_gen, "t_train_s": t_train,
         "generate": {"newton_its": int(sum(x["newton_its"] for x in gen))},
     }
@@ -578,7 +680,7 @@ def run_transient(nelem=(128, 32, 32), nsteps=200, ntrain=100, dt=1e-3, epochs=6
             hist = np.concatenate([U[k - 1], U[k - 2]])
             rq = float(np.linalg.norm(A.residual(np.concatenate([cq, hist]))))
             ratios_val.append(float(np.linalg.norm(A.residual(np.concatenate([cl, hist])))) / rq if rq > 0 else 1.0)
-    use_corr = bool(tr["correction_enabled"] and ratios_val and np.median(ratios_val) < 1.0)
+    use_corr = bool(corr_on and ratios_val and np.median(ratios_val) < 1.0)
     say(f"  held-out residual ratio learned/quadratic: "
         f"{np.median(ratios_val) if ratios_val else float('nan'):.3g}; correction {'used' if use_corr else 'off'}")
     t_train = time.perf_counter() - t0
@@ -663,6 +765,9 @@ def main():
     ap.add_argument("--filter-targets", type=int, default=0)
     ap.add_argument("--neval", type=int, default=0, help="evaluate only this many steps after training (0: all)")
     ap.add_argument("--smooth", type=int, default=0, help="smooth every candidate once before the residual test")
+    ap.add_argument("--residual-epochs", type=int, default=0, help="epochs of training on the candidate's residual")
+    ap.add_argument("--residual-lr", type=float, default=5e-4)
+    ap.add_argument("--residual-cosine", type=int, default=0, help="half-cosine decay of the residual training's lr")
     ap.add_argument("--transient", action="store_true", help="time steps of gradelasttime instead of load steps")
     ap.add_argument("--dt", type=float, default=1e-3)
     a = ap.parse_args()
@@ -675,7 +780,8 @@ def main():
         return
     s = run(nel, a.steps, a.train, a.tip, a.epochs, a.lr, a.prec, a.pc_order, rtol=a.rtol, gen_rtol=a.gen_rtol,
             ksp_factor=a.ksp_factor, arms=tuple(a.arms.split(",")), pc=a.pc, coarse_filter=a.coarse_filter,
-            filter_targets=bool(a.filter_targets), neval=a.neval or None, smooth=bool(a.smooth))
+            filter_targets=bool(a.filter_targets), neval=a.neval or None, smooth=bool(a.smooth),
+            residual_epochs=a.residual_epochs, residual_lr=a.residual_lr, residual_cosine=bool(a.residual_cosine))
     print(json.dumps(s), flush=True)
 
 

@@ -153,3 +153,21 @@ def test_warm_start_ablation_small_beam(ws, igap, solver):
     else:
         assert rl == rq and l["newton_its"] == q["newton_its"] and l["ksp_its"] == q["ksp_its"]
         assert l["final_rel_diff_vs_plain"] == q["final_rel_diff_vs_plain"]
+
+
+def test_residual_training_lowers_the_candidate_residual(ws):
+    """Training on the candidate's linearised residual (through
+    igap_gpu_tangent_apply) on the 16x4x4 ramp: some epoch scores a lower
+    held-out residual ratio learned / quadratic than the mean-square-trained
+    network it starts from (a gradient of the wrong sign raises it every
+    epoch), and every arm still reaches the plain answer to 1e-8."""
+    s = ws.run(nelem=(16, 4, 4), nsteps=60, ntrain=30, tip_total=0.6, epochs=40, lr=2e-3, verbose=False, pc="mg",
+               prec="mixed", coarse_filter=0, residual_epochs=20, residual_lr=2e-3, neval=10,
+               arms=("plain", "quadratic", "learned"))
+    rt = s["residual_training"]
+    print({k: rt[k] for k in ("val_ratio_start", "val_ratio_best", "alpha")}, "used:", s["correction_used"])
+    assert rt["val_ratio_best"] < rt["val_ratio_start"]
+    for arm in ("plain", "quadratic", "learned"):
+        print(arm, s[arm])
+        assert s[arm]["all_converged"]
+        assert s[arm]["final_rel_diff_vs_plain"] < 1e-8

@@ -69,3 +69,46 @@ def test_coarse_filter_projection():
     i, j, k = np.meshgrid(*[np.arange(n) for n in m.nb], indexing="ij")
     chk = np.repeat(((-1.0) ** (i + j))[..., None], 3, axis=3).reshape(-1)
     assert np.linalg.norm(f(chk)) < 0.4 * np.linalg.norm(chk)
+
+
+def test_residual_shifted_target_gives_the_residual_gradient():
+    """The network trained on the shifted target descends the linearised
+    residual of its candidate: the mean square gradient at the shifted target
+    (the network oracle, float64) times N / (2 alpha) equals the central
+    difference of |rq + K (km y(p))|^2 / 2 in the last layer's parameters, for a
+    symmetric K; the target shifted the other way (a planted sign error)
+    gives the opposite gradient and fails."""
+    from oracle import convnet as cn
+    rng = np.random.default_rng(7)
+    grid, cin, cout, dil = (4, 3, 3), 9, 3, (1, 2, 1, 1)
+    V = int(np.prod(grid))
+    n = V * cout
+    p = 0.2 * rng.standard_normal(cn.NL * cn.LAYER_P)
+    x = rng.standard_normal((V, cin))
+    B = rng.standard_normal((n, n))
+    K = B @ B.T / n + np.eye(n)
+    km = (rng.random(n) > 0.2).astype(np.float64)
+    rq = rng.standard_normal(n)
+    alpha = 0.37
+
+    def y_of(q):
+        return cn.forward(q, x, grid, cin, cout, dil, emulate=False)[0].reshape(-1)
+
+    def loss(q):
+        r = rq + K @ (km * y_of(q))
+        return 0.5 * float(r @ r)
+
+    y = y_of(p)
+    r, t = ws.residual_shifted_target(y, rq, lambda v: K @ v, km, alpha)
+    assert np.allclose(r, rq + K @ (km * y), rtol=0, atol=1e-12)
+    g = cn.gradients(p, x, t, grid, cin, cout, dil, emulate=False)[1] * n / (2 * alpha)
+    # a direction in the last (linear) layer: the loss is then a quadratic
+    # in the step, and its central difference is exact up to rounding
+    d = np.zeros(p.size)
+    d[(cn.NL - 1) * cn.LAYER_P:] = rng.standard_normal(cn.LAYER_P)
+    eps = 1e-4
+    fd = (loss(p + eps * d) - loss(p - eps * d)) / (2 * eps)
+    assert abs(g @ d - fd) <= 1e-6 * abs(fd), (g @ d, fd)
+    t_wrong = 2 * y - t
+    g_wrong = cn.gradients(p, x, t_wrong, grid, cin, cout, dil, emulate=False)[1] * n / (2 * alpha)
+    assert abs(g_wrong @ d - fd) > 0.5 * abs(fd)
