@@ -23,7 +23,10 @@ quadratic extrapolation).  The learned term is projected onto the
 next-coarser spline space (CoarseFilter: exact knot-insertion prolongation
 after interpolation at the coarse Greville abscissae), which keeps its smooth
 part and drops the rough error the stiffness would amplify.  Training keeps the epoch with the lowest loss on
-the last training steps (held out), and the learned term is used at all
+the last training steps (held out); optionally (residual_finetune) it then
+continues on the candidate's residual, linearised with the assembled tangent
+(igap_gpu_tangent_apply), instead of the distance to the next solution.  The
+learned term is used at all
 only if it lowers the residual norm of the quadratic extrapolation on those
 held-out steps.  The candidate is used only if its residual norm is below
 the residual norm of the linear extrapolation U_ext, which otherwise stays
