@@ -2132,16 +2132,19 @@ __global__ __launch_bounds__(128) void k_tangent_gather(MeshK M, const TK *__res
 // and adds them into an LDS accumulator indexed by the stencil column point,
 // with a barrier between elements.  Every entry gets its contributions in
 // ascending element order (bitwise repeatable); the loads are whole-row,
-// coalesced segments.
+// coalesced segments.  The row starts and the Dirichlet flags of the rows and
+// stencil columns are loaded into LDS before the element rounds, so that the
+// write-out does not wait on them.
 template <typename TK, typename TV>
 __global__ __launch_bounds__(256) void k_tangent_gather_rows(MeshK M, const TK *__restrict__ Ke, const int64_t *__restrict__ rowptr,
                                                              const int32_t *__restrict__ mask, TV *__restrict__ vals)
 {
     __shared__ double acc[125 * 9];
-    const int64_t a = blockIdx.x;
+    // control-point and dof indices fit in 32 bits (the CSR columns are int32)
+    const int a = (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int nb1 = M.nb[1], nb2 = M.nb[2];
-    const int az = (int)(a % nb2), ay = (int)((a / nb2) % nb1), ax = (int)(a / ((int64_t)nb1 * nb2));
+    const int az = a % nb2, ay = (a / nb2) % nb1, ax = a / (nb1 * nb2);
     int Sx[5], Sy[5], Sz[5];
     const int nsx = stencil1(ax, M.nb[0], M.periodic[0], Sx);
     const int nsy = stencil1(ay, M.nb[1], M.periodic[1], Sy);
@@ -2172,6 +2175,8 @@ __global__ __launch_bounds__(256) void k_tangent_gather_rows(MeshK M, const TK *
             for (int k = 0; k < nsz; k++) if (Sz[k] == bz) szv[x] = k;
         }
     }
+    __shared__ int64_t s_rp[3];
+    __shared__ int32_t s_mrow[3], s_mcol[125 * 3];
     double v[27];
 #pragma unroll
     for (int x = 0; x < 3; x++)
@@ -2187,6 +2192,20 @@ __global__ __launch_bounds__(256) void k_tangent_gather_rows(MeshK M, const TK *
                 }
                 v[9 * x + 3 * y + z] = t;
             }
+    // the output phase's row starts and Dirichlet flags, loaded now so that
+    // their latency overlaps the element rounds
+    for (int q = tid; q < 3 * nst + 6; q += 256) {
+        if (q < 3 * nst) {
+            const int t = q / 3, jj = q - 3 * t;
+            const int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
+            const int bcp = (nb1 * nb2) * Sx[sx] + nb2 * Sy[sy] + Sz[sz];
+            s_mcol[q] = mask[3 * bcp + jj];
+        } else if (q < 3 * nst + 3) {
+            s_rp[q - 3 * nst] = rowptr[3 * a + (q - 3 * nst)];
+        } else {
+            s_mrow[q - 3 * nst - 3] = mask[3 * a + (q - 3 * nst - 3)];
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < 3; x++)
@@ -2206,10 +2225,10 @@ __global__ __launch_bounds__(256) void k_tangent_gather_rows(MeshK M, const TK *
     for (int idx = tid; idx < nst * 9; idx += 256) {
         const int ii = idx / (3 * nst), r = idx - ii * 3 * nst, t = r / 3, jj = r - 3 * t;
         const int sx = t / (nsy * nsz), sy = (t / nsz) % nsy, sz = t % nsz;
-        const int64_t bcp = ((int64_t)M.nb[1] * M.nb[2]) * Sx[sx] + (int64_t)M.nb[2] * Sy[sy] + Sz[sz];
-        const int64_t row = 3 * a + ii, col = 3 * bcp + jj;
-        const double val = (mask[row] || mask[col]) ? (row == col ? 1.0 : 0.0) : acc[(3 * t + ii) * 3 + jj];
-        vals[rowptr[row] + r] = (TV)val;
+        const int bcp = (nb1 * nb2) * Sx[sx] + nb2 * Sy[sy] + Sz[sz];
+        const int row = 3 * a + ii, col = 3 * bcp + jj;
+        const double val = (s_mrow[ii] || s_mcol[3 * t + jj]) ? (row == col ? 1.0 : 0.0) : acc[(3 * t + ii) * 3 + jj];
+        vals[s_rp[ii] + r] = (TV)val;
     }
 }
 
